@@ -1,0 +1,187 @@
+"""Benchmark: Wilson-clover dslash on one MI355X at 32^3 x 64.
+
+One step = the even-odd clover operator on one checkerboard, i.e. the two
+hot kernels back to back, each over 1,048,576 sites:
+    t   = A_ee^-1 D_eo psi_o          clov_dslash (cb 0)
+    out = A_oo chi_o - beta D_oe t    clov_dslash_achimbdpsi (cb 1)
+FP32 storage and arithmetic, 12-real links, packed Hermitian clover.
+Inputs are synthetic (seeded random SU(3) links, Gaussian spinors, random
+clover blocks and their FP64-computed inverses) and resident in HBM before
+timing starts; two input/output sets alternate so the Infinity Cache cannot
+hold a step's working set.
+
+Prints one JSON line (metric, value = GFLOP/s of the step, roofline of the
+dominant kernel from HIP-event timing, CPU baseline from the FP64 oracle).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "dslash GFLOP/s + % HBM roofline @32^3x64; mixed vs FP64 CG time to 1e-10"
+DIMS = (32, 32, 32, 64)
+PEAK_HBM_GBS = 8000.0
+FLOP_CLOVER = 1824          # flop/site, clover dslash
+FLOP_CLOVER_ACHI = 1872     # flop/site, clover a*chi - b*D*psi (A chi + hop)
+BYTES_F32 = 4
+
+
+def algorithmic_bytes(recon, clover_reals, chi, sbytes):
+    """Compulsory bytes per output site: input spinor once, 8 links, the
+    clover blocks, chi, the output spinor."""
+    return (24 + 8 * recon + clover_reals + (24 if chi else 0) + 24) * sbytes
+
+
+def time_kernel(fn, n):
+    import torch
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn(0)
+    torch.cuda.synchronize()
+    s.record()
+    for i in range(n):
+        fn(i)
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / n * 1e-3
+
+
+def cpu_baseline(seconds=10.0):
+    """FP64 oracle (scalar C, one thread) on a bounded sample of the same
+    workload: the clover dslash at sampled sites of a 32^3x64 lattice."""
+    import numpy as np
+    import torch
+    from oracle import oracle as orc
+    import __graft_entry__ as ge
+    q = ge.load_package()
+    L = q.lattice
+    # a smaller lattice with identical per-site work keeps host memory small
+    dims = (16, 16, 16, 32)
+    U = L.random_gauge(dims, 0.3, 5).numpy()
+    psi = L.random_spinor(dims, 6).numpy()
+    clov = L.invert_clover(L.random_clover(dims, 7)).numpy()
+    V = U.shape[0]
+    sites = np.arange(0, V, 2, dtype=np.int64)
+    done, t0 = 0, time.perf_counter()
+    n = 2048
+    while time.perf_counter() - t0 < seconds:
+        chunk = sites[(done % len(sites)):(done % len(sites)) + n]
+        orc.dslash_sites(dims, chunk, psi, U, 1, kind=orc.CLOVER, clov=clov)
+        done += len(chunk)
+    dt = time.perf_counter() - t0
+    del torch
+    return {"value": round(done * FLOP_CLOVER / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"FP64 C oracle, clover dslash at {done} sites of a 16^3x32 lattice "
+                      f"(same per-site work as 32^3x64), {dt:.1f} s, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+    q = ge.load_package()
+    L = q.lattice
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    prec, recon = q.F32, 12
+    geom = q.Geometry(DIMS, t_boundary=-1.0)
+    vcb = geom.vcb
+
+    sets = []
+    for k in range(2):
+        U = L.random_gauge(DIMS, 0.3, 1000 + 10 * k, device=dev)
+        clov = L.random_clover(DIMS, 1003 + 10 * k, device=dev)
+        inv = L.invert_clover(clov)
+        psi = L.random_spinor(DIMS, 1001 + 10 * k, device=dev)
+        s = dict(g0=geom.pack_gauge(U, prec, recon, 0), g1=geom.pack_gauge(U, prec, recon, 1),
+                 inv0=geom.pack_clover(inv, prec, 0), clov1=geom.pack_clover(clov, prec, 1),
+                 psi1=geom.pack_spinor(psi, prec, 1), t0=geom.new_spinor(prec, cb=0),
+                 out1=geom.new_spinor(prec, cb=1))
+        del U, clov, inv, psi
+        sets.append(s)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+    op = q.ClovDslash(geom)
+    beta = 0.25
+
+    def step(i):
+        s = sets[i & 1]
+        op.dslash(s["t0"], s["psi1"], s["g0"], s["inv0"], 1, 0)
+        op.dslashAChiMinusBDPsi(s["out1"], s["t0"], s["psi1"], s["g1"], s["clov1"], beta, 1, 1)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t_step = time_kernel(step, args.steps)
+
+    # the dominant kernel alone (clover dslash), same stream, HIP events
+    def k_dslash(i):
+        s = sets[i & 1]
+        op.dslash(s["t0"], s["psi1"], s["g0"], s["inv0"], 1, 0)
+
+    def k_achi(i):
+        s = sets[i & 1]
+        op.dslashAChiMinusBDPsi(s["out1"], s["t0"], s["psi1"], s["g1"], s["clov1"], beta, 1, 1)
+
+    t_d = time_kernel(k_dslash, args.steps)
+    t_a = time_kernel(k_achi, args.steps)
+    b_d = algorithmic_bytes(recon, 72, False, BYTES_F32) * vcb
+    b_a = algorithmic_bytes(recon, 72, True, BYTES_F32) * vcb
+    flops = vcb * (FLOP_CLOVER + FLOP_CLOVER_ACHI)
+    gflops = flops / t_step / 1e9
+    achieved = b_d / t_d / 1e9
+
+    line = {
+        "metric": METRIC,
+        "value": round(gflops, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "even-odd Wilson-clover operator: clov_dslash + clov_dslash_achimbdpsi, "
+                        "FP32, 12-real links, 32^3x64 (1,048,576 sites/cb)",
+            "lattice": list(DIMS), "sites_per_cb": vcb, "recon": recon,
+            "flop_per_site": {"clov_dslash": FLOP_CLOVER, "clov_dslash_achimbdpsi": FLOP_CLOVER_ACHI},
+            "kernel_us": {"clov_dslash": round(t_d * 1e6, 2),
+                          "clov_dslash_achimbdpsi": round(t_a * 1e6, 2)},
+            "kernel_gflops": {"clov_dslash": round(vcb * FLOP_CLOVER / t_d / 1e9, 1),
+                              "clov_dslash_achimbdpsi": round(vcb * FLOP_CLOVER_ACHI / t_a / 1e9, 1)},
+            "kernel_hbm_frac": {"clov_dslash": round(b_d / t_d / 1e9 / PEAK_HBM_GBS, 4),
+                                "clov_dslash_achimbdpsi": round(b_a / t_a / 1e9 / PEAK_HBM_GBS, 4)},
+            "parallelism": "single GPU",
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "clov_dslash f32 recon12",
+                     "algorithmic_bytes_per_launch": b_d},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

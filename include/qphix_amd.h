@@ -1,0 +1,183 @@
+/*
+ * qphix_amd.h -- C ABI of the MI355X (gfx950) Wilson-type Dirac operators.
+ *
+ * The reference (qphix-codegen) exposes its operators as C++ function
+ * templates in namespace QPhiX, one call per SIMD block of VECLEN sites of
+ * one checkerboard, specialised on <FT, veclen, soalen, compress12, tbc...>
+ * (codegen/jinja/dslash_decl.h.j2:6-115, clov_dslash_decl.h.j2:6-113,
+ * tm_dslash_decl.h.j2:6-109, tm_clov_dslash_decl.h.j2:6-113).  A GPU cannot
+ * be driven per 16-site block, so every entry point below takes a whole
+ * checkerboard: the caller's OpenMP block loop becomes the HIP grid.  The
+ * template parameters become run-time arguments (precision, 12/18 link
+ * reconstruction, isign, checkerboard), the per-block scalars keep their
+ * names and meaning (coeff_s, coeff_t_f, coeff_t_b, alpha, mu, muinv,
+ * prec_mass_rho, tbc_phases).
+ *
+ * Conventions
+ *  - All field pointers are DEVICE pointers in the library's native layout
+ *    (site-fastest, per checkerboard; qphix_amd_field_bytes gives sizes).
+ *    The pack/unpack entry points convert from/to a lexicographic FP64
+ *    layout on the device.
+ *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
+ *    stream) and returns 0 on success or a negative QPHIX_AMD_E* code.
+ *    Nothing ever exits the process (the reference prints and exit(1)s on
+ *    a bad direction, codegen/jinja/dslash_spec.h.j2:28-31).
+ *  - The caller owns every buffer; kernels write only `out`.
+ *  - Calls on distinct contexts or distinct output buffers are thread-safe.
+ */
+#ifndef QPHIX_AMD_H
+#define QPHIX_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define QPHIX_AMD_OK 0
+#define QPHIX_AMD_EINVAL -1     /* bad argument (null, range, variant)     */
+#define QPHIX_AMD_EGEOM -2      /* unsupported lattice geometry            */
+#define QPHIX_AMD_EHIP -3       /* HIP runtime error                       */
+#define QPHIX_AMD_ENOMEM -4     /* device allocation failed                */
+#define QPHIX_AMD_ENOCONV -5    /* solver did not converge                 */
+#define QPHIX_AMD_EALIGN -6     /* pointer not 16-byte aligned             */
+
+/* ---- enums ------------------------------------------------------------ */
+/* storage precision; arithmetic is FP64 for F64 and FP32 otherwise
+   (codegen/data_types.h:7-39: FP16 storage with FP32 arithmetic)        */
+#define QPHIX_AMD_F64 0
+#define QPHIX_AMD_F32 1
+#define QPHIX_AMD_F16 2
+#define QPHIX_AMD_BF16 3
+
+/* field kinds for qphix_amd_field_bytes */
+#define QPHIX_AMD_FIELD_SPINOR 0      /* 24 reals / site                  */
+#define QPHIX_AMD_FIELD_GAUGE 1       /* 8 links x 12|18 reals / site     */
+#define QPHIX_AMD_FIELD_CLOVER 2      /* 2 x (6 + 15*2) reals / site      */
+#define QPHIX_AMD_FIELD_FULLCLOVER 3  /* 2 x 6 x 6 x 2 reals / site       */
+
+typedef struct qphix_amd_ctx qphix_amd_ctx;
+
+/* ---- context / geometry ----------------------------------------------- */
+/* Lattice Lx x Ly x Lz x Lt (Lx even; every extent >= 2).  t_boundary is
+   the factor (+1 periodic, -1 antiperiodic) applied to T hops that cross
+   the t = 0 / t = Lt-1 boundary; the reference takes it through the
+   per-block coeff_t_b / coeff_t_f (codegen/dslash.cc:79-86).            */
+int qphix_amd_ctx_create(qphix_amd_ctx **ctx, int Lx, int Ly, int Lz, int Lt,
+                         double t_boundary);
+int qphix_amd_ctx_destroy(qphix_amd_ctx *ctx);
+/* Twisted boundary conditions: tbc[mu] != 0 multiplies every forward hop in
+   dimension mu by (re,im) = tbc_phases[2mu], tbc_phases[2mu+1] and every
+   backward hop by its conjugate (codegen/dslash_common.cc:1491-1524; the
+   reference's template flags tbc_x..tbc_t and tbc_phases[4][2]).         */
+int qphix_amd_set_tbc(qphix_amd_ctx *ctx, const int tbc[4],
+                      const double tbc_phases[8]);
+/* Bytes of one checkerboard of a field in native layout.                 */
+size_t qphix_amd_field_bytes(const qphix_amd_ctx *ctx, int field, int prec,
+                             int recon);
+long qphix_amd_sites_cb(const qphix_amd_ctx *ctx);
+const char *qphix_amd_error_string(int code);
+
+/* ---- layout converters (device lex FP64 <-> device native) ------------
+   Lexicographic layouts, site = ((t*Lz+z)*Ly+y)*Lx+x:
+     spinor      [V][4 spin][3 colour][2]
+     gauge       [V][4 mu][3][3][2]   U_mu(x) row-major
+     clover      [V][2 block][6 diag + 15x2 lower-triangle]
+     full clover [V][2 block][6][6][2]
+   pack_gauge stores, per site x of checkerboard cb, the 8 hop matrices in
+   the reference's direction order d = 2*mu + (0 back | 1 forward):
+   U_mu(x - mu)^dagger and U_mu(x) (codegen/include/decl_common.h:13,
+   8 links per site); recon 12 keeps two rows and rebuilds the third in the
+   kernel (codegen/dslash_common.cc:1657-1676).                          */
+int qphix_amd_pack_spinor(qphix_amd_ctx *ctx, void *dst, int prec,
+                          const double *src_lex, int cb, void *stream);
+int qphix_amd_unpack_spinor(qphix_amd_ctx *ctx, double *dst_lex,
+                            const void *src, int prec, int cb, void *stream);
+int qphix_amd_pack_gauge(qphix_amd_ctx *ctx, void *dst, int prec, int recon,
+                         const double *u_lex, int cb, void *stream);
+int qphix_amd_pack_clover(qphix_amd_ctx *ctx, void *dst, int prec,
+                          const double *clov_lex, int cb, void *stream);
+int qphix_amd_pack_full_clover(qphix_amd_ctx *ctx, void *dst, int prec,
+                               const double *fclov_lex, int cb, void *stream);
+
+/* ---- operators ----------------------------------------------------------
+   out (checkerboard cb) from in (checkerboard 1-cb).  hop(x) =
+     sum_mu b_mu(x) (1 + s g_mu) U_mu(x-mu)^+ psi(x-mu)
+          + f_mu(x) (1 - s g_mu) U_mu(x) psi(x+mu),     s = isign = +1|-1
+   in the DeGrand-Rossi basis, with b = f = coeff_s for x,y,z and
+   coeff_t_b / coeff_t_f for t (times t_boundary across the boundary).   */
+
+/* replaces QPhiX::dslash_{plus,minus}_vec (codegen/jinja/dslash_decl.h.j2:6-43)
+   out = hop                                                              */
+int qphix_amd_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign,
+                     int cb, void *out, const void *in, const void *gauge,
+                     double coeff_s, double coeff_t_f, double coeff_t_b,
+                     void *stream);
+/* replaces QPhiX::dslash_achimbdpsi_{plus,minus}_vec
+   (codegen/jinja/dslash_decl.h.j2:45-89): out = alpha chi - hop          */
+int qphix_amd_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon,
+                                int isign, int cb, void *out, const void *in,
+                                const void *chi, const void *gauge,
+                                double alpha, double coeff_s,
+                                double coeff_t_f, double coeff_t_b,
+                                void *stream);
+/* replaces QPhiX::clov_dslash_{plus,minus}_vec
+   (codegen/jinja/clov_dslash_decl.h.j2:6-46): out = A^-1 hop, where the
+   caller passes the packed Hermitian A^-1 as `clover`                    */
+int qphix_amd_clov_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign,
+                          int cb, void *out, const void *in,
+                          const void *gauge, const void *clover,
+                          double coeff_s, double coeff_t_f, double coeff_t_b,
+                          void *stream);
+/* replaces QPhiX::clov_dslash_achimbdpsi_{plus,minus}_vec
+   (codegen/jinja/clov_dslash_decl.h.j2:48-94):
+   out = A chi + isign*i*prec_mass_rho*g5 chi - hop                        */
+int qphix_amd_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon,
+                                     int isign, int cb, void *out,
+                                     const void *in, const void *chi,
+                                     const void *gauge, const void *clover,
+                                     double coeff_s, double coeff_t_f,
+                                     double coeff_t_b, double prec_mass_rho,
+                                     void *stream);
+/* replaces QPhiX::tm_dslash_{plus,minus}_vec
+   (codegen/jinja/tm_dslash_decl.h.j2:6-45):
+   out = muinv (1 - isign*i*mu*g5) hop, mu = mu_phys/alpha,
+   muinv = alpha/(alpha^2 + mu_phys^2) (codegen/dslash_common.cc:1256-1266) */
+int qphix_amd_tm_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign,
+                        int cb, void *out, const void *in, const void *gauge,
+                        double coeff_s, double coeff_t_f, double coeff_t_b,
+                        double mu, double muinv, void *stream);
+/* replaces QPhiX::tm_dslash_achimbdpsi_{plus,minus}_vec
+   (codegen/jinja/tm_dslash_decl.h.j2:47-91):
+   out = alpha (1 + isign*i*mu*g5) chi - hop                              */
+int qphix_amd_tm_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon,
+                                   int isign, int cb, void *out,
+                                   const void *in, const void *chi,
+                                   const void *gauge, double alpha,
+                                   double coeff_s, double coeff_t_f,
+                                   double coeff_t_b, double mu, void *stream);
+/* replaces QPhiX::tm_clov_dslash_{plus,minus}_vec
+   (codegen/jinja/tm_clov_dslash_decl.h.j2:6-46): out = B hop, B = two full
+   complex 6x6 blocks passed as `fclover`                                 */
+int qphix_amd_tm_clov_dslash(qphix_amd_ctx *ctx, int prec, int recon,
+                             int isign, int cb, void *out, const void *in,
+                             const void *gauge, const void *fclover,
+                             double coeff_s, double coeff_t_f,
+                             double coeff_t_b, void *stream);
+/* replaces QPhiX::tm_clov_dslash_achimbdpsi_{plus,minus}_vec
+   (codegen/jinja/tm_clov_dslash_decl.h.j2:48-94):
+   out = B chi + isign*i*prec_mass_rho*g5 chi - hop                        */
+int qphix_amd_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
+                                        int recon, int isign, int cb,
+                                        void *out, const void *in,
+                                        const void *chi, const void *gauge,
+                                        const void *fclover, double coeff_s,
+                                        double coeff_t_f, double coeff_t_b,
+                                        double prec_mass_rho, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QPHIX_AMD_H */

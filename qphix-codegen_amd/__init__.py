@@ -1,0 +1,9 @@
+"""qphix-codegen_amd: MI355X-native (gfx950 HIP) Wilson-type Dirac operators
+with the semantics of the kernels qphix-codegen generates, behind a C ABI
+(include/qphix_amd.h).  This package is the Python host side: ctypes
+binding, operator classes mirroring QPhiX's, and synthetic lattices."""
+from ._lib import (BF16, F16, F32, F64, FIELD_CLOVER, FIELD_FULLCLOVER, FIELD_GAUGE,  # noqa: F401
+                   FIELD_SPINOR, LIB_PATH, QPhiXAMDError, lib)
+from .dslash import (PREC_NAME, TORCH_DTYPE, ClovDslash, Dslash, Field, Geometry,  # noqa: F401
+                     TMClovDslash, TMDslash)
+from . import lattice  # noqa: F401

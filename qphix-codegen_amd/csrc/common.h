@@ -1,0 +1,185 @@
+// common.h -- device-side building blocks shared by every kernel of the
+// library: storage traits, the native site-fastest field layout, complex
+// arithmetic and checkerboard site decoding.
+//
+// Native layout (one checkerboard, vcb sites): a field with N reals per
+// site is stored as vcb-long rows of W-element vectors,
+//     element k of site i  at  ((k / W) * vcb + i) * W + (k % W),
+// W = the widest of 16/8/4 bytes that divides the field's unit (a spinor,
+// one link, one clover block), so one load instruction of a 64-lane
+// wavefront reads 64 consecutive sites' chunk: 1 KiB at 16 B per lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "../../include/qphix_amd.h"
+
+namespace qamd {
+
+// operator families: plain Wilson, clover, twisted mass, twisted-mass clover
+enum { OP_WILSON = 0, OP_CLOVER = 1, OP_TM = 2, OP_TMCLOVER = 3 };
+
+// ---- storage traits ------------------------------------------------------
+template <typename S> struct Traits;
+template <> struct Traits<double> {
+  using R = double;
+  static constexpr int prec = QPHIX_AMD_F64;
+};
+template <> struct Traits<float> {
+  using R = float;
+  static constexpr int prec = QPHIX_AMD_F32;
+};
+template <> struct Traits<__half> {
+  using R = float;
+  static constexpr int prec = QPHIX_AMD_F16;
+};
+template <> struct Traits<__hip_bfloat16> {
+  using R = float;
+  static constexpr int prec = QPHIX_AMD_BF16;
+};
+
+__device__ __forceinline__ double to_real(double v) { return v; }
+__device__ __forceinline__ float to_real(float v) { return v; }
+__device__ __forceinline__ float to_real(__half v) { return __half2float(v); }
+__device__ __forceinline__ float to_real(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+template <typename S> __device__ __forceinline__ S from_real(double v);
+template <> __device__ __forceinline__ double from_real<double>(double v) { return v; }
+template <> __device__ __forceinline__ float from_real<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ __half from_real<__half>(double v) { return __float2half((float)v); }
+template <> __device__ __forceinline__ __hip_bfloat16 from_real<__hip_bfloat16>(double v) { return __float2bfloat16((float)v); }
+
+template <typename S> __device__ __forceinline__ S from_realf(float v);
+template <> __device__ __forceinline__ double from_realf<double>(float v) { return v; }
+template <> __device__ __forceinline__ float from_realf<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half from_realf<__half>(float v) { return __float2half(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 from_realf<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
+template <typename S> __device__ __forceinline__ S to_storage(double v) { return from_real<S>(v); }
+template <typename S> __device__ __forceinline__ S to_storage(float v) { return from_realf<S>(v); }
+
+// Vector width (elements) for a unit of N reals of storage type S.
+template <typename S> constexpr int pick_w(int N)
+{
+  return (N % (16 / (int)sizeof(S)) == 0) ? 16 / (int)sizeof(S)
+         : (N % (8 / (int)sizeof(S)) == 0) ? 8 / (int)sizeof(S)
+                                            : 4 / (int)sizeof(S);
+}
+
+template <int BYTES> struct RawVec;
+template <> struct RawVec<16> { using T = uint4; };
+template <> struct RawVec<8> { using T = uint2; };
+template <> struct RawVec<4> { using T = uint32_t; };
+
+// Load reals [K0, K0+N) of site `site` of a field whose unit width is W.
+// K0 and N are multiples of W.
+template <typename S, int W, int K0, int N, typename R>
+__device__ __forceinline__ void load_reals(const S *__restrict__ base, int vcb,
+                                           int site, R *__restrict__ out)
+{
+  static_assert(K0 % W == 0 && N % W == 0, "unit misaligned");
+  using V = typename RawVec<W * sizeof(S)>::T;
+  const V *vb = reinterpret_cast<const V *>(base);
+#pragma unroll
+  for (int c = 0; c < N / W; c++) {
+    V v = vb[(unsigned)((K0 / W + c) * vcb + site)];
+    S tmp[W];
+    __builtin_memcpy(tmp, &v, sizeof(V));
+#pragma unroll
+    for (int e = 0; e < W; e++) out[c * W + e] = (R)to_real(tmp[e]);
+  }
+}
+
+template <typename S, int W, int K0, int N, typename R>
+__device__ __forceinline__ void store_reals(S *__restrict__ base, int vcb,
+                                            int site, const R *__restrict__ in)
+{
+  static_assert(K0 % W == 0 && N % W == 0, "unit misaligned");
+  using V = typename RawVec<W * sizeof(S)>::T;
+  V *vb = reinterpret_cast<V *>(base);
+#pragma unroll
+  for (int c = 0; c < N / W; c++) {
+    S tmp[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) tmp[e] = to_storage<S>(in[c * W + e]);
+    V v;
+    __builtin_memcpy(&v, tmp, sizeof(V));
+    vb[(unsigned)((K0 / W + c) * vcb + site)] = v;
+  }
+}
+
+// ---- complex arithmetic ---------------------------------------------------
+template <typename R> struct cx {
+  R re, im;
+};
+template <typename R> __device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
+template <typename R> __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R> __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R> __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b)
+{
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R> __device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return {s * a.re, s * a.im}; }
+template <typename R> __device__ __forceinline__ cx<R> conj(cx<R> a) { return {a.re, -a.im}; }
+// i*a
+template <typename R> __device__ __forceinline__ cx<R> timesI(cx<R> a) { return {-a.im, a.re}; }
+// conj(a) * b
+template <typename R> __device__ __forceinline__ cx<R> cmul_conj(cx<R> a, cx<R> b)
+{
+  return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+// ---- checkerboard geometry ------------------------------------------------
+// cb index i = ((t*Lz + z)*Ly + y)*Lxh + xh,  x = 2 xh + ((cb+y+z+t) & 1).
+struct Geom {
+  int Lx, Ly, Lz, Lt, Lxh;
+  int vcb;         // sites per checkerboard
+  int sy, sz, st;  // cb-index strides of y, z, t
+};
+
+struct SiteCoord {
+  int x, y, z, t, xh;
+};
+
+// 32-bit decode (every supported lattice has vcb < 2^31 sites).
+__device__ __forceinline__ SiteCoord decode(const Geom &g, int i, int cb)
+{
+  SiteCoord c;
+  unsigned r = (unsigned)i;
+  unsigned q = r / (unsigned)g.Lxh;
+  c.xh = (int)(r - q * (unsigned)g.Lxh);
+  r = q;
+  q = r / (unsigned)g.Ly;
+  c.y = (int)(r - q * (unsigned)g.Ly);
+  r = q;
+  q = r / (unsigned)g.Lz;
+  c.z = (int)(r - q * (unsigned)g.Lz);
+  c.t = (int)q;
+  c.x = 2 * c.xh + ((cb + c.y + c.z + c.t) & 1);
+  return c;
+}
+
+// Lexicographic full-lattice index of a cb site.
+__device__ __forceinline__ long lex_index(const Geom &g, const SiteCoord &c)
+{
+  return (((long)c.t * g.Lz + c.z) * g.Ly + c.y) * g.Lx + c.x;
+}
+
+// XCD-aware work-group remap: the dispatcher hands work-group b to XCD
+// b % 8, so logical block (b % 8) * per_xcd + b / 8 gives each XCD one
+// contiguous slab of the lattice (neighbour re-reads then hit its own L2).
+// The grid is padded to a multiple of 8; padded blocks return.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_block()
+{
+  int b = (int)blockIdx.x;
+  int per = (int)gridDim.x / kNumXcd;
+  return (b % kNumXcd) * per + b / kNumXcd;
+}
+
+inline int xcd_grid(long nblocks) { return (int)((nblocks + kNumXcd - 1) / kNumXcd * kNumXcd); }
+
+}  // namespace qamd

@@ -1,0 +1,375 @@
+// dslash_kernel.h -- the Wilson-type dslash family as one templated HIP
+// kernel: one output site per lane, the 8 neighbour spinors gathered with
+// coalesced site-fastest loads, SU(3) links rebuilt from 12 or 18 reals in
+// registers, and the clover / twisted-mass / a*chi - b*D*psi epilogues
+// fused before the single store.
+//
+// The arithmetic follows the reference's generated body
+// (codegen/dslash_common.cc:1729-1932, direction loop codegen/dslash.cc:200-323):
+//   per direction d = 2*mu + fb: project the neighbour 4-spinor to a
+//   half-spinor with (1 + s*gamma_mu) (s = isign for backward hops, -isign for
+//   forward hops; codegen/dslash_common.cc:748-766), multiply by the hop
+//   matrix (U^dagger backward, U forward; codegen/dslash_common.cc:1678-1727),
+//   apply the twisted-BC phase (codegen/dslash_common.cc:1491-1524) and the
+//   direction coefficient, then reconstruct the lower two spins
+//   (codegen/dslash_common.cc:784-890).
+#pragma once
+
+#include "common.h"
+
+namespace qamd {
+
+template <typename R> struct DslashParams {
+  void *out;
+  const void *in;
+  const void *chi;
+  const void *gauge;
+  const void *clov;
+  Geom g;
+  int cb;
+  int nblocks;  // real (unpadded) work-groups
+  R coeff_s, coeff_tf, coeff_tb, tbound;
+  int tbc_mask;
+  R tbc_re[4], tbc_im[4];
+  R alpha, mu, muinv, rho;
+};
+
+constexpr int kDslashBlock = 256;
+
+// Half-spinor projection (1 + SGN*gamma_MU) psi -> (h0, h1), DeGrand-Rossi.
+template <int MU, int SGN, typename R>
+__device__ __forceinline__ void project(const R *psi, cx<R> h[2][3])
+{
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    cx<R> p0 = {psi[(0 * 3 + c) * 2], psi[(0 * 3 + c) * 2 + 1]};
+    cx<R> p1 = {psi[(1 * 3 + c) * 2], psi[(1 * 3 + c) * 2 + 1]};
+    cx<R> p2 = {psi[(2 * 3 + c) * 2], psi[(2 * 3 + c) * 2 + 1]};
+    cx<R> p3 = {psi[(3 * 3 + c) * 2], psi[(3 * 3 + c) * 2 + 1]};
+    if (MU == 0) {  // h0 = p0 + s i p3, h1 = p1 + s i p2
+      h[0][c] = SGN > 0 ? p0 + timesI(p3) : p0 - timesI(p3);
+      h[1][c] = SGN > 0 ? p1 + timesI(p2) : p1 - timesI(p2);
+    } else if (MU == 1) {  // h0 = p0 - s p3, h1 = p1 + s p2
+      h[0][c] = SGN > 0 ? p0 - p3 : p0 + p3;
+      h[1][c] = SGN > 0 ? p1 + p2 : p1 - p2;
+    } else if (MU == 2) {  // h0 = p0 + s i p2, h1 = p1 - s i p3
+      h[0][c] = SGN > 0 ? p0 + timesI(p2) : p0 - timesI(p2);
+      h[1][c] = SGN > 0 ? p1 - timesI(p3) : p1 + timesI(p3);
+    } else {  // h0 = p0 + s p2, h1 = p1 + s p3
+      h[0][c] = SGN > 0 ? p0 + p2 : p0 - p2;
+      h[1][c] = SGN > 0 ? p1 + p3 : p1 - p3;
+    }
+  }
+}
+
+// Accumulate the reconstructed 4-spinor of (1 + SGN*gamma_MU) from w = U h.
+template <int MU, int SGN, typename R>
+__device__ __forceinline__ void reconstruct(cx<R> acc[4][3], const cx<R> w[2][3])
+{
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    acc[0][c] = acc[0][c] + w[0][c];
+    acc[1][c] = acc[1][c] + w[1][c];
+    if (MU == 0) {  // r2 = -s i w1, r3 = -s i w0
+      acc[2][c] = SGN > 0 ? acc[2][c] - timesI(w[1][c]) : acc[2][c] + timesI(w[1][c]);
+      acc[3][c] = SGN > 0 ? acc[3][c] - timesI(w[0][c]) : acc[3][c] + timesI(w[0][c]);
+    } else if (MU == 1) {  // r2 = s w1, r3 = -s w0
+      acc[2][c] = SGN > 0 ? acc[2][c] + w[1][c] : acc[2][c] - w[1][c];
+      acc[3][c] = SGN > 0 ? acc[3][c] - w[0][c] : acc[3][c] + w[0][c];
+    } else if (MU == 2) {  // r2 = -s i w0, r3 = s i w1
+      acc[2][c] = SGN > 0 ? acc[2][c] - timesI(w[0][c]) : acc[2][c] + timesI(w[0][c]);
+      acc[3][c] = SGN > 0 ? acc[3][c] + timesI(w[1][c]) : acc[3][c] - timesI(w[1][c]);
+    } else {  // r2 = s w0, r3 = s w1
+      acc[2][c] = SGN > 0 ? acc[2][c] + w[0][c] : acc[2][c] - w[0][c];
+      acc[3][c] = SGN > 0 ? acc[3][c] + w[1][c] : acc[3][c] - w[1][c];
+    }
+  }
+}
+
+// Load hop matrix d of a site: rows 0,1 stored, row 2 stored (18) or
+// rebuilt as conj(row0 x row1) (12; codegen/dslash_common.cc:1664-1674).
+template <typename S, int RECON, int D, typename R>
+__device__ __forceinline__ void load_link(const S *gauge, int vcb, int site, cx<R> u[3][3])
+{
+  constexpr int WL = pick_w<S>(RECON);
+  R lk[RECON];
+  load_reals<S, WL, D * RECON, RECON>(gauge, vcb, site, lk);
+#pragma unroll
+  for (int r = 0; r < RECON / 6; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) u[r][c] = {lk[(r * 3 + c) * 2], lk[(r * 3 + c) * 2 + 1]};
+  if (RECON == 12) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      u[2][c] = conj(u[0][c1] * u[1][c2] - u[0][c2] * u[1][c1]);
+    }
+  }
+}
+
+template <typename S, int RECON, int MU, int FB, int ISIGN, typename R>
+__device__ __forceinline__ void hop_dir(cx<R> acc[4][3], const S *__restrict__ in,
+                                        const S *__restrict__ gauge, int vcb,
+                                        int nbr, int site, R beta, bool tbc,
+                                        cx<R> phase)
+{
+  constexpr int SGN = FB == 0 ? ISIGN : -ISIGN;
+  constexpr int WS = pick_w<S>(24);
+  R psi[24];
+  load_reals<S, WS, 0, 24>(in, vcb, nbr, psi);
+  cx<R> u[3][3];
+  load_link<S, RECON, 2 * MU + FB>(gauge, vcb, site, u);
+  cx<R> h[2][3];
+  project<MU, SGN>(psi, h);
+  cx<R> w[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) w[k][i] = u[i][0] * h[k][0] + u[i][1] * h[k][1] + u[i][2] * h[k][2];
+  if (tbc) {
+    cx<R> f = {beta * phase.re, beta * phase.im};
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) w[k][i] = f * w[k][i];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) w[k][i] = beta * w[k][i];
+  }
+  reconstruct<MU, SGN>(acc, w);
+}
+
+// out = A in, A = two packed Hermitian 6x6 blocks
+// (codegen/dslash_common.cc:1098-1186).
+template <typename S, typename R>
+__device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
+                                              const cx<R> in[4][3], cx<R> out[4][3])
+{
+  constexpr int WC = pick_w<S>(36);
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    R blk[36];
+    if (b == 0)
+      load_reals<S, WC, 0, 36>(clov, vcb, site, blk);
+    else
+      load_reals<S, WC, 36, 36>(clov, vcb, site, blk);
+    cx<R> v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      cx<R> res = blk[r] * v[r];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        if (c < r) {
+          const int od = r * (r - 1) / 2 + c;
+          cx<R> a = {blk[6 + 2 * od], blk[6 + 2 * od + 1]};
+          res = res + a * v[c];
+        } else if (c > r) {
+          const int od = c * (c - 1) / 2 + r;
+          cx<R> a = {blk[6 + 2 * od], blk[6 + 2 * od + 1]};
+          res = res + cmul_conj(a, v[c]);
+        }
+      }
+      out[2 * b + r / 3][r % 3] = res;
+    }
+  }
+}
+
+// out = B in, B = two full complex 6x6 blocks, row-major
+// (codegen/dslash_common.cc:1188-1232).
+template <typename S, typename R>
+__device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
+                                            const cx<R> in[4][3], cx<R> out[4][3])
+{
+  constexpr int WF = pick_w<S>(72);
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    R blk[72];
+    if (b == 0)
+      load_reals<S, WF, 0, 72>(clov, vcb, site, blk);
+    else
+      load_reals<S, WF, 72, 72>(clov, vcb, site, blk);
+    cx<R> v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      cx<R> res = {R(0), R(0)};
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        cx<R> a = {blk[(r * 6 + c) * 2], blk[(r * 6 + c) * 2 + 1]};
+        res = res + a * v[c];
+      }
+      out[2 * b + r / 3][r % 3] = res;
+    }
+  }
+}
+
+// (a + i*f*g5) applied to in, f real: g5 = +1 on spins 0,1, -1 on 2,3.
+template <typename R>
+__device__ __forceinline__ void twist(const cx<R> in[4][3], cx<R> out[4][3], R a, R f)
+{
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const R fs = s < 2 ? f : -f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      cx<R> v = in[s][c];
+      out[s][c] = {a * v.re - fs * v.im, a * v.im + fs * v.re};
+    }
+  }
+}
+
+template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
+__global__ __launch_bounds__(kDslashBlock) void dslash_kernel(DslashParams<typename Traits<S>::R> p)
+{
+  using R = typename Traits<S>::R;
+  const int blk = xcd_block();
+  if (blk >= p.nblocks) return;
+  const int idx = blk * kDslashBlock + (int)threadIdx.x;
+  const Geom &g = p.g;
+  if (idx >= g.vcb) return;
+  const SiteCoord c = decode(g, idx, p.cb);
+  const S *__restrict__ in = static_cast<const S *>(p.in);
+  const S *__restrict__ gauge = static_cast<const S *>(p.gauge);
+  const int vcb = g.vcb;
+
+  // neighbour cb indices (the input lives on the other checkerboard)
+  const int row = idx - c.xh;
+  const int xb = c.x == 0 ? g.Lx - 1 : c.x - 1;
+  const int xf = c.x == g.Lx - 1 ? 0 : c.x + 1;
+  const int n_xb = row + (xb >> 1), n_xf = row + (xf >> 1);
+  const int n_yb = c.y == 0 ? idx + (g.Ly - 1) * g.sy : idx - g.sy;
+  const int n_yf = c.y == g.Ly - 1 ? idx - (g.Ly - 1) * g.sy : idx + g.sy;
+  const int n_zb = c.z == 0 ? idx + (g.Lz - 1) * g.sz : idx - g.sz;
+  const int n_zf = c.z == g.Lz - 1 ? idx - (g.Lz - 1) * g.sz : idx + g.sz;
+  const int n_tb = c.t == 0 ? idx + (g.Lt - 1) * g.st : idx - g.st;
+  const int n_tf = c.t == g.Lt - 1 ? idx - (g.Lt - 1) * g.st : idx + g.st;
+  const R bs = p.coeff_s;
+  const R btb = c.t == 0 ? p.coeff_tb * p.tbound : p.coeff_tb;
+  const R btf = c.t == g.Lt - 1 ? p.coeff_tf * p.tbound : p.coeff_tf;
+
+  cx<R> acc[4][3];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) acc[s][k] = {R(0), R(0)};
+
+  const int tm = p.tbc_mask;
+  const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
+  const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
+  hop_dir<S, RECON, 0, 0, ISIGN>(acc, in, gauge, vcb, n_xb, idx, bs, tm & 1, conj(ph0));
+  hop_dir<S, RECON, 0, 1, ISIGN>(acc, in, gauge, vcb, n_xf, idx, bs, tm & 1, ph0);
+  hop_dir<S, RECON, 1, 0, ISIGN>(acc, in, gauge, vcb, n_yb, idx, bs, tm & 2, conj(ph1));
+  hop_dir<S, RECON, 1, 1, ISIGN>(acc, in, gauge, vcb, n_yf, idx, bs, tm & 2, ph1);
+  hop_dir<S, RECON, 2, 0, ISIGN>(acc, in, gauge, vcb, n_zb, idx, bs, tm & 4, conj(ph2));
+  hop_dir<S, RECON, 2, 1, ISIGN>(acc, in, gauge, vcb, n_zf, idx, bs, tm & 4, ph2);
+  hop_dir<S, RECON, 3, 0, ISIGN>(acc, in, gauge, vcb, n_tb, idx, btb, tm & 8, conj(ph3));
+  hop_dir<S, RECON, 3, 1, ISIGN>(acc, in, gauge, vcb, n_tf, idx, btf, tm & 8, ph3);
+
+  cx<R> res[4][3];
+  if (!ACHI) {
+    if (OP == OP_WILSON) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) res[s][k] = acc[s][k];
+    } else if (OP == OP_CLOVER) {
+      clover_packed<S>(static_cast<const S *>(p.clov), vcb, idx, acc, res);
+    } else if (OP == OP_TMCLOVER) {
+      clover_full<S>(static_cast<const S *>(p.clov), vcb, idx, acc, res);
+    } else {  // muinv (1 - isign i mu g5) (codegen/dslash_common.cc:1234-1359)
+      twist<R>(acc, res, p.muinv, -(R)ISIGN * p.mu * p.muinv);
+    }
+  } else {
+    R chr[24];
+    load_reals<S, pick_w<S>(24), 0, 24>(static_cast<const S *>(p.chi), vcb, idx, chr);
+    cx<R> chi[4][3];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) chi[s][k] = {chr[(s * 3 + k) * 2], chr[(s * 3 + k) * 2 + 1]};
+    if (OP == OP_WILSON) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) res[s][k] = p.alpha * chi[s][k];
+    } else if (OP == OP_TM) {  // alpha (1 + isign i mu g5) chi
+      twist<R>(chi, res, p.alpha, (R)ISIGN * p.mu * p.alpha);
+    } else {
+      if (OP == OP_CLOVER)
+        clover_packed<S>(static_cast<const S *>(p.clov), vcb, idx, chi, res);
+      else
+        clover_full<S>(static_cast<const S *>(p.clov), vcb, idx, chi, res);
+      // + isign i rho g5 chi (codegen/dslash_common.cc:1873-1906)
+      const R rho = (R)ISIGN * p.rho;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const R f = s < 2 ? rho : -rho;
+#pragma unroll
+        for (int k = 0; k < 3; k++) res[s][k] = res[s][k] + f * timesI(chi[s][k]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) res[s][k] = res[s][k] - acc[s][k];
+  }
+
+  R o[24];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      o[(s * 3 + k) * 2] = res[s][k].re;
+      o[(s * 3 + k) * 2 + 1] = res[s][k].im;
+    }
+  store_reals<S, pick_w<S>(24), 0, 24>(static_cast<S *>(p.out), vcb, idx, o);
+}
+
+// Host-side launch for one instantiation.
+template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
+hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream_t stream)
+{
+  const int grid = xcd_grid(p.nblocks);
+  hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN>), dim3(grid), dim3(kDslashBlock), 0,
+                     stream, p);
+  return hipGetLastError();
+}
+
+// Run-time dispatch over (recon, op, achi, isign) for one storage type.
+template <typename S>
+hipError_t dispatch_dslash(int recon, int op, bool achi, int isign,
+                           const DslashParams<typename Traits<S>::R> &p, hipStream_t st)
+{
+#define QAMD_ISIGN(RC, OPV, AC)                                              \
+  return isign > 0 ? launch_dslash<S, RC, OPV, AC, 1>(p, st)                 \
+                   : launch_dslash<S, RC, OPV, AC, -1>(p, st);
+#define QAMD_ACHI(RC, OPV)                                                   \
+  if (achi) {                                                                \
+    QAMD_ISIGN(RC, OPV, true)                                                \
+  } else {                                                                   \
+    QAMD_ISIGN(RC, OPV, false)                                               \
+  }
+#define QAMD_OP(RC)                                                          \
+  switch (op) {                                                              \
+    case OP_WILSON: QAMD_ACHI(RC, OP_WILSON)                                 \
+    case OP_CLOVER: QAMD_ACHI(RC, OP_CLOVER)                                 \
+    case OP_TM: QAMD_ACHI(RC, OP_TM)                                         \
+    case OP_TMCLOVER: QAMD_ACHI(RC, OP_TMCLOVER)                             \
+    default: return hipErrorInvalidValue;                                    \
+  }
+  if (recon == 12) {
+    QAMD_OP(12)
+  } else {
+    QAMD_OP(18)
+  }
+#undef QAMD_OP
+#undef QAMD_ACHI
+#undef QAMD_ISIGN
+}
+
+}  // namespace qamd

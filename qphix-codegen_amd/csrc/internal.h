@@ -1,0 +1,37 @@
+// internal.h -- host-side glue between the C ABI and the kernel templates.
+#pragma once
+
+#include "common.h"
+
+struct qphix_amd_ctx {
+  qamd::Geom g;
+  double tbound;
+  int tbc_mask;
+  double tbc_re[4], tbc_im[4];
+};
+
+namespace qamd {
+
+// One operator call, with every scalar still in double.
+struct DslashCall {
+  int prec, recon, op;
+  bool achi;
+  int isign, cb;
+  void *out;
+  const void *in, *chi, *gauge, *clov;
+  double coeff_s, coeff_tf, coeff_tb, alpha, mu, muinv, rho;
+};
+
+hipError_t run_dslash_f64(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
+hipError_t run_dslash_f32(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
+hipError_t run_dslash_f16(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
+hipError_t run_dslash_bf16(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
+
+hipError_t run_pack(const qphix_amd_ctx *ctx, int field, int prec, int recon, void *dst,
+                    const double *src, int cb, hipStream_t st);
+hipError_t run_unpack_spinor(const qphix_amd_ctx *ctx, int prec, double *dst, const void *src,
+                             int cb, hipStream_t st);
+
+inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace qamd

@@ -1,0 +1,256 @@
+"""Host-side operator classes over the C ABI, mirroring QPhiX's caller-side
+interface to the reference's generated kernels.
+
+QPhiX's Dslash / ClovDslash / TMDslash / TMClovDslash classes drive the
+generated bodies (codegen/jinja/*_decl.h.j2) block by block; here each
+method is one whole-checkerboard call into libqphix_amd.so.  Method names,
+argument order and meaning follow those classes:
+
+  Dslash.dslash(res, psi, u, isign, cb)                      res = D psi
+  Dslash.dslashAChiMinusBDPsi(res, psi, chi, u, alpha, beta, isign, cb)
+                                                   res = alpha chi - beta D psi
+  ClovDslash.dslash(res, psi, u, invclov, isign, cb)         res = A^-1 D psi
+  ClovDslash.dslashAChiMinusBDPsi(res, psi, chi, u, clov, beta, isign, cb)
+                                        res = A chi (+ i rho g5 chi) - beta D psi
+  TMDslash / TMClovDslash likewise, with the twisted mass.
+
+The hopping term D carries the anisotropy coefficients (coeff_s for x, y, z,
+coeff_t for t) and the geometry's t_boundary on hops across the time
+boundary (codegen/dslash.cc:79-86).  Fields are device tensors in the
+library's native layout (``Geometry.pack_*``).  Errors raise QPhiXAMDError.
+"""
+import ctypes
+
+import torch
+
+from ._lib import (BF16, F16, F32, F64, FIELD_CLOVER, FIELD_FULLCLOVER, FIELD_GAUGE,
+                   FIELD_SPINOR, QPhiXAMDError, check, lib)
+
+TORCH_DTYPE = {F64: torch.float64, F32: torch.float32, F16: torch.float16, BF16: torch.bfloat16}
+PREC_NAME = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16"}
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Field:
+    """A native-layout field on one checkerboard: a flat device tensor plus
+    its storage precision (and link reconstruction for gauge fields)."""
+
+    def __init__(self, data, prec, kind, recon=0, cb=None):
+        self.data = data
+        self.prec = prec
+        self.kind = kind
+        self.recon = recon
+        self.cb = cb
+
+    def ptr(self):
+        return ctypes.c_void_p(self.data.data_ptr())
+
+
+class Geometry:
+    """A lattice Lx x Ly x Lz x Lt (all even) with its boundary conditions."""
+
+    def __init__(self, dims, t_boundary=-1.0):
+        self.dims = tuple(int(d) for d in dims)
+        self.t_boundary = float(t_boundary)
+        ctx = ctypes.c_void_p()
+        check(lib().qphix_amd_ctx_create(ctypes.byref(ctx), *self.dims, self.t_boundary),
+              "qphix_amd_ctx_create")
+        self._ctx = ctx
+        self.vcb = int(lib().qphix_amd_sites_cb(ctx))
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().qphix_amd_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tbc(self, tbc, phases):
+        """tbc: 4 flags; phases: 4 complex numbers (forward-hop phase per dim)."""
+        t = (ctypes.c_int * 4)(*[int(bool(b)) for b in tbc])
+        p = (ctypes.c_double * 8)()
+        for m, z in enumerate(phases):
+            p[2 * m], p[2 * m + 1] = complex(z).real, complex(z).imag
+        check(lib().qphix_amd_set_tbc(self._ctx, t, p), "qphix_amd_set_tbc")
+
+    # ---- allocation ---------------------------------------------------
+    def field_bytes(self, kind, prec, recon=18):
+        return int(lib().qphix_amd_field_bytes(self._ctx, kind, prec, recon))
+
+    def new_field(self, kind, prec, recon=18, device="cuda", cb=None):
+        nbytes = self.field_bytes(kind, prec, recon)
+        if nbytes == 0:
+            raise QPhiXAMDError("invalid field request")
+        dt = TORCH_DTYPE[prec]
+        n = nbytes // torch.tensor([], dtype=dt).element_size()
+        return Field(torch.empty(n, dtype=dt, device=device), prec, kind, recon, cb)
+
+    def new_spinor(self, prec, device="cuda", cb=None):
+        return self.new_field(FIELD_SPINOR, prec, device=device, cb=cb)
+
+    # ---- converters (device lex FP64 <-> native) ------------------------
+    @staticmethod
+    def _lex(t):
+        if t.dtype != torch.float64 or not t.is_cuda:
+            raise QPhiXAMDError("lexicographic fields must be float64 device tensors")
+        return t.contiguous()
+
+    def pack_spinor(self, lex, prec, cb):
+        lex = self._lex(lex)
+        f = self.new_spinor(prec, device=lex.device, cb=cb)
+        check(lib().qphix_amd_pack_spinor(self._ctx, f.ptr(), prec, _ptr(lex), cb, _stream()),
+              "qphix_amd_pack_spinor")
+        return f
+
+    def unpack_spinor(self, f, out=None):
+        """Native spinor -> lex FP64 (only the field's checkerboard is written)."""
+        V = self.vcb * 2
+        if out is None:
+            out = torch.zeros(V, 4, 3, 2, dtype=torch.float64, device=f.data.device)
+        check(lib().qphix_amd_unpack_spinor(self._ctx, _ptr(out), f.ptr(), f.prec, f.cb, _stream()),
+              "qphix_amd_unpack_spinor")
+        return out
+
+    def pack_gauge(self, u_lex, prec, recon, cb):
+        u_lex = self._lex(u_lex)
+        f = self.new_field(FIELD_GAUGE, prec, recon, device=u_lex.device, cb=cb)
+        check(lib().qphix_amd_pack_gauge(self._ctx, f.ptr(), prec, recon, _ptr(u_lex), cb, _stream()),
+              "qphix_amd_pack_gauge")
+        return f
+
+    def pack_clover(self, clov_lex, prec, cb):
+        clov_lex = self._lex(clov_lex)
+        f = self.new_field(FIELD_CLOVER, prec, device=clov_lex.device, cb=cb)
+        check(lib().qphix_amd_pack_clover(self._ctx, f.ptr(), prec, _ptr(clov_lex), cb, _stream()),
+              "qphix_amd_pack_clover")
+        return f
+
+    def pack_full_clover(self, fclov_lex, prec, cb):
+        fclov_lex = self._lex(fclov_lex)
+        f = self.new_field(FIELD_FULLCLOVER, prec, device=fclov_lex.device, cb=cb)
+        check(lib().qphix_amd_pack_full_clover(self._ctx, f.ptr(), prec, _ptr(fclov_lex), cb, _stream()),
+              "qphix_amd_pack_full_clover")
+        return f
+
+
+def _check_args(res, psi, u, isign, cb):
+    if isign not in (1, -1) or cb not in (0, 1):
+        raise QPhiXAMDError("isign must be +-1 and cb 0 or 1")
+    if not (res.prec == psi.prec == u.prec):
+        raise QPhiXAMDError("res, psi and u must share a storage precision")
+    if u.kind != FIELD_GAUGE:
+        raise QPhiXAMDError("u must be a packed gauge field")
+    if u.cb is not None and u.cb != cb:
+        raise QPhiXAMDError(f"gauge packed for cb {u.cb}, operator applied on cb {cb}")
+    res.cb = cb
+
+
+class Dslash:
+    """Wilson hopping term (QPhiX Dslash)."""
+
+    def __init__(self, geom, aniso_coeff_s=1.0, aniso_coeff_t=1.0):
+        self.geom = geom
+        self.aniso_s = float(aniso_coeff_s)
+        self.aniso_t = float(aniso_coeff_t)
+
+    def dslash(self, res, psi, u, isign, cb):
+        _check_args(res, psi, u, isign, cb)
+        check(lib().qphix_amd_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(), psi.ptr(),
+                                     u.ptr(), self.aniso_s, self.aniso_t, self.aniso_t, _stream()),
+              "qphix_amd_dslash")
+
+    def dslashAChiMinusBDPsi(self, res, psi, chi, u, alpha, beta, isign, cb):
+        _check_args(res, psi, u, isign, cb)
+        b = float(beta)
+        check(lib().qphix_amd_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
+                                                psi.ptr(), chi.ptr(), u.ptr(), float(alpha),
+                                                b * self.aniso_s, b * self.aniso_t, b * self.aniso_t,
+                                                _stream()),
+              "qphix_amd_dslash_achimbdpsi")
+
+
+class ClovDslash(Dslash):
+    """Wilson-clover (QPhiX ClovDslash): A^-1 applied after the hop."""
+
+    def dslash(self, res, psi, u, invclov, isign, cb):
+        _check_args(res, psi, u, isign, cb)
+        check(lib().qphix_amd_clov_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
+                                          psi.ptr(), u.ptr(), invclov.ptr(), self.aniso_s,
+                                          self.aniso_t, self.aniso_t, _stream()),
+              "qphix_amd_clov_dslash")
+
+    def dslashAChiMinusBDPsi(self, res, psi, chi, u, clov, beta, isign, cb, prec_mass_rho=0.0):
+        _check_args(res, psi, u, isign, cb)
+        b = float(beta)
+        check(lib().qphix_amd_clov_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb,
+                                                     res.ptr(), psi.ptr(), chi.ptr(), u.ptr(),
+                                                     clov.ptr(), b * self.aniso_s, b * self.aniso_t,
+                                                     b * self.aniso_t, float(prec_mass_rho),
+                                                     _stream()),
+              "qphix_amd_clov_dslash_achimbdpsi")
+
+
+class TMDslash(Dslash):
+    """Twisted-mass Wilson (QPhiX TMDslash).  alpha = 4 + mass; the kernels
+    take mu = twisted_mass/alpha and muinv = alpha/(alpha^2 + twisted_mass^2)
+    (codegen/dslash_common.cc:1256-1266)."""
+
+    def __init__(self, geom, mass, twisted_mass, aniso_coeff_s=1.0, aniso_coeff_t=1.0):
+        super().__init__(geom, aniso_coeff_s, aniso_coeff_t)
+        self.alpha = 4.0 + float(mass)
+        self.mu_phys = float(twisted_mass)
+        self.mu = self.mu_phys / self.alpha
+        self.muinv = self.alpha / (self.alpha ** 2 + self.mu_phys ** 2)
+
+    def dslash(self, res, psi, u, isign, cb):
+        _check_args(res, psi, u, isign, cb)
+        check(lib().qphix_amd_tm_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
+                                        psi.ptr(), u.ptr(), self.aniso_s, self.aniso_t, self.aniso_t,
+                                        self.mu, self.muinv, _stream()),
+              "qphix_amd_tm_dslash")
+
+    def dslashAChiMinusBDPsi(self, res, psi, chi, u, alpha, beta, isign, cb):
+        _check_args(res, psi, u, isign, cb)
+        b = float(beta)
+        check(lib().qphix_amd_tm_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb,
+                                                   res.ptr(), psi.ptr(), chi.ptr(), u.ptr(),
+                                                   float(alpha), b * self.aniso_s, b * self.aniso_t,
+                                                   b * self.aniso_t, self.mu_phys / float(alpha),
+                                                   _stream()),
+              "qphix_amd_tm_dslash_achimbdpsi")
+
+
+class TMClovDslash(Dslash):
+    """Twisted-mass clover (QPhiX TMClovDslash): full complex 6x6 blocks."""
+
+    def dslash(self, res, psi, u, invclov, isign, cb):
+        _check_args(res, psi, u, isign, cb)
+        check(lib().qphix_amd_tm_clov_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
+                                             psi.ptr(), u.ptr(), invclov.ptr(), self.aniso_s,
+                                             self.aniso_t, self.aniso_t, _stream()),
+              "qphix_amd_tm_clov_dslash")
+
+    def dslashAChiMinusBDPsi(self, res, psi, chi, u, clov, beta, isign, cb, prec_mass_rho=0.0):
+        _check_args(res, psi, u, isign, cb)
+        b = float(beta)
+        check(lib().qphix_amd_tm_clov_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb,
+                                                        res.ptr(), psi.ptr(), chi.ptr(), u.ptr(),
+                                                        clov.ptr(), b * self.aniso_s,
+                                                        b * self.aniso_t, b * self.aniso_t,
+                                                        float(prec_mass_rho), _stream()),
+              "qphix_amd_tm_clov_dslash_achimbdpsi")

@@ -1,0 +1,58 @@
+"""Shared helpers for the tests: DeGrand-Rossi gamma matrices, checkerboard
+masks and complex views of the lexicographic FP64 arrays."""
+import numpy as np
+
+# DeGrand-Rossi basis (the basis the reference's projector tables encode).
+GAMMA = np.zeros((4, 4, 4), dtype=complex)
+GAMMA[0][0, 3], GAMMA[0][1, 2], GAMMA[0][2, 1], GAMMA[0][3, 0] = 1j, 1j, -1j, -1j
+GAMMA[1][0, 3], GAMMA[1][1, 2], GAMMA[1][2, 1], GAMMA[1][3, 0] = -1, 1, 1, -1
+GAMMA[2][0, 2], GAMMA[2][1, 3], GAMMA[2][2, 0], GAMMA[2][3, 1] = 1j, -1j, -1j, 1j
+GAMMA[3][0, 2], GAMMA[3][1, 3], GAMMA[3][2, 0], GAMMA[3][3, 1] = 1, 1, 1, 1
+GAMMA5 = GAMMA[0] @ GAMMA[1] @ GAMMA[2] @ GAMMA[3]
+
+
+def cplx(a):
+    """(..., 2) float -> (...) complex."""
+    a = np.asarray(a)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def real(z):
+    return np.stack([z.real, z.imag], axis=-1)
+
+
+def coords(dims):
+    Lx, Ly, Lz, Lt = dims
+    t, z, y, x = np.meshgrid(np.arange(Lt), np.arange(Lz), np.arange(Ly), np.arange(Lx), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel(), t.ravel()], axis=1)
+
+
+def parity(dims):
+    return coords(dims).sum(axis=1) & 1
+
+
+def cb_mask(dims, cb):
+    return parity(dims) == cb
+
+
+def g5(psi):
+    """gamma_5 on a (V,4,3,2) real array."""
+    out = psi.copy()
+    out[:, 2:] *= -1
+    return out
+
+
+def inner(a, b):
+    """<a, b> = sum conj(a) b over (V,4,3,2) real arrays."""
+    return np.sum(np.conj(cplx(a)) * cplx(b))
+
+
+def on_cb(a, dims, cb):
+    out = np.zeros_like(a)
+    m = cb_mask(dims, cb)
+    out[m] = a[m]
+    return out
+
+
+def rel_err(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))

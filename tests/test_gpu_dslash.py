@@ -1,0 +1,250 @@
+"""GPU parity of every dslash family against the FP64 oracle.
+
+8^4 (2048 sites/cb), every family x {dslash, achimbdpsi} x isign +-1 x cb 0/1
+x 12/18 links, in FP64 (tolerance 1e-13 relative) and FP32 (1e-6 relative);
+FP16 and BF16 storage with FP32 arithmetic at tolerances set by the storage
+rounding (FP16 unit roundoff 2^-11, BF16 2^-8).  Relative error = max-abs
+difference over all checkerboard sites / max-abs oracle value.
+
+32^3x64 (the benchmark lattice): the oracle at 4096 sampled sites, 12- vs
+18-link agreement, and D_+^dag = D_- via FP64 inner products.
+"""
+import numpy as np
+import pytest
+import torch
+
+from _util import cb_mask, inner, on_cb, parity
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+TOL = {0: 1e-13, 1: 1e-6, 2: 5e-3, 3: 4e-2}
+ALPHA = 1.0 / (2 * 0.1350)   # 4 + m at kappa = 0.1350
+MU = 0.1                      # twisted mass
+BETA = 0.25                   # b in a chi - b D psi
+RHO = 0.05                    # prec_mass_rho
+
+
+def make_lattice(qamd, dims, seed, device="cpu"):
+    L = qamd.lattice
+    U = L.random_gauge(dims, 0.3, seed, device)
+    psi = L.random_spinor(dims, seed + 1, device)
+    chi = L.random_spinor(dims, seed + 2, device)
+    clov = L.random_clover(dims, seed + 3, device)
+    lat = dict(U=U, psi=psi, chi=chi, clov=clov, inv=L.invert_clover(clov))
+    for s in (1, -1):
+        lat[f"B{s}"] = L.tmclover_blocks(clov, MU, s, True)   # (A + i s mu g5)^-1
+        lat[f"F{s}"] = L.tmclover_blocks(clov, MU, s, False)  # A + i s mu g5
+    return lat
+
+
+@pytest.fixture(scope="module")
+def lat8(qamd):
+    return make_lattice(qamd, (8, 8, 8, 8), 1000)
+
+
+def epilogue_field(kind, achi, isign, lat):
+    """The clover-like field the kernel takes for this family."""
+    if kind == 1:
+        return lat["clov"] if achi else lat["inv"]
+    if kind == 3:
+        return lat[f"F{isign}"] if achi else lat[f"B{isign}"]
+    return None
+
+
+def run_gpu(qamd, geom, kind, achi, prec, recon, isign, cb, lat):
+    dev = DEV
+    g = geom.pack_gauge(lat["U"].to(dev), prec, recon, cb)
+    psi = geom.pack_spinor(lat["psi"].to(dev), prec, 1 - cb)
+    chi = geom.pack_spinor(lat["chi"].to(dev), prec, cb)
+    res = geom.new_spinor(prec, cb=cb)
+    ep = epilogue_field(kind, achi, isign, lat)
+    if kind == 0:
+        op = qamd.Dslash(geom)
+        if achi:
+            op.dslashAChiMinusBDPsi(res, psi, chi, g, ALPHA, BETA, isign, cb)
+        else:
+            op.dslash(res, psi, g, isign, cb)
+    elif kind == 2:
+        op = qamd.TMDslash(geom, ALPHA - 4.0, MU)
+        if achi:
+            op.dslashAChiMinusBDPsi(res, psi, chi, g, ALPHA, BETA, isign, cb)
+        else:
+            op.dslash(res, psi, g, isign, cb)
+    else:
+        full = kind == 3
+        cl = (geom.pack_full_clover if full else geom.pack_clover)(ep.to(dev), prec, cb)
+        op = (qamd.TMClovDslash if full else qamd.ClovDslash)(geom)
+        if achi:
+            op.dslashAChiMinusBDPsi(res, psi, chi, g, cl, BETA, isign, cb, RHO)
+        else:
+            op.dslash(res, psi, g, cl, isign, cb)
+    torch.cuda.synchronize()
+    return geom.unpack_spinor(res).cpu().numpy()
+
+
+def oracle_args(kind, achi, isign, lat, dims):
+    coeff = BETA if achi else 1.0
+    kw = dict(kind=kind, achi=achi, coeff_s=coeff, coeff_t_f=coeff, coeff_t_b=coeff,
+              t_boundary=-1.0, alpha=ALPHA, rho=RHO)
+    if achi:
+        kw["chi"] = lat["chi"].numpy()
+    ep = epilogue_field(kind, achi, isign, lat)
+    if ep is not None:
+        kw["clov"] = ep.numpy()
+    if kind == 2:
+        kw["mu"] = MU / ALPHA
+        kw["muinv"] = ALPHA / (ALPHA ** 2 + MU ** 2)
+    return kw
+
+
+def run_oracle(orc, dims, kind, achi, isign, cb, lat):
+    kw = oracle_args(kind, achi, isign, lat, dims)
+    return orc.dslash(dims, lat["psi"].numpy(), lat["U"].numpy(), isign, cb, **kw)
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["f64", "f32"])
+@pytest.mark.parametrize("achi", [False, True], ids=["dslash", "achimbdpsi"])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["wilson", "clover", "tm", "tmclover"])
+def test_parity_8x4(qamd, orc, lat8, kind, achi, prec):
+    dims = (8, 8, 8, 8)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    for recon in (12, 18):
+        for isign in (1, -1):
+            for cb in (0, 1):
+                print(f"case kind={kind} achi={achi} prec={prec} recon={recon} isign={isign} cb={cb}",
+                      flush=True)
+                out = run_gpu(qamd, geom, kind, achi, prec, recon, isign, cb, lat8)
+                ref = run_oracle(orc, dims, kind, achi, isign, cb, lat8)
+                m = cb_mask(dims, cb)
+                err = np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max()
+                assert err < TOL[prec], (recon, isign, cb, err)
+                assert np.all(out[~m] == 0)
+
+
+@pytest.mark.parametrize("prec", [2, 3], ids=["f16", "bf16"])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["wilson", "clover", "tm", "tmclover"])
+def test_parity_low_precision_storage(qamd, orc, lat8, kind, prec):
+    dims = (8, 8, 8, 8)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    for achi in (False, True):
+        for isign in (1, -1):
+            cb = 1
+            out = run_gpu(qamd, geom, kind, achi, prec, 12, isign, cb, lat8)
+            ref = run_oracle(orc, dims, kind, achi, isign, cb, lat8)
+            m = cb_mask(dims, cb)
+            err = np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max()
+            assert err < TOL[prec], (achi, isign, err)
+
+
+def test_tbc_anisotropy_periodic(qamd, orc, lat8):
+    """Twisted BC phases in every dimension, anisotropic coefficients and a
+    periodic time boundary (FP64)."""
+    dims = (8, 8, 8, 8)
+    geom = qamd.Geometry(dims, t_boundary=1.0)
+    phases = [np.exp(1j * t) for t in (0.3, -0.7, 1.1, 2.0)]
+    geom.set_tbc([1, 1, 0, 1], phases)
+    U = lat8["U"].to(DEV)
+    for isign in (1, -1):
+        cb = 0
+        g = geom.pack_gauge(U, 0, 18, cb)
+        psi = geom.pack_spinor(lat8["psi"].to(DEV), 0, 1)
+        res = geom.new_spinor(0, cb=cb)
+        op = qamd.Dslash(geom, aniso_coeff_s=0.8, aniso_coeff_t=1.3)
+        op.dslash(res, psi, g, isign, cb)
+        out = geom.unpack_spinor(res).cpu().numpy()
+        ref = orc.dslash(dims, lat8["psi"].numpy(), lat8["U"].numpy(), isign, cb, coeff_s=0.8,
+                         coeff_t_f=1.3, coeff_t_b=1.3, t_boundary=1.0, tbc=[1, 1, 0, 1],
+                         tbc_phases=phases)
+        m = cb_mask(dims, cb)
+        assert np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max() < 1e-13
+
+
+def test_small_lattice_4x4(qamd, orc):
+    """Smallest extents (2 sites in x per checkerboard row), FP64 clover."""
+    dims = (4, 2, 4, 2)
+    lat = make_lattice(qamd, dims, 77)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            out = run_gpu(qamd, geom, 1, True, 0, 12, isign, cb, lat)
+            ref = run_oracle(orc, dims, 1, True, isign, cb, lat)
+            m = cb_mask(dims, cb)
+            assert np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max() < 1e-13
+
+
+def test_bad_arguments_raise(qamd):
+    geom = qamd.Geometry((8, 8, 8, 8))
+    with pytest.raises(qamd.QPhiXAMDError):
+        qamd.Geometry((8, 8, 8, 7))
+    a = geom.new_spinor(1, cb=0)
+    b = geom.new_spinor(1, cb=1)
+    g = geom.new_field(qamd.FIELD_GAUGE, 1, 12, cb=0)
+    with pytest.raises(qamd.QPhiXAMDError):
+        qamd.Dslash(geom).dslash(a, b, g, 2, 0)
+    with pytest.raises(qamd.QPhiXAMDError):
+        qamd.Dslash(geom).dslash(a, a, g, 1, 0)   # out aliases in
+
+
+# ---------------------------------------------------------------------------
+# the benchmark lattice, 32^3 x 64
+@pytest.fixture(scope="module")
+def lat32(qamd):
+    return make_lattice(qamd, (32, 32, 32, 64), 1100, device=DEV)
+
+
+def test_parity_32x64_sampled_sites(qamd, orc, lat32):
+    dims = (32, 32, 32, 64)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    rng = np.random.default_rng(3)
+    cpu = {k: v.cpu() for k, v in lat32.items()}
+    par = parity(dims)
+    for kind, achi in ((1, False), (1, True), (3, False), (2, True), (0, False)):
+        for isign in (1, -1):
+            cb = 0 if isign > 0 else 1
+            print(f"case 32^3x64 kind={kind} achi={achi} isign={isign}", flush=True)
+            out = run_gpu(qamd, geom, kind, achi, 1, 12, isign, cb, lat32)
+            sites = rng.choice(np.nonzero(par == cb)[0], 4096, replace=False)
+            kw = oracle_args(kind, achi, isign, cpu, dims)
+            ref = orc.dslash_sites(dims, sites, cpu["psi"].numpy(), cpu["U"].numpy(), isign, **kw)
+            err = np.abs(out[sites] - ref).max() / np.abs(ref).max()
+            assert err < 1e-6, (kind, achi, isign, err)
+
+
+def test_recon12_equals_recon18_32x64(qamd, lat32):
+    dims = (32, 32, 32, 64)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    op = qamd.ClovDslash(geom)
+    cb = 1
+    outs = []
+    for recon in (12, 18):
+        g = geom.pack_gauge(lat32["U"], 1, recon, cb)
+        cl = geom.pack_clover(lat32["inv"], 1, cb)
+        psi = geom.pack_spinor(lat32["psi"], 1, 0)
+        res = geom.new_spinor(1, cb=cb)
+        op.dslash(res, psi, g, cl, 1, cb)
+        outs.append(res.data.double())
+    d = (outs[0] - outs[1]).abs().max().item() / outs[1].abs().max().item()
+    assert d < 2e-6, d
+
+
+def test_adjointness_32x64(qamd, lat32):
+    """<chi, D_+ psi> = <D_- chi, psi> with FP64 storage on the full lattice."""
+    dims = (32, 32, 32, 64)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    op = qamd.Dslash(geom)
+    g0 = geom.pack_gauge(lat32["U"], 0, 12, 0)
+    g1 = geom.pack_gauge(lat32["U"], 0, 12, 1)
+    psi1 = geom.pack_spinor(lat32["psi"], 0, 1)
+    chi0 = geom.pack_spinor(lat32["chi"], 0, 0)
+    a0 = geom.new_spinor(0, cb=0)
+    b1 = geom.new_spinor(0, cb=1)
+    op.dslash(a0, psi1, g0, 1, 0)
+    op.dslash(b1, chi0, g1, -1, 1)
+    za = torch.view_as_complex(geom.unpack_spinor(a0))
+    zb = torch.view_as_complex(geom.unpack_spinor(b1))
+    zc = torch.view_as_complex(geom.unpack_spinor(chi0))
+    zp = torch.view_as_complex(geom.unpack_spinor(psi1))
+    lhs = torch.vdot(zc.reshape(-1), za.reshape(-1)).item()
+    rhs = torch.vdot(zb.reshape(-1), zp.reshape(-1)).item()
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
