@@ -69,14 +69,23 @@ template <typename S> constexpr int pick_w(int N)
                                             : 4 / (int)sizeof(S);
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 template <int BYTES> struct RawVec;
-template <> struct RawVec<16> { using T = uint4; };
-template <> struct RawVec<8> { using T = uint2; };
-template <> struct RawVec<4> { using T = uint32_t; };
+template <> struct RawVec<16> { using T = u32x4; };
+template <> struct RawVec<8> { using T = u32x2; };
+template <> struct RawVec<4> { using T = unsigned int; };
+
+// Cache policy of a field access.  Fields read once per launch (links,
+// clover blocks) and the output are streamed with non-temporal accesses so
+// that L2 keeps the neighbour spinors, which every site reads 8 times.
+#ifndef QAMD_NT_STREAM
+#define QAMD_NT_STREAM 1
+#endif
 
 // Load reals [K0, K0+N) of site `site` of a field whose unit width is W.
 // K0 and N are multiples of W.
-template <typename S, int W, int K0, int N, typename R>
+template <typename S, int W, int K0, int N, typename R, bool NT = false>
 __device__ __forceinline__ void load_reals(const S *__restrict__ base, int vcb,
                                            int site, R *__restrict__ out)
 {
@@ -85,7 +94,8 @@ __device__ __forceinline__ void load_reals(const S *__restrict__ base, int vcb,
   const V *vb = reinterpret_cast<const V *>(base);
 #pragma unroll
   for (int c = 0; c < N / W; c++) {
-    V v = vb[(unsigned)((K0 / W + c) * vcb + site)];
+    const V *a = vb + (unsigned)((K0 / W + c) * vcb + site);
+    V v = (NT && QAMD_NT_STREAM) ? __builtin_nontemporal_load(a) : *a;
     S tmp[W];
     __builtin_memcpy(tmp, &v, sizeof(V));
 #pragma unroll
@@ -93,7 +103,7 @@ __device__ __forceinline__ void load_reals(const S *__restrict__ base, int vcb,
   }
 }
 
-template <typename S, int W, int K0, int N, typename R>
+template <typename S, int W, int K0, int N, typename R, bool NT = false>
 __device__ __forceinline__ void store_reals(S *__restrict__ base, int vcb,
                                             int site, const R *__restrict__ in)
 {
@@ -107,7 +117,11 @@ __device__ __forceinline__ void store_reals(S *__restrict__ base, int vcb,
     for (int e = 0; e < W; e++) tmp[e] = to_storage<S>(in[c * W + e]);
     V v;
     __builtin_memcpy(&v, tmp, sizeof(V));
-    vb[(unsigned)((K0 / W + c) * vcb + site)] = v;
+    V *a = vb + (unsigned)((K0 / W + c) * vcb + site);
+    if (NT && QAMD_NT_STREAM)
+      __builtin_nontemporal_store(v, a);
+    else
+      *a = v;
   }
 }
 

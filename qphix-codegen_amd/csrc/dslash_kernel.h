@@ -34,7 +34,14 @@ template <typename R> struct DslashParams {
   R alpha, mu, muinv, rho;
 };
 
-constexpr int kDslashBlock = 256;
+#ifndef QAMD_DSLASH_BLOCK
+#define QAMD_DSLASH_BLOCK 256
+#endif
+constexpr int kDslashBlock = QAMD_DSLASH_BLOCK;
+// Minimum waves per SIMD requested from the register allocator (0 = none).
+#ifndef QAMD_DSLASH_MIN_WAVES
+#define QAMD_DSLASH_MIN_WAVES 0
+#endif
 
 // Half-spinor projection (1 + SGN*gamma_MU) psi -> (h0, h1), DeGrand-Rossi.
 template <int MU, int SGN, typename R>
@@ -93,7 +100,7 @@ __device__ __forceinline__ void load_link(const S *gauge, int vcb, int site, cx<
 {
   constexpr int WL = pick_w<S>(RECON);
   R lk[RECON];
-  load_reals<S, WL, D * RECON, RECON>(gauge, vcb, site, lk);
+  load_reals<S, WL, D * RECON, RECON, R, true>(gauge, vcb, site, lk);
 #pragma unroll
   for (int r = 0; r < RECON / 6; r++)
 #pragma unroll
@@ -152,9 +159,9 @@ __device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
   for (int b = 0; b < 2; b++) {
     R blk[36];
     if (b == 0)
-      load_reals<S, WC, 0, 36>(clov, vcb, site, blk);
+      load_reals<S, WC, 0, 36, R, true>(clov, vcb, site, blk);
     else
-      load_reals<S, WC, 36, 36>(clov, vcb, site, blk);
+      load_reals<S, WC, 36, 36, R, true>(clov, vcb, site, blk);
     cx<R> v[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
@@ -189,9 +196,9 @@ __device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
   for (int b = 0; b < 2; b++) {
     R blk[72];
     if (b == 0)
-      load_reals<S, WF, 0, 72>(clov, vcb, site, blk);
+      load_reals<S, WF, 0, 72, R, true>(clov, vcb, site, blk);
     else
-      load_reals<S, WF, 72, 72>(clov, vcb, site, blk);
+      load_reals<S, WF, 72, 72, R, true>(clov, vcb, site, blk);
     cx<R> v[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
@@ -223,8 +230,14 @@ __device__ __forceinline__ void twist(const cx<R> in[4][3], cx<R> out[4][3], R a
   }
 }
 
+#if QAMD_DSLASH_MIN_WAVES > 0
+#define QAMD_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(QAMD_DSLASH_MIN_WAVES)))
+#else
+#define QAMD_WAVES_ATTR
+#endif
+
 template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
-__global__ __launch_bounds__(kDslashBlock) void dslash_kernel(DslashParams<typename Traits<S>::R> p)
+__global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {
   using R = typename Traits<S>::R;
   const int blk = xcd_block();
@@ -286,7 +299,7 @@ __global__ __launch_bounds__(kDslashBlock) void dslash_kernel(DslashParams<typen
     }
   } else {
     R chr[24];
-    load_reals<S, pick_w<S>(24), 0, 24>(static_cast<const S *>(p.chi), vcb, idx, chr);
+    load_reals<S, pick_w<S>(24), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
     cx<R> chi[4][3];
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -327,7 +340,7 @@ __global__ __launch_bounds__(kDslashBlock) void dslash_kernel(DslashParams<typen
       o[(s * 3 + k) * 2] = res[s][k].re;
       o[(s * 3 + k) * 2 + 1] = res[s][k].im;
     }
-  store_reals<S, pick_w<S>(24), 0, 24>(static_cast<S *>(p.out), vcb, idx, o);
+  store_reals<S, pick_w<S>(24), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
 }
 
 // Host-side launch for one instantiation.
