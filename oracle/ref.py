@@ -1,0 +1,66 @@
+"""ctypes wrapper of oracle/_ref/libref_<config>.so: the reference's own
+generated kernel bodies (qphix-codegen) under a QPhiX-style block driver
+(oracle/ref_driver.cpp).  Built from /root/reference by oracle/ref.mk.
+
+TEST INFRASTRUCTURE ONLY: the tests use it to pin the FP64 oracle and write
+golden fixtures, bench.py times it as the CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+CONFIGS = ("scalar_f64", "avx512_f64", "avx512_f32", "avx2_f64", "avx2_f32")
+_libs = {}
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def path(config):
+    return os.path.join(_HERE, "_ref", f"libref_{config}.so")
+
+
+def available(config):
+    return os.path.exists(path(config))
+
+
+def lib(config):
+    if config not in _libs:
+        l = ctypes.CDLL(path(config))
+        i, d = ctypes.c_int, ctypes.c_double
+        l.ref_apply.argtypes = [i, i, i, i, i, i, i, i, i, _dp, _dp, _dp, _dp, _dp, d, d, d, d, d,
+                                d, d, d, i, i, _dp]
+        l.ref_apply.restype = i
+        l.ref_veclen.restype = i
+        l.ref_soalen.restype = i
+        _libs[config] = l
+    return _libs[config]
+
+
+def soalen(config):
+    return lib(config).ref_soalen()
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=None, recon=18,
+          coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.0, t_boundary=-1.0, alpha=0.0, mu=0.0,
+          muinv=0.0, rho=0.0, nthreads=1, reps=0):
+    """Run one reference kernel over checkerboard cb.  Returns (out, seconds
+    per repetition or None); out is (V,4,3,2) with the cb sites set."""
+    Lx, Ly, Lz, Lt = dims
+    V = Lx * Ly * Lz * Lt
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    psi, U, chi, clov = c(psi), c(U), c(chi), c(clov)
+    out = np.zeros((V, 4, 3, 2))
+    sec = ctypes.c_double(0.0)
+    rc = lib(config).ref_apply(kind, int(achi), isign, recon, cb, Lx, Ly, Lz, Lt, _p(psi), _p(chi),
+                               _p(U), _p(clov), _p(out), coeff_s, coeff_t_f, coeff_t_b, t_boundary,
+                               alpha, mu, muinv, rho, nthreads, reps, ctypes.byref(sec))
+    if rc != 0:
+        raise ValueError(f"reference kernel rejected the call (config {config}, dims {dims})")
+    return out, (sec.value if reps > 0 else None)

@@ -1,0 +1,331 @@
+// ref_driver.cpp -- a QPhiX-style block driver around the reference's own
+// generated kernel bodies (qphix-codegen), compiled per ISA configuration by
+// oracle/ref.mk into oracle/_ref/libref_<config>.so.
+//
+// TEST INFRASTRUCTURE ONLY: the tests use it to pin the FP64 oracle against
+// the reference's kernels, and bench.py times it as the CPU baseline.
+// QPhiX itself (the runtime caller of these bodies) is not part of the
+// reference tree, so this file plays its part:
+//   * packs lexicographic FP64 fields into the reference's block layouts
+//     (codegen/include/decl_common.h:9-62): FourSpinorBlock
+//     FT[3 colour][4 spin][2][SOALEN], SU3MatrixBlock FT[8][2|3][3][2][VECLEN]
+//     holding the transpose of each hop matrix (codegen/dslash_common.cc:
+//     1678-1727), CloverBlock / FullCloverBlock;
+//   * computes the per-block neighbour offsets, including the X-direction
+//     split loads whose lane shift depends on the row parity
+//     (codegen/data_types.cc:140-198, loadSplitSOAFVec in inst_sp_vec16.cc);
+//   * passes coeff_t_b / coeff_t_f multiplied by the time boundary factor on
+//     the first / last time slice (codegen/dslash.cc:79-86);
+//   * runs the block loop with OpenMP, accumulate[] = all lanes.
+// Requires SOALEN == VECLEN (one x-row segment per vector) and Lx/2 a
+// multiple of SOALEN.
+#include <immintrin.h>
+#include <omp.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "decl_common.h"
+#if defined(AVX2) && !defined(AVX512)
+#include "avx_utils.h"
+#endif
+
+#if VECLEN == 1
+// the reference's own glue removes prefetches from scalar bodies the same way
+// (codegen/jinja/complete_specialization.h.j2:54-57)
+#define _mm_prefetch(a, b) \
+  {                        \
+  }
+#endif
+
+#if PRECISION == 1
+typedef float FT;
+#else
+typedef double FT;
+#endif
+
+#if SOALEN != VECLEN
+#error "ref_driver supports SOALEN == VECLEN only"
+#endif
+
+struct RefArgs {
+  const void *xyBase, *zbBase, *zfBase, *tbBase, *tfBase, *chiBase;
+  void *oBase;
+  const void *gBase, *clBase;
+  int xbOffs[VECLEN], xfOffs[VECLEN], ybOffs[VECLEN], yfOffs[VECLEN], offs[VECLEN],
+      gOffs[VECLEN], pfyOffs[VECLEN];
+  unsigned int accumulate[8];
+  FT alpha, coeff_s, coeff_t_f, coeff_t_b, mu, muinv, prec_mass_rho;
+  FT tbc_phases[4][2];
+};
+
+typedef void (*ref_body_fn)(const RefArgs &);
+
+// Binds every name a body may use.
+#define REF_BIND_COMMON                                                              \
+  const Ty::FourSpinorBlock *xyBase = (const Ty::FourSpinorBlock *)A.xyBase;         \
+  const Ty::FourSpinorBlock *zbBase = (const Ty::FourSpinorBlock *)A.zbBase;         \
+  const Ty::FourSpinorBlock *zfBase = (const Ty::FourSpinorBlock *)A.zfBase;         \
+  const Ty::FourSpinorBlock *tbBase = (const Ty::FourSpinorBlock *)A.tbBase;         \
+  const Ty::FourSpinorBlock *tfBase = (const Ty::FourSpinorBlock *)A.tfBase;         \
+  const Ty::FourSpinorBlock *chiBase = (const Ty::FourSpinorBlock *)A.chiBase;       \
+  Ty::FourSpinorBlock *oBase = (Ty::FourSpinorBlock *)A.oBase;                       \
+  const Ty::SU3MatrixBlock *gBase = (const Ty::SU3MatrixBlock *)A.gBase;             \
+  const int *xbOffs = A.xbOffs, *xfOffs = A.xfOffs, *ybOffs = A.ybOffs;              \
+  const int *yfOffs = A.yfOffs, *offs = A.offs, *gOffs = A.gOffs;                    \
+  const int *pfyOffs = A.pfyOffs;                                                    \
+  const Ty::FourSpinorBlock *pfBase1 = 0, *pfBase2 = 0, *pfBase3 = 0, *pfBase4 = 0;  \
+  const Ty::FourSpinorBlock *pfBaseChi = 0;                                          \
+  const int siprefdist1 = 0, siprefdist2 = 0, siprefdist3 = 0, siprefdist4 = 0;      \
+  const int chiprefdist = 0, gprefdist = 0, clprefdist = 0;                          \
+  const unsigned int *accumulate = A.accumulate;                                     \
+  const FT alpha = A.alpha, coeff_s = A.coeff_s, coeff_t_f = A.coeff_t_f;            \
+  const FT coeff_t_b = A.coeff_t_b, mu = A.mu, muinv = A.muinv;                      \
+  const FT prec_mass_rho = A.prec_mass_rho;                                          \
+  const FT(*tbc_phases)[2] = A.tbc_phases;
+
+using namespace QPhiX;
+
+#include "bodies.inc"
+
+namespace {
+
+constexpr int S = SOALEN;
+constexpr int BLK = 24 * S;  // FT elements per FourSpinorBlock
+
+struct Lat {
+  int L[4], Lxh, nxb;
+  long V;
+  long block(int t, int z, int y, int xb) const
+  {
+    return (((long)t * L[2] + z) * L[1] + y) * nxb + xb;
+  }
+  long lex(int x, int y, int z, int t) const
+  {
+    return (((long)t * L[2] + z) * L[1] + y) * L[0] + x;
+  }
+  int xof(int xh, int y, int z, int t, int cb) const { return 2 * xh + ((cb + y + z + t) & 1); }
+};
+
+template <typename T> T *amalloc(size_t n)
+{
+  void *p = aligned_alloc(64, ((n * sizeof(T) + 63) / 64) * 64);
+  memset(p, 0, n * sizeof(T));
+  return (T *)p;
+}
+
+// lex FP64 spinor [V][4][3][2] -> blocks FT[3][4][2][S] of checkerboard cb
+void pack_spinor(FT *dst, const double *src, const Lat &g, int cb)
+{
+  for (int t = 0; t < g.L[3]; t++)
+    for (int z = 0; z < g.L[2]; z++)
+      for (int y = 0; y < g.L[1]; y++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          FT *b = dst + g.block(t, z, y, xb) * BLK;
+          for (int i = 0; i < S; i++) {
+            long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
+            for (int c = 0; c < 3; c++)
+              for (int sp = 0; sp < 4; sp++)
+                for (int r = 0; r < 2; r++)
+                  b[((c * 4 + sp) * 2 + r) * S + i] = (FT)src[((s * 4 + sp) * 3 + c) * 2 + r];
+          }
+        }
+}
+
+void unpack_spinor(double *dst, const FT *src, const Lat &g, int cb)
+{
+  for (int t = 0; t < g.L[3]; t++)
+    for (int z = 0; z < g.L[2]; z++)
+      for (int y = 0; y < g.L[1]; y++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          const FT *b = src + g.block(t, z, y, xb) * BLK;
+          for (int i = 0; i < S; i++) {
+            long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
+            for (int c = 0; c < 3; c++)
+              for (int sp = 0; sp < 4; sp++)
+                for (int r = 0; r < 2; r++)
+                  dst[((s * 4 + sp) * 3 + c) * 2 + r] = (double)b[((c * 4 + sp) * 2 + r) * S + i];
+          }
+        }
+}
+
+// Gauge blocks FT[8][nrow][3][2][V]: S[d][i][j] = M_d[j][i], with
+// M_{2mu} = U_mu(x - mu) (applied as its adjoint) and M_{2mu+1} = U_mu(x).
+void pack_gauge(FT *dst, const double *U, const Lat &g, int cb, int nrow)
+{
+  const long gblk = 8L * nrow * 3 * 2 * VECLEN;
+  for (int t = 0; t < g.L[3]; t++)
+    for (int z = 0; z < g.L[2]; z++)
+      for (int y = 0; y < g.L[1]; y++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          FT *b = dst + g.block(t, z, y, xb) * gblk;
+          for (int i = 0; i < S; i++) {
+            int c4[4] = {g.xof(xb * S + i, y, z, t, cb), y, z, t};
+            for (int d = 0; d < 8; d++) {
+              int mu = d / 2, fb = d % 2;
+              int n[4] = {c4[0], c4[1], c4[2], c4[3]};
+              if (!fb) n[mu] = (n[mu] - 1 + g.L[mu]) % g.L[mu];
+              long s = g.lex(n[0], n[1], n[2], n[3]);
+              const double *m = U + (s * 4 + mu) * 18;
+              for (int r = 0; r < nrow; r++)
+                for (int col = 0; col < 3; col++)
+                  for (int ri = 0; ri < 2; ri++)
+                    b[(((d * nrow + r) * 3 + col) * 2 + ri) * VECLEN + i] =
+                        (FT)m[(col * 3 + r) * 2 + ri];
+            }
+          }
+        }
+}
+
+// lex [V][2][36] -> CloverBlock {diag1[6][V], off1[15][2][V], diag2, off2}
+void pack_clover(FT *dst, const double *cl, const Lat &g, int cb)
+{
+  const long cblk = 72L * VECLEN;
+  for (int t = 0; t < g.L[3]; t++)
+    for (int z = 0; z < g.L[2]; z++)
+      for (int y = 0; y < g.L[1]; y++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          FT *b = dst + g.block(t, z, y, xb) * cblk;
+          for (int i = 0; i < S; i++) {
+            long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
+            for (int blk = 0; blk < 2; blk++) {
+              const double *src = cl + s * 72 + blk * 36;
+              FT *diag = b + blk * 36 * VECLEN;
+              FT *off = diag + 6 * VECLEN;
+              for (int k = 0; k < 6; k++) diag[k * VECLEN + i] = (FT)src[k];
+              for (int k = 0; k < 15; k++)
+                for (int ri = 0; ri < 2; ri++) off[(k * 2 + ri) * VECLEN + i] = (FT)src[6 + 2 * k + ri];
+            }
+          }
+        }
+}
+
+// lex [V][2][6][6][2] -> FullCloverBlock {block1[6][6][2][V], block2}
+void pack_full_clover(FT *dst, const double *cl, const Lat &g, int cb)
+{
+  const long cblk = 144L * VECLEN;
+  for (int t = 0; t < g.L[3]; t++)
+    for (int z = 0; z < g.L[2]; z++)
+      for (int y = 0; y < g.L[1]; y++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          FT *b = dst + g.block(t, z, y, xb) * cblk;
+          for (int i = 0; i < S; i++) {
+            long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
+            for (int k = 0; k < 144; k++) b[k * VECLEN + i] = (FT)cl[s * 144 + k];
+          }
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ref_config(void) { return REF_CONFIG_NAME; }
+int ref_veclen(void) { return VECLEN; }
+int ref_soalen(void) { return SOALEN; }
+int ref_precision(void) { return PRECISION; }
+
+// kind: 0 wilson, 1 clover, 2 tm, 3 tm-clover.  Inputs are lexicographic
+// FP64 arrays; psi is read on checkerboard 1-cb, chi on cb, and out is
+// written on cb.  Returns 0, or -1 for an unsupported geometry/argument.
+// reps > 0 times the block loop (not the packing) and stores seconds/rep.
+int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, int Lz, int Lt,
+              const double *psi, const double *chi, const double *U, const double *clov,
+              double *out, double coeff_s, double coeff_t_f, double coeff_t_b,
+              double t_boundary, double alpha, double mu, double muinv, double rho,
+              int nthreads, int reps, double *seconds)
+{
+  if (kind < 0 || kind > 3 || (recon != 12 && recon != 18) || (cb != 0 && cb != 1)) return -1;
+  if ((Lx | Ly | Lz | Lt) & 1) return -1;
+  Lat g;
+  g.L[0] = Lx;
+  g.L[1] = Ly;
+  g.L[2] = Lz;
+  g.L[3] = Lt;
+  g.Lxh = Lx / 2;
+  if (g.Lxh % S) return -1;
+  g.nxb = g.Lxh / S;
+  const long nblk = (long)Lt * Lz * Ly * g.nxb;
+  const int nrow = recon == 12 ? 2 : 3;
+  FT *in = amalloc<FT>(nblk * BLK);
+  FT *chib = amalloc<FT>(nblk * BLK);
+  FT *o = amalloc<FT>(nblk * BLK);
+  const long gblk = 8L * nrow * 3 * 2 * VECLEN;
+  FT *gb = amalloc<FT>(nblk * gblk);
+  const long cblk = (kind == 3 ? 144L : 72L) * VECLEN;
+  FT *cl = (kind == 1 || kind == 3) ? amalloc<FT>(nblk * cblk) : nullptr;
+  pack_spinor(in, psi, g, 1 - cb);
+  if (achi) pack_spinor(chib, chi, g, cb);
+  pack_gauge(gb, U, g, cb, nrow);
+  if (kind == 1) pack_clover(cl, clov, g, cb);
+  if (kind == 3) pack_full_clover(cl, clov, g, cb);
+
+  ref_body_fn fn = ref_table[kind][achi ? 1 : 0][isign > 0 ? 0 : 1][recon == 12 ? 0 : 1];
+  const unsigned int all = VECLEN >= 32 ? 0xffffffffu : ((1u << VECLEN) - 1u);
+
+  auto run = [&]() {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long bi = 0; bi < nblk; bi++) {
+      long r = bi;
+      const int xb = (int)(r % g.nxb);
+      r /= g.nxb;
+      const int y = (int)(r % Ly);
+      r /= Ly;
+      const int z = (int)(r % Lz);
+      const int t = (int)(r / Lz);
+      const int p = (cb + y + z + t) & 1;  // x = 2 xh + p on this row
+      RefArgs A;
+      memset(&A, 0, sizeof(A));
+      const FT *blk = in + bi * BLK;
+      A.xyBase = blk;
+      const long xprev = g.block(t, z, y, (xb + g.nxb - 1) % g.nxb);
+      const long xnext = g.block(t, z, y, (xb + 1) % g.nxb);
+      if (p == 1) {  // x-1 = 2 xh: same lanes; x+1 = 2 xh + 2: shift by one
+        A.xbOffs[0] = 0;
+        if (S > 1) A.xbOffs[1] = 1;
+        A.xfOffs[0] = 1;
+        A.xfOffs[S - 1] = (int)((xnext - bi) * BLK);
+      } else {  // x-1 = 2 xh - 1: shift by one; x+1 = 2 xh + 1: same lanes
+        A.xbOffs[0] = (int)((xprev - bi) * BLK + (S - 1));
+        if (S > 1) A.xbOffs[1] = 0;
+        A.xfOffs[0] = 0;
+        A.xfOffs[S - 1] = S - 1;
+      }
+      A.ybOffs[0] = (int)((g.block(t, z, (y + Ly - 1) % Ly, xb) - bi) * BLK);
+      A.yfOffs[0] = (int)((g.block(t, z, (y + 1) % Ly, xb) - bi) * BLK);
+      A.zbBase = in + g.block(t, (z + Lz - 1) % Lz, y, xb) * BLK;
+      A.zfBase = in + g.block(t, (z + 1) % Lz, y, xb) * BLK;
+      A.tbBase = in + g.block((t + Lt - 1) % Lt, z, y, xb) * BLK;
+      A.tfBase = in + g.block((t + 1) % Lt, z, y, xb) * BLK;
+      A.chiBase = chib + bi * BLK;
+      A.oBase = o + bi * BLK;
+      A.gBase = gb + bi * gblk;
+      A.clBase = cl ? cl + bi * cblk : nullptr;
+      for (int d = 0; d < 8; d++) A.accumulate[d] = all;
+      A.alpha = (FT)alpha;
+      A.coeff_s = (FT)coeff_s;
+      A.coeff_t_b = (FT)(t == 0 ? coeff_t_b * t_boundary : coeff_t_b);
+      A.coeff_t_f = (FT)(t == Lt - 1 ? coeff_t_f * t_boundary : coeff_t_f);
+      A.mu = (FT)mu;
+      A.muinv = (FT)muinv;
+      A.prec_mass_rho = (FT)rho;
+      fn(A);
+    }
+  };
+  run();
+  if (reps > 0) {
+    double t0 = omp_get_wtime();
+    for (int r = 0; r < reps; r++) run();
+    if (seconds) *seconds = (omp_get_wtime() - t0) / reps;
+  }
+  unpack_spinor(out, o, g, cb);
+  free(in);
+  free(chib);
+  free(o);
+  free(gb);
+  free(cl);
+  return 0;
+}
+
+}  // extern "C"

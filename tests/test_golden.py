@@ -1,0 +1,114 @@
+"""The oracle against the reference's own kernels.
+
+* committed fixtures written by tests/golden/make_golden.py from the
+  reference's generated scalar FP64 and AVX-512 FP32 kernel bodies;
+* when oracle/_ref is built (it needs /root/reference at build time), the
+  reference kernels of every configuration live, on seeded lattices, for
+  every family x {dslash, achimbdpsi} x isign x cb x 12/18 links.
+
+Tolerances: 1e-13 relative for FP64 reference kernels, 1e-6 for FP32
+(relative = max-abs difference / max-abs value).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from _util import cb_mask
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLD, name), allow_pickle=False))
+
+
+def params(f):
+    return {str(k): float(v) for k, v in zip(f["param_names"], f["params"])}
+
+
+def epilogue(qamd, f, kind, achi, isign):
+    import torch
+    L = qamd.lattice
+    clov = torch.from_numpy(np.asarray(f["clov"], dtype=np.float64))
+    if kind == 1:
+        if achi:
+            return clov.numpy()
+        return f["inv"].astype(np.float64) if "inv" in f else L.invert_clover(clov).numpy()
+    if kind == 3:
+        return L.tmclover_blocks(clov, float(f["tm_mu"]), isign, not achi).numpy()
+    return None
+
+
+def test_oracle_matches_reference_scalar_f64_fixture(qamd, orc):
+    f = load("ref_scalar_f64_4x4.npz")
+    dims = tuple(int(d) for d in f["dims"])
+    p = params(f)
+    for case, sites, ref_out in zip(f["cases"], f["sites"], f["outs"]):
+        kind, achi, isign, cb, recon = (int(c) for c in case)
+        out = orc.dslash_sites(dims, sites, f["psi"], f["U"], isign, kind=kind, achi=bool(achi),
+                               chi=f["chi"], clov=epilogue(qamd, f, kind, achi, isign), **p)
+        err = np.abs(out - ref_out).max() / np.abs(ref_out).max()
+        assert err < 1e-13, (case, err)
+
+
+def test_oracle_matches_reference_avx512_f32_fixture(qamd, orc):
+    f = load("ref_avx512_f32_32x2.npz")
+    dims = tuple(int(d) for d in f["dims"])
+    p = params(f)
+    d64 = {k: np.asarray(f[k], dtype=np.float64) for k in ("U", "psi", "chi", "clov", "inv")}
+    for case, ref_out in zip(f["cases"], f["outs"]):
+        kind, achi, isign, cb, recon = (int(c) for c in case)
+        clov = d64["clov"] if achi else d64["inv"]
+        out = orc.dslash(dims, d64["psi"], d64["U"], isign, cb, kind=kind, achi=bool(achi),
+                         chi=d64["chi"], clov=clov, **p)
+        m = cb_mask(dims, cb)
+        err = np.abs(out[m] - ref_out).max() / np.abs(ref_out).max()
+        assert err < 1e-6, (case, err)
+
+
+REF_DIMS = {"scalar_f64": (8, 8, 8, 8), "avx512_f64": (16, 4, 4, 8), "avx512_f32": (32, 4, 4, 4),
+            "avx2_f64": (8, 8, 8, 8), "avx2_f32": (16, 4, 4, 8)}
+
+
+def _cpu_has(flag):
+    try:
+        return flag in open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+
+
+@pytest.mark.parametrize("config", list(REF_DIMS))
+def test_oracle_matches_reference_kernels_live(qamd, orc, config):
+    from oracle import ref
+    if not ref.available(config):
+        pytest.skip("oracle/_ref not built (needs the reference sources at build time)")
+    if "avx512" in config and not _cpu_has("avx512f"):
+        pytest.skip("host CPU has no AVX-512")
+    if "avx2" in config and not _cpu_has("avx2"):
+        pytest.skip("host CPU has no AVX2")
+    L = qamd.lattice
+    dims = REF_DIMS[config]
+    U = L.random_gauge(dims, 0.3, 31).numpy()
+    psi = L.random_spinor(dims, 32).numpy()
+    chi = L.random_spinor(dims, 33).numpy()
+    clt = L.random_clover(dims, 34)
+    tol = 1e-13 if config.endswith("f64") else 1e-6
+    kw0 = dict(alpha=3.7, mu=0.027, muinv=0.26, rho=0.05, coeff_s=0.9, coeff_t_f=1.1, coeff_t_b=1.2,
+               t_boundary=-1.0)
+    for kind in range(4):
+        for achi in (0, 1):
+            for isign in (1, -1):
+                for cb in (0, 1):
+                    kw = dict(kw0, kind=kind, achi=bool(achi), chi=chi)
+                    if kind == 1:
+                        kw["clov"] = clt.numpy() if achi else L.invert_clover(clt).numpy()
+                    if kind == 3:
+                        kw["clov"] = L.tmclover_blocks(clt, 0.1, isign, not achi).numpy()
+                    a = orc.dslash(dims, psi, U, isign, cb, **kw)
+                    m = cb_mask(dims, cb)
+                    for recon in (12, 18):
+                        b, _ = ref.apply(config, dims, psi, U, isign, cb, recon=recon, **kw)
+                        err = np.abs(a[m] - b[m]).max() / np.abs(a[m]).max()
+                        assert err < tol, (kind, achi, isign, cb, recon, err)

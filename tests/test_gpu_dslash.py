@@ -248,3 +248,98 @@ def test_adjointness_32x64(qamd, lat32):
     lhs = torch.vdot(zc.reshape(-1), za.reshape(-1)).item()
     rhs = torch.vdot(zb.reshape(-1), zp.reshape(-1)).item()
     assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# directly against the reference's own kernels (committed fixtures)
+def _fixture(name):
+    import os
+    return dict(np.load(os.path.join(os.path.dirname(__file__), "golden", name), allow_pickle=False))
+
+
+def _fixture_lat(qamd, f, isign, kind, achi):
+    L = qamd.lattice
+    d = {k: torch.from_numpy(np.asarray(f[k], dtype=np.float64)) for k in ("U", "psi", "chi", "clov")}
+    lat = dict(U=d["U"], psi=d["psi"], chi=d["chi"], clov=d["clov"])
+    lat["inv"] = (torch.from_numpy(np.asarray(f["inv"], dtype=np.float64)) if "inv" in f
+                  else L.invert_clover(d["clov"]))
+    mu = float(f["tm_mu"])
+    lat[f"B{isign}"] = L.tmclover_blocks(d["clov"], mu, isign, True)
+    lat[f"F{isign}"] = L.tmclover_blocks(d["clov"], mu, isign, False)
+    return lat
+
+
+def _run_fixture_case(qamd, f, geom, prec, case):
+    kind, achi, isign, cb, recon = (int(c) for c in case)
+    p = {str(k): float(v) for k, v in zip(f["param_names"], f["params"])}
+    lat = _fixture_lat(qamd, f, isign, kind, achi)
+    g = geom.pack_gauge(lat["U"].to(DEV), prec, recon, cb)
+    psi = geom.pack_spinor(lat["psi"].to(DEV), prec, 1 - cb)
+    chi = geom.pack_spinor(lat["chi"].to(DEV), prec, cb)
+    res = geom.new_spinor(prec, cb=cb)
+    lib = qamd.lib()
+    ctx = geom.ctx
+    st = None
+    cs, ctf, ctb = p["coeff_s"], p["coeff_t_f"], p["coeff_t_b"]
+    ep = epilogue_field(kind, achi, isign, lat)
+    cl = None
+    if kind == 1:
+        cl = geom.pack_clover(ep.to(DEV), prec, cb)
+    elif kind == 3:
+        cl = geom.pack_full_clover(ep.to(DEV), prec, cb)
+    import ctypes
+    P = ctypes.c_void_p
+    if kind == 0 and not achi:
+        rc = lib.qphix_amd_dslash(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(), g.ptr(), cs, ctf, ctb, st)
+    elif kind == 0:
+        rc = lib.qphix_amd_dslash_achimbdpsi(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(), chi.ptr(),
+                                             g.ptr(), p["alpha"], cs, ctf, ctb, st)
+    elif kind == 1 and not achi:
+        rc = lib.qphix_amd_clov_dslash(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(), g.ptr(), cl.ptr(),
+                                       cs, ctf, ctb, st)
+    elif kind == 1:
+        rc = lib.qphix_amd_clov_dslash_achimbdpsi(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(),
+                                                  chi.ptr(), g.ptr(), cl.ptr(), cs, ctf, ctb, p["rho"], st)
+    elif kind == 2 and not achi:
+        rc = lib.qphix_amd_tm_dslash(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(), g.ptr(), cs, ctf,
+                                     ctb, p["mu"], p["muinv"], st)
+    elif kind == 2:
+        rc = lib.qphix_amd_tm_dslash_achimbdpsi(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(),
+                                                chi.ptr(), g.ptr(), p["alpha"], cs, ctf, ctb, p["mu"], st)
+    elif not achi:
+        rc = lib.qphix_amd_tm_clov_dslash(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(), g.ptr(),
+                                          cl.ptr(), cs, ctf, ctb, st)
+    else:
+        rc = lib.qphix_amd_tm_clov_dslash_achimbdpsi(ctx, prec, recon, isign, cb, res.ptr(), psi.ptr(),
+                                                     chi.ptr(), g.ptr(), cl.ptr(), cs, ctf, ctb, p["rho"],
+                                                     st)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return geom.unpack_spinor(res).cpu().numpy()
+
+
+def test_gpu_f64_matches_reference_scalar_fixture(qamd):
+    """Every operator family through the C ABI vs the reference's scalar
+    FP64 kernels, at the fixture's sampled sites (1e-13 relative)."""
+    f = _fixture("ref_scalar_f64_4x4.npz")
+    dims = tuple(int(d) for d in f["dims"])
+    p = {str(k): float(v) for k, v in zip(f["param_names"], f["params"])}
+    geom = qamd.Geometry(dims, t_boundary=p["t_boundary"])
+    for case, sites, ref_out in zip(f["cases"], f["sites"], f["outs"]):
+        out = _run_fixture_case(qamd, f, geom, 0, case)
+        err = np.abs(out[sites] - ref_out).max() / np.abs(ref_out).max()
+        assert err < 1e-13, (case, err)
+
+
+def test_gpu_f32_matches_reference_avx512_fixture(qamd):
+    """FP32 clover family vs the reference's AVX-512 FP32 kernels (1e-6)."""
+    f = _fixture("ref_avx512_f32_32x2.npz")
+    dims = tuple(int(d) for d in f["dims"])
+    p = {str(k): float(v) for k, v in zip(f["param_names"], f["params"])}
+    geom = qamd.Geometry(dims, t_boundary=p["t_boundary"])
+    for case, ref_out in zip(f["cases"], f["outs"]):
+        cb = int(case[3])
+        out = _run_fixture_case(qamd, f, geom, 1, case)
+        m = cb_mask(dims, cb)
+        err = np.abs(out[m] - ref_out).max() / np.abs(ref_out).max()
+        assert err < 1e-6, (case, err)
