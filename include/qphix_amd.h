@@ -176,6 +176,58 @@ int qphix_amd_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
                                         double coeff_t_f, double coeff_t_b,
                                         double prec_mass_rho, void *stream);
 
+/* ---- clover term from links (no counterpart in the reference, whose
+   kernels take A or A^-1 as input) ------------------------------------
+   A(x) = alpha + (csw/2) sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
+   sigma_{mu nu} = (i/2)[g_mu, g_nu], F_{mu nu} = traceless part of
+   (Q - Q^dagger)/(8i), Q = sum of the four plaquette leaves at x.
+   All FP64 on lexicographic device arrays.                              */
+int qphix_amd_clover_from_links(qphix_amd_ctx *ctx, double *clov_lex, const double *u_lex,
+                                double alpha, double csw, void *stream);
+/* packed Hermitian blocks -> packed Hermitian inverse blocks            */
+int qphix_amd_clover_invert(qphix_amd_ctx *ctx, double *inv_lex, const double *clov_lex,
+                            void *stream);
+/* full complex blocks of A + i*sign*mu*g5 (invert = 0) or of its inverse  */
+int qphix_amd_tmclover_blocks(qphix_amd_ctx *ctx, double *out_lex, const double *clov_lex,
+                              double mu, int sign, int invert, void *stream);
+
+/* ---- even-odd operator and CG (no counterpart in the reference; these
+   are the callers QPhiX builds on top of its generated kernels) ----------
+   M = F_oo - 1/4 D_oe F_ee^-1 D_eo on the odd checkerboard (cb 1),
+   F = A + i mu g5 (mu = 0: Wilson-clover with packed Hermitian blocks).
+   The operator keeps an FP64 copy (12-real links) and a sloppy copy in
+   prec_sloppy / recon_sloppy storage; it allocates device memory.       */
+typedef struct qphix_amd_eo_op qphix_amd_eo_op;
+int qphix_amd_eo_op_create(qphix_amd_eo_op **op, qphix_amd_ctx *ctx, const double *u_lex,
+                           const double *clov_lex, double mu, int prec_sloppy,
+                           int recon_sloppy, void *stream);
+int qphix_amd_eo_op_destroy(qphix_amd_eo_op *op);
+/* out = M in (dagger = 0) or M^dagger in (dagger = 1); prec is
+   QPHIX_AMD_F64 or the operator's sloppy precision; cb-1 native spinors. */
+int qphix_amd_eo_op_apply(qphix_amd_eo_op *op, int prec, int dagger, void *out, const void *in,
+                          void *stream);
+
+typedef struct {
+  double rsd_target; /* stop at |b - M^dag M x| / |b| < rsd_target (FP64)   */
+  int max_iters;
+  double delta;      /* reliable update when |r| < delta * max|r| (def 0.1) */
+  int mixed;         /* 0: CG in FP64; 1: sloppy iterations + reliable updates */
+} qphix_amd_cg_params;
+
+typedef struct {
+  int iters;            /* CG iterations                                  */
+  int reliable_updates; /* FP64 true-residual recomputations             */
+  double rsd_final;     /* final true relative residual (FP64)            */
+  double seconds;       /* wall time of the solve (host clock)            */
+  long mv_apps;         /* applications of M or M^dagger                 */
+} qphix_amd_cg_result;
+
+/* Solves M^dag M x = b; x, b are FP64 native spinors on cb 1.  Returns
+   QPHIX_AMD_ENOCONV (with res filled) when max_iters is reached.         */
+int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b,
+                       const qphix_amd_cg_params *params, qphix_amd_cg_result *res,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

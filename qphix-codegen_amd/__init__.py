@@ -7,3 +7,5 @@ from ._lib import (BF16, F16, F32, F64, FIELD_CLOVER, FIELD_FULLCLOVER, FIELD_GA
 from .dslash import (PREC_NAME, TORCH_DTYPE, ClovDslash, Dslash, Field, Geometry,  # noqa: F401
                      TMClovDslash, TMDslash)
 from . import lattice  # noqa: F401
+from .solver import (EvenOddCloverOperator, InvCG, clover_from_links, clover_invert,  # noqa: F401
+                     tmclover_blocks)

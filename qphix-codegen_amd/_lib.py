@@ -14,6 +14,19 @@ FIELD_SPINOR, FIELD_GAUGE, FIELD_CLOVER, FIELD_FULLCLOVER = 0, 1, 2, 3
 
 _c_int, _c_double, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 
+
+class CGParams(ctypes.Structure):
+    """qphix_amd_cg_params"""
+    _fields_ = [("rsd_target", ctypes.c_double), ("max_iters", ctypes.c_int),
+                ("delta", ctypes.c_double), ("mixed", ctypes.c_int)]
+
+
+class CGResult(ctypes.Structure):
+    """qphix_amd_cg_result"""
+    _fields_ = [("iters", ctypes.c_int), ("reliable_updates", ctypes.c_int),
+                ("rsd_final", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("mv_apps", ctypes.c_long)]
+
 # name -> (restype, argtypes); the order follows include/qphix_amd.h
 SIGNATURES = {
     "qphix_amd_ctx_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_double]),
@@ -44,6 +57,13 @@ SIGNATURES = {
     "qphix_amd_tm_clov_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                                      _vp, _vp, _c_double, _c_double, _c_double, _c_double,
                                                      _vp]),
+    "qphix_amd_clover_from_links": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _vp]),
+    "qphix_amd_clover_invert": (_c_int, [_vp, _vp, _vp, _vp]),
+    "qphix_amd_tmclover_blocks": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _vp]),
+    "qphix_amd_eo_op_create": (_c_int, [ctypes.POINTER(_vp), _vp, _vp, _vp, _c_double, _c_int, _c_int, _vp]),
+    "qphix_amd_eo_op_destroy": (_c_int, [_vp]),
+    "qphix_amd_eo_op_apply": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
+    "qphix_amd_cg_solve": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(CGParams), ctypes.POINTER(CGResult), _vp]),
 }
 
 

@@ -1,0 +1,38 @@
+// blas.h -- vector kernels and device-resident scalars of the solvers.
+#pragma once
+
+#include "internal.h"
+
+namespace qamd {
+
+// Scalars of a CG iteration, resident in device memory (FP64).
+struct CGScalars {
+  double rr;       // current |r|^2 (solver units)
+  double rr_prev;  // previous |r|^2
+  double pAp;      // <p, A p>
+  double alpha, beta;
+  double tmp;      // result of a plain reduction
+};
+
+constexpr int kMaxPartials = 1024;
+
+int blas_grid(long ngroups);
+// Re<a, b> over n reals (n % 8 == 0); kind: 0 -> alpha = rr / result,
+// 2 -> sc->tmp = result.
+void blas_dot(int prec, const void *a, const void *b, long n, double *partial, CGScalars *sc,
+              int kind, hipStream_t st);
+// y = a x + b y (types may differ); with sc != null, sc->tmp = |y|^2.
+void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double b, long n,
+                double *partial, CGScalars *sc, hipStream_t st);
+// r -= alpha Ap, x += alpha p, then beta = |r|^2 / rr, rr = |r|^2.
+void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const void *Ap, long n,
+                    double *partial, CGScalars *sc, hipStream_t st);
+// p = r + beta p
+void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st);
+
+hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const double *U,
+                                 double alpha, double csw, hipStream_t st);
+hipError_t run_clover_blocks(const qphix_amd_ctx *ctx, double *out, const double *clov, int mode,
+                             double mu, int sign, hipStream_t st);
+
+}  // namespace qamd

@@ -29,15 +29,7 @@ int run(qphix_amd_ctx *ctx, const DslashCall &c, void *stream)
   if (!aligned16(c.out) || !aligned16(c.in) || !aligned16(c.gauge) ||
       (c.chi && !aligned16(c.chi)) || (c.clov && !aligned16(c.clov)))
     return QPHIX_AMD_EALIGN;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  hipError_t e;
-  switch (c.prec) {
-    case QPHIX_AMD_F64: e = qamd::run_dslash_f64(ctx, c, st); break;
-    case QPHIX_AMD_F32: e = qamd::run_dslash_f32(ctx, c, st); break;
-    case QPHIX_AMD_F16: e = qamd::run_dslash_f16(ctx, c, st); break;
-    default: e = qamd::run_dslash_bf16(ctx, c, st); break;
-  }
-  return hip_status(e);
+  return hip_status(qamd::run_dslash_any(ctx, c, static_cast<hipStream_t>(stream)));
 }
 
 DslashCall make_call(int prec, int recon, int op, bool achi, int isign, int cb, void *out,
@@ -63,6 +55,18 @@ DslashCall make_call(int prec, int recon, int op, bool achi, int isign, int cb, 
 }
 
 }  // namespace
+
+namespace qamd {
+hipError_t run_dslash_any(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st)
+{
+  switch (c.prec) {
+    case QPHIX_AMD_F64: return run_dslash_f64(ctx, c, st);
+    case QPHIX_AMD_F32: return run_dslash_f32(ctx, c, st);
+    case QPHIX_AMD_F16: return run_dslash_f16(ctx, c, st);
+    default: return run_dslash_bf16(ctx, c, st);
+  }
+}
+}  // namespace qamd
 
 extern "C" {
 

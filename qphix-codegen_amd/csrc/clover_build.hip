@@ -1,0 +1,332 @@
+// clover_build.hip -- the clover term from the gauge links, its inverse and
+// the twisted-mass clover blocks, all in FP64 on the lexicographic layout.
+//
+// No counterpart in the reference: its kernels take A or A^-1 as input
+// (codegen/dslash_common.cc:1098-1232).  Convention:
+//   A(x) = alpha + (csw/2) sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
+//   sigma_{mu nu} = (i/2)[g_mu, g_nu] (DeGrand-Rossi), F_{mu nu} the
+//   traceless part of (Q - Q^dagger)/(8i), Q the four plaquette leaves at x.
+// sigma_{mu nu} is block diagonal in spin (blocks {0,1} and {2,3}), so A is
+// stored as two packed Hermitian 6x6 blocks (6 real diagonals + 15 complex
+// lower-triangle entries, od(r,c) = r(r-1)/2 + c).
+#include "blas.h"
+
+namespace qamd {
+
+namespace {
+
+struct M3 {
+  cx<double> m[3][3];
+};
+
+__device__ __forceinline__ M3 ld_link(const double *U, long s, int mu)
+{
+  M3 r;
+  const double *p = U + (s * 4 + mu) * 18;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) r.m[i][j] = {p[(i * 3 + j) * 2], p[(i * 3 + j) * 2 + 1]};
+  return r;
+}
+
+__device__ __forceinline__ M3 mul(const M3 &a, const M3 &b)
+{
+  M3 r;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      cx<double> v = a.m[i][0] * b.m[0][j];
+      v = v + a.m[i][1] * b.m[1][j];
+      v = v + a.m[i][2] * b.m[2][j];
+      r.m[i][j] = v;
+    }
+  return r;
+}
+
+// a * b^dagger
+__device__ __forceinline__ M3 mul_nd(const M3 &a, const M3 &b)
+{
+  M3 r;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      cx<double> v = a.m[i][0] * conj(b.m[j][0]);
+      v = v + a.m[i][1] * conj(b.m[j][1]);
+      v = v + a.m[i][2] * conj(b.m[j][2]);
+      r.m[i][j] = v;
+    }
+  return r;
+}
+
+// a^dagger * b
+__device__ __forceinline__ M3 mul_dn(const M3 &a, const M3 &b)
+{
+  M3 r;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      cx<double> v = cmul_conj(a.m[0][i], b.m[0][j]);
+      v = v + cmul_conj(a.m[1][i], b.m[1][j]);
+      v = v + cmul_conj(a.m[2][i], b.m[2][j]);
+      r.m[i][j] = v;
+    }
+  return r;
+}
+
+__device__ __forceinline__ long lex_of(const Geom &g, const int c[4])
+{
+  return (((long)c[3] * g.Lz + c[2]) * g.Ly + c[1]) * g.Lx + c[0];
+}
+
+__device__ __forceinline__ long shifted(const Geom &g, const int c0[4], int mu, int dmu, int nu, int dnu)
+{
+  const int L[4] = {g.Lx, g.Ly, g.Lz, g.Lt};
+  int c[4] = {c0[0], c0[1], c0[2], c0[3]};
+  c[mu] = (c[mu] + dmu + L[mu]) % L[mu];
+  c[nu] = (c[nu] + dnu + L[nu]) % L[nu];
+  return lex_of(g, c);
+}
+
+// sigma_{mu nu}[a][b] for the DeGrand-Rossi basis, computed once on the
+// host and passed by value (16 complex entries per plane, 6 planes).
+struct SigmaTable {
+  double re[6][4][4], im[6][4][4];
+};
+
+__global__ __launch_bounds__(128) void clover_from_links_k(double *clov, const double *U, Geom g,
+                                                           double alpha, double csw, SigmaTable sig)
+{
+  const long V = (long)g.vcb * 2;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  int c[4];
+  {
+    long r = s;
+    c[0] = (int)(r % g.Lx);
+    r /= g.Lx;
+    c[1] = (int)(r % g.Ly);
+    r /= g.Ly;
+    c[2] = (int)(r % g.Lz);
+    c[3] = (int)(r / g.Lz);
+  }
+  cx<double> A[2][6][6];
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+      for (int j = 0; j < 6; j++) A[b][i][j] = {i == j ? alpha : 0.0, 0.0};
+  int plane = 0;
+  for (int mu = 0; mu < 4; mu++) {
+    for (int nu = mu + 1; nu < 4; nu++, plane++) {
+      const long xpm = shifted(g, c, mu, 1, nu, 0), xpn = shifted(g, c, mu, 0, nu, 1);
+      const long xmm = shifted(g, c, mu, -1, nu, 0), xmn = shifted(g, c, mu, 0, nu, -1);
+      const long xmmpn = shifted(g, c, mu, -1, nu, 1), xmmmn = shifted(g, c, mu, -1, nu, -1);
+      const long xpmmn = shifted(g, c, mu, 1, nu, -1);
+      // four leaves, each starting and ending at x
+      M3 P1 = mul_nd(mul(ld_link(U, s, mu), ld_link(U, xpm, nu)), mul(ld_link(U, s, nu), ld_link(U, xpn, mu)));
+      M3 P2 = mul(mul_nd(ld_link(U, s, nu), ld_link(U, xmmpn, mu)),
+                  mul_dn(ld_link(U, xmm, nu), ld_link(U, xmm, mu)));
+      M3 P3 = mul(mul_dn(mul(ld_link(U, xmmmn, nu), ld_link(U, xmm, mu)), ld_link(U, xmmmn, mu)),
+                  ld_link(U, xmn, nu));
+      M3 P4 = mul(mul_dn(ld_link(U, xmn, nu), ld_link(U, xmn, mu)),
+                  mul_nd(ld_link(U, xpmmn, nu), ld_link(U, s, mu)));
+      cx<double> F[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          cx<double> q = P1.m[i][j] + P2.m[i][j] + P3.m[i][j] + P4.m[i][j];
+          cx<double> qd = P1.m[j][i] + P2.m[j][i] + P3.m[j][i] + P4.m[j][i];
+          // (q - conj(qd)) / (8 i)
+          cx<double> d = {q.re - qd.re, q.im + qd.im};
+          F[i][j] = {d.im / 8.0, -d.re / 8.0};
+        }
+      cx<double> tr = {(F[0][0].re + F[1][1].re + F[2][2].re) / 3.0,
+                       (F[0][0].im + F[1][1].im + F[2][2].im) / 3.0};
+#pragma unroll
+      for (int i = 0; i < 3; i++) F[i][i] = F[i][i] - tr;
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 6; r++)
+#pragma unroll
+          for (int q = 0; q < 6; q++) {
+            const int s1 = 2 * b + r / 3, s2 = 2 * b + q / 3;
+            cx<double> sg = {sig.re[plane][s1][s2], sig.im[plane][s1][s2]};
+            A[b][r][q] = A[b][r][q] + (0.5 * csw) * (sg * F[r % 3][q % 3]);
+          }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    double *blk = clov + s * 72 + b * 36;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      blk[r] = A[b][r][r].re;
+#pragma unroll
+      for (int q = 0; q < r; q++) {
+        const int od = r * (r - 1) / 2 + q;
+        blk[6 + 2 * od] = A[b][r][q].re;
+        blk[6 + 2 * od + 1] = A[b][r][q].im;
+      }
+    }
+  }
+}
+
+// Dense complex 6x6 inverse by Gauss-Jordan with partial pivoting (FP64).
+__device__ void invert6(cx<double> A[6][6], cx<double> R[6][6])
+{
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j < 6; j++) R[i][j] = {i == j ? 1.0 : 0.0, 0.0};
+  for (int col = 0; col < 6; col++) {
+    int piv = col;
+    double best = A[col][col].re * A[col][col].re + A[col][col].im * A[col][col].im;
+    for (int r = col + 1; r < 6; r++) {
+      double v = A[r][col].re * A[r][col].re + A[r][col].im * A[r][col].im;
+      if (v > best) {
+        best = v;
+        piv = r;
+      }
+    }
+    if (piv != col)
+      for (int j = 0; j < 6; j++) {
+        cx<double> t = A[col][j];
+        A[col][j] = A[piv][j];
+        A[piv][j] = t;
+        t = R[col][j];
+        R[col][j] = R[piv][j];
+        R[piv][j] = t;
+      }
+    const cx<double> a = A[col][col];
+    const double den = a.re * a.re + a.im * a.im;
+    const cx<double> inv = {a.re / den, -a.im / den};
+    for (int j = 0; j < 6; j++) {
+      A[col][j] = inv * A[col][j];
+      R[col][j] = inv * R[col][j];
+    }
+    for (int r = 0; r < 6; r++) {
+      if (r == col) continue;
+      const cx<double> f = A[r][col];
+      for (int j = 0; j < 6; j++) {
+        A[r][j] = A[r][j] - f * A[col][j];
+        R[r][j] = R[r][j] - f * R[col][j];
+      }
+    }
+  }
+}
+
+__device__ void load_packed(const double *blk, cx<double> A[6][6])
+{
+  for (int r = 0; r < 6; r++) {
+    A[r][r] = {blk[r], 0.0};
+    for (int q = 0; q < r; q++) {
+      const int od = r * (r - 1) / 2 + q;
+      A[r][q] = {blk[6 + 2 * od], blk[6 + 2 * od + 1]};
+      A[q][r] = {blk[6 + 2 * od], -blk[6 + 2 * od + 1]};
+    }
+  }
+}
+
+// mode 0: packed Hermitian inverse; mode 1: full blocks of A + i sign mu g5;
+// mode 2: full blocks of (A + i sign mu g5)^-1.
+__global__ __launch_bounds__(64) void clover_blocks_k(double *out, const double *clov, long nblk,
+                                                      int mode, double mu, int sign)
+{
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblk) return;
+  const int b = (int)(k & 1);
+  cx<double> A[6][6], R[6][6];
+  load_packed(clov + k * 36, A);
+  if (mode != 0) {
+    const double g5 = b == 0 ? 1.0 : -1.0;
+    for (int i = 0; i < 6; i++) A[i][i].im += sign * mu * g5;
+  }
+  if (mode == 1) {
+    double *o = out + k * 72;
+    for (int r = 0; r < 6; r++)
+      for (int q = 0; q < 6; q++) {
+        o[(r * 6 + q) * 2] = A[r][q].re;
+        o[(r * 6 + q) * 2 + 1] = A[r][q].im;
+      }
+    return;
+  }
+  invert6(A, R);
+  if (mode == 0) {
+    double *o = out + k * 36;
+    for (int r = 0; r < 6; r++) {
+      o[r] = R[r][r].re;
+      for (int q = 0; q < r; q++) {
+        const int od = r * (r - 1) / 2 + q;
+        // symmetrise the rounding of the Hermitian inverse
+        o[6 + 2 * od] = 0.5 * (R[r][q].re + R[q][r].re);
+        o[6 + 2 * od + 1] = 0.5 * (R[r][q].im - R[q][r].im);
+      }
+    }
+  } else {
+    double *o = out + k * 72;
+    for (int r = 0; r < 6; r++)
+      for (int q = 0; q < 6; q++) {
+        o[(r * 6 + q) * 2] = R[r][q].re;
+        o[(r * 6 + q) * 2 + 1] = R[r][q].im;
+      }
+  }
+}
+
+SigmaTable make_sigma()
+{
+  // DeGrand-Rossi gamma matrices as (re, im) tables
+  double gr[4][4][4] = {}, gi[4][4][4] = {};
+  gi[0][0][3] = 1; gi[0][1][2] = 1; gi[0][2][1] = -1; gi[0][3][0] = -1;
+  gr[1][0][3] = -1; gr[1][1][2] = 1; gr[1][2][1] = 1; gr[1][3][0] = -1;
+  gi[2][0][2] = 1; gi[2][1][3] = -1; gi[2][2][0] = -1; gi[2][3][1] = 1;
+  gr[3][0][2] = 1; gr[3][1][3] = 1; gr[3][2][0] = 1; gr[3][3][1] = 1;
+  SigmaTable t{};
+  int plane = 0;
+  for (int mu = 0; mu < 4; mu++)
+    for (int nu = mu + 1; nu < 4; nu++, plane++)
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+          double cr = 0, ci = 0;  // [g_mu, g_nu]_{ab}
+          for (int k = 0; k < 4; k++) {
+            cr += gr[mu][a][k] * gr[nu][k][b] - gi[mu][a][k] * gi[nu][k][b];
+            ci += gr[mu][a][k] * gi[nu][k][b] + gi[mu][a][k] * gr[nu][k][b];
+            cr -= gr[nu][a][k] * gr[mu][k][b] - gi[nu][a][k] * gi[mu][k][b];
+            ci -= gr[nu][a][k] * gi[mu][k][b] + gi[nu][a][k] * gr[mu][k][b];
+          }
+          // (i/2) * commutator
+          t.re[plane][a][b] = -0.5 * ci;
+          t.im[plane][a][b] = 0.5 * cr;
+        }
+  return t;
+}
+
+}  // namespace
+
+hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const double *U,
+                                 double alpha, double csw, hipStream_t st)
+{
+  const long V = (long)ctx->g.vcb * 2;
+  const int bs = 128;
+  hipLaunchKernelGGL(clover_from_links_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, clov, U, ctx->g,
+                     alpha, csw, make_sigma());
+  return hipGetLastError();
+}
+
+hipError_t run_clover_blocks(const qphix_amd_ctx *ctx, double *out, const double *clov, int mode,
+                             double mu, int sign, hipStream_t st)
+{
+  const long nblk = (long)ctx->g.vcb * 2 * 2;
+  const int bs = 64;
+  hipLaunchKernelGGL(clover_blocks_k, dim3(blocks_for(nblk, bs)), dim3(bs), 0, st, out, clov, nblk,
+                     mode, mu, sign);
+  return hipGetLastError();
+}
+
+}  // namespace qamd

@@ -27,6 +27,9 @@ hipError_t run_dslash_f32(const qphix_amd_ctx *ctx, const DslashCall &c, hipStre
 hipError_t run_dslash_f16(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
 hipError_t run_dslash_bf16(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
 
+// dispatch on c.prec
+hipError_t run_dslash_any(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
+
 hipError_t run_pack(const qphix_amd_ctx *ctx, int field, int prec, int recon, void *dst,
                     const double *src, int cb, hipStream_t st);
 hipError_t run_unpack_spinor(const qphix_amd_ctx *ctx, int prec, double *dst, const void *src,

@@ -1,0 +1,377 @@
+// solver.hip -- the even-odd preconditioned (twisted-mass) clover operator
+// and a CG on its normal equations, plain FP64 or mixed precision with
+// reliable updates.
+//
+// Operator on the odd checkerboard (cb 1), F = A + i mu g5 per site:
+//   M       = F_oo - 1/4 D_oe F_ee^-1 D_eo
+//   M^dag   = F_oo^dag - 1/4 D_oe^- (F_ee^dag)^-1 D_eo^-
+// i.e. one (tm_)clov_dslash on cb 0 followed by one
+// (tm_)clov_dslash_achimbdpsi with beta = 1/4 on cb 1; the dagger uses
+// isign = -1 and the blocks of -mu (D_+^dag = D_-).  With mu = 0 the
+// Hermitian packed clover kernels are used.  This is the Schur complement
+// that QPhiX's even-odd clover operator builds from the same two generated
+// kernels (codegen/jinja/clov_dslash_decl.h.j2).
+//
+// Mixed-precision CG: the iterated residual r, search direction p and A p
+// live in the sloppy storage type (FP16/BF16/FP32 storage, FP32 arithmetic),
+// in units of a scale sigma = |r_true| / sqrt(n) fixed at each reliable
+// update so that FP16's range holds them.  The iterate increment is
+// accumulated in FP32 and folded into the FP64 solution at each reliable
+// update, where the true residual b - M^dag M x is recomputed in FP64 and
+// replaces r (beta = |r_true|^2 / |r_old|^2, p rescaled to the new units).
+// All reductions are FP64 with a fixed summation order.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "blas.h"
+
+using namespace qamd;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t bytes)
+  {
+    n = bytes;
+    return hipMalloc(&p, bytes);
+  }
+  ~DevBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int sbytes(int prec) { return prec == QPHIX_AMD_F64 ? 8 : prec == QPHIX_AMD_F32 ? 4 : 2; }
+
+}  // namespace
+
+struct qphix_amd_eo_op {
+  qphix_amd_ctx *ctx;
+  int tm;  // 1: twisted-mass clover (full blocks)
+  double mu;
+  int prec_s, recon_s;
+  // per precision level (0 = FP64, 1 = sloppy): links on cb 0 and cb 1;
+  // epilogue fields: dslash blocks on cb 0 and achimbdpsi blocks on cb 1,
+  // index [dagger]
+  DevBuf gauge[2][2];
+  DevBuf dsl[2][2], achi[2][2];
+  DevBuf tmp[2];  // cb-0 intermediate per level
+};
+
+namespace {
+
+int level_prec(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? QPHIX_AMD_F64 : op->prec_s; }
+int level_recon(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? 12 : op->recon_s; }
+
+hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
+                    hipStream_t st)
+{
+  const int isign = dagger ? -1 : 1;
+  DslashCall c{};
+  c.prec = level_prec(op, lvl);
+  c.recon = level_recon(op, lvl);
+  c.op = op->tm ? OP_TMCLOVER : OP_CLOVER;
+  c.isign = isign;
+  c.coeff_s = c.coeff_tf = c.coeff_tb = 1.0;
+  // t = F_ee^-1 D_eo in
+  c.achi = false;
+  c.cb = 0;
+  c.out = op->tmp[lvl].p;
+  c.in = in;
+  c.gauge = op->gauge[lvl][0].p;
+  c.clov = op->dsl[lvl][dagger].p;
+  hipError_t e = run_dslash_any(op->ctx, c, st);
+  if (e != hipSuccess) return e;
+  // out = F_oo in - 1/4 D_oe t
+  c.achi = true;
+  c.cb = 1;
+  c.out = out;
+  c.in = op->tmp[lvl].p;
+  c.chi = in;
+  c.gauge = op->gauge[lvl][1].p;
+  c.clov = op->achi[lvl][dagger].p;
+  c.coeff_s = c.coeff_tf = c.coeff_tb = 0.25;
+  c.rho = 0.0;
+  return run_dslash_any(op->ctx, c, st);
+}
+
+// out = M^dag M in, via the cb-1 temporary mid
+hipError_t eo_normal(qphix_amd_eo_op *op, int lvl, void *out, void *mid, const void *in,
+                     hipStream_t st)
+{
+  hipError_t e = eo_apply(op, lvl, 0, mid, in, st);
+  if (e != hipSuccess) return e;
+  return eo_apply(op, lvl, 1, out, mid, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qphix_amd_clover_from_links(qphix_amd_ctx *ctx, double *clov_lex, const double *u_lex,
+                                double alpha, double csw, void *stream)
+{
+  if (!ctx || !clov_lex || !u_lex) return QPHIX_AMD_EINVAL;
+  return run_clover_from_links(ctx, clov_lex, u_lex, alpha, csw, (hipStream_t)stream) == hipSuccess
+             ? QPHIX_AMD_OK
+             : QPHIX_AMD_EHIP;
+}
+
+int qphix_amd_clover_invert(qphix_amd_ctx *ctx, double *inv_lex, const double *clov_lex, void *stream)
+{
+  if (!ctx || !inv_lex || !clov_lex || inv_lex == clov_lex) return QPHIX_AMD_EINVAL;
+  return run_clover_blocks(ctx, inv_lex, clov_lex, 0, 0.0, 1, (hipStream_t)stream) == hipSuccess
+             ? QPHIX_AMD_OK
+             : QPHIX_AMD_EHIP;
+}
+
+int qphix_amd_tmclover_blocks(qphix_amd_ctx *ctx, double *out_lex, const double *clov_lex, double mu,
+                              int sign, int invert, void *stream)
+{
+  if (!ctx || !out_lex || !clov_lex || (sign != 1 && sign != -1)) return QPHIX_AMD_EINVAL;
+  return run_clover_blocks(ctx, out_lex, clov_lex, invert ? 2 : 1, mu, sign, (hipStream_t)stream) ==
+                 hipSuccess
+             ? QPHIX_AMD_OK
+             : QPHIX_AMD_EHIP;
+}
+
+int qphix_amd_eo_op_create(qphix_amd_eo_op **out_op, qphix_amd_ctx *ctx, const double *u_lex,
+                           const double *clov_lex, double mu, int prec_sloppy, int recon_sloppy,
+                           void *stream)
+{
+  if (!out_op || !ctx || !u_lex || !clov_lex) return QPHIX_AMD_EINVAL;
+  if (prec_sloppy < QPHIX_AMD_F64 || prec_sloppy > QPHIX_AMD_BF16) return QPHIX_AMD_EINVAL;
+  if (recon_sloppy != 12 && recon_sloppy != 18) return QPHIX_AMD_EINVAL;
+  *out_op = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  qphix_amd_eo_op *op = new (std::nothrow) qphix_amd_eo_op();
+  if (!op) return QPHIX_AMD_ENOMEM;
+  op->ctx = ctx;
+  op->tm = mu != 0.0;
+  op->mu = mu;
+  op->prec_s = prec_sloppy;
+  op->recon_s = recon_sloppy;
+  const long V = (long)ctx->g.vcb * 2;
+  const int nreal_cl = op->tm ? 144 : 72;
+  // FP64 lex epilogue fields: [dagger] blocks for the cb-0 dslash (inverse)
+  // and for the cb-1 achimbdpsi (forward)
+  DevBuf lexd[2], lexa[2];
+  int rc = QPHIX_AMD_OK;
+  for (int d = 0; d < 2 && rc == QPHIX_AMD_OK; d++) {
+    if (lexd[d].alloc(V * nreal_cl * 8) != hipSuccess || lexa[d].alloc(V * nreal_cl * 8) != hipSuccess)
+      rc = QPHIX_AMD_ENOMEM;
+  }
+  hipError_t e = hipSuccess;
+  if (rc == QPHIX_AMD_OK) {
+    for (int d = 0; d < 2; d++) {
+      const int sign = d ? -1 : 1;
+      if (op->tm) {
+        e = run_clover_blocks(ctx, (double *)lexd[d].p, clov_lex, 2, mu, sign, st);
+        if (e == hipSuccess) e = run_clover_blocks(ctx, (double *)lexa[d].p, clov_lex, 1, mu, sign, st);
+      } else {
+        e = run_clover_blocks(ctx, (double *)lexd[d].p, clov_lex, 0, 0.0, 1, st);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(lexa[d].p, clov_lex, V * 72 * 8, hipMemcpyDeviceToDevice, st);
+      }
+      if (e != hipSuccess) break;
+    }
+  }
+  const int field = op->tm ? QPHIX_AMD_FIELD_FULLCLOVER : QPHIX_AMD_FIELD_CLOVER;
+  for (int lvl = 0; lvl < 2 && rc == QPHIX_AMD_OK && e == hipSuccess; lvl++) {
+    const int prec = level_prec(op, lvl), recon = level_recon(op, lvl);
+    const size_t gb = qphix_amd_field_bytes(ctx, QPHIX_AMD_FIELD_GAUGE, prec, recon);
+    const size_t cb = qphix_amd_field_bytes(ctx, field, prec, 0);
+    const size_t sb = qphix_amd_field_bytes(ctx, QPHIX_AMD_FIELD_SPINOR, prec, 0);
+    for (int c = 0; c < 2; c++)
+      if (op->gauge[lvl][c].alloc(gb) != hipSuccess) rc = QPHIX_AMD_ENOMEM;
+    for (int d = 0; d < 2; d++)
+      if (op->dsl[lvl][d].alloc(cb) != hipSuccess || op->achi[lvl][d].alloc(cb) != hipSuccess)
+        rc = QPHIX_AMD_ENOMEM;
+    if (op->tmp[lvl].alloc(sb) != hipSuccess) rc = QPHIX_AMD_ENOMEM;
+    if (rc != QPHIX_AMD_OK) break;
+    for (int c = 0; c < 2 && e == hipSuccess; c++)
+      e = run_pack(ctx, QPHIX_AMD_FIELD_GAUGE, prec, recon, op->gauge[lvl][c].p, u_lex, c, st);
+    for (int d = 0; d < 2 && e == hipSuccess; d++) {
+      e = run_pack(ctx, field, prec, 0, op->dsl[lvl][d].p, (const double *)lexd[d].p, 0, st);
+      if (e == hipSuccess)
+        e = run_pack(ctx, field, prec, 0, op->achi[lvl][d].p, (const double *)lexa[d].p, 1, st);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // lex temporaries are freed next
+  if (rc == QPHIX_AMD_OK && e != hipSuccess) rc = QPHIX_AMD_EHIP;
+  if (rc != QPHIX_AMD_OK) {
+    delete op;
+    return rc;
+  }
+  *out_op = op;
+  return QPHIX_AMD_OK;
+}
+
+int qphix_amd_eo_op_destroy(qphix_amd_eo_op *op)
+{
+  if (op) (void)hipDeviceSynchronize();
+  delete op;
+  return QPHIX_AMD_OK;
+}
+
+int qphix_amd_eo_op_apply(qphix_amd_eo_op *op, int prec, int dagger, void *out, const void *in,
+                          void *stream)
+{
+  if (!op || !out || !in || out == in) return QPHIX_AMD_EINVAL;
+  int lvl;
+  if (prec == QPHIX_AMD_F64)
+    lvl = 0;
+  else if (prec == op->prec_s)
+    lvl = 1;
+  else
+    return QPHIX_AMD_EINVAL;
+  return eo_apply(op, lvl, dagger ? 1 : 0, out, in, (hipStream_t)stream) == hipSuccess ? QPHIX_AMD_OK
+                                                                                       : QPHIX_AMD_EHIP;
+}
+
+int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_amd_cg_params *prm,
+                       qphix_amd_cg_result *res, void *stream)
+{
+  if (!op || !x || !b || !prm || !res || x == b) return QPHIX_AMD_EINVAL;
+  if (!(prm->rsd_target > 0.0) || prm->max_iters <= 0) return QPHIX_AMD_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  auto t0 = std::chrono::steady_clock::now();
+  const long vcb = op->ctx->g.vcb;
+  const long n = 24 * vcb;
+  const bool mixed = prm->mixed != 0;
+  const int ps = mixed ? op->prec_s : QPHIX_AMD_F64;
+  const int px = ps == QPHIX_AMD_F64 ? QPHIX_AMD_F64 : QPHIX_AMD_F32;  // increment accumulator
+  const int lvl = mixed ? 1 : 0;
+  const double delta = prm->delta > 0.0 ? prm->delta : 0.1;
+
+  DevBuf r64, ax64, rs, p, ap, xs, mid, scb, part, host;
+  if (r64.alloc(n * 8) != hipSuccess || ax64.alloc(n * 8) != hipSuccess ||
+      rs.alloc(n * sbytes(ps)) != hipSuccess || p.alloc(n * sbytes(ps)) != hipSuccess ||
+      ap.alloc(n * sbytes(ps)) != hipSuccess || xs.alloc(n * sbytes(px)) != hipSuccess ||
+      mid.alloc(n * (size_t)sbytes(QPHIX_AMD_F64)) != hipSuccess ||
+      scb.alloc(sizeof(CGScalars)) != hipSuccess || part.alloc(kMaxPartials * 8) != hipSuccess)
+    return QPHIX_AMD_ENOMEM;
+  CGScalars *sc = (CGScalars *)scb.p;
+  double *partial = (double *)part.p;
+  CGScalars *hsc = nullptr;
+  if (hipHostMalloc((void **)&hsc, sizeof(CGScalars), hipHostMallocDefault) != hipSuccess)
+    return QPHIX_AMD_ENOMEM;
+  struct HostFree {
+    CGScalars *p;
+    ~HostFree() { (void)hipHostFree(p); }
+  } hf{hsc};
+
+  auto fetch = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+  };
+  auto set_scalars = [&](double rr, double beta) -> hipError_t {
+    CGScalars h{};
+    h.rr = rr;
+    h.beta = beta;
+    hipError_t e = hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // h is on the stack
+    return e;
+  };
+  // r64 = b - M^dag M x64, returns |r64|^2 in *r2
+  auto true_residual = [&](double *r2) -> hipError_t {
+    hipError_t e = eo_normal(op, 0, ax64.p, mid.p, x, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(r64.p, b, n * 8, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+    blas_axpby(QPHIX_AMD_F64, ax64.p, QPHIX_AMD_F64, r64.p, -1.0, 1.0, n, partial, sc, st);
+    e = fetch();
+    *r2 = hsc->tmp;
+    return e;
+  };
+
+  long mv = 0;
+  int iters = 0, updates = 0;
+  hipError_t e = hipMemsetAsync(x, 0, n * 8, st);
+  if (e == hipSuccess) e = hipMemsetAsync(xs.p, 0, n * sbytes(px), st);
+  // |b|^2
+  blas_axpby(QPHIX_AMD_F64, b, QPHIX_AMD_F64, r64.p, 1.0, 0.0, n, partial, sc, st);
+  if (e == hipSuccess) e = fetch();
+  if (e != hipSuccess) return QPHIX_AMD_EHIP;
+  const double b2 = hsc->tmp;
+  const double target2 = prm->rsd_target * prm->rsd_target * b2;
+  double r2 = b2;
+  // sloppy units: r_s = r / sigma
+  double sigma = mixed ? std::sqrt(r2 / (double)n) : 1.0;
+  if (!(sigma > 0.0)) sigma = 1.0;
+  blas_axpby(QPHIX_AMD_F64, r64.p, ps, rs.p, 1.0 / sigma, 0.0, n, partial, nullptr, st);
+  blas_axpby(ps, rs.p, ps, p.p, 1.0, 0.0, n, partial, nullptr, st);
+  e = set_scalars(r2 / (sigma * sigma), 0.0);
+  double rmax = std::sqrt(r2);  // max true-unit residual since the last update
+  bool converged = b2 == 0.0;
+  int rc = QPHIX_AMD_OK;
+  while (e == hipSuccess && !converged && iters < prm->max_iters) {
+    e = eo_normal(op, lvl, ap.p, mid.p, p.p, st);
+    mv += 2;
+    if (e != hipSuccess) break;
+    blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
+    blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
+    iters++;
+    e = fetch();
+    if (e != hipSuccess) break;
+    const double rr = hsc->rr;              // |r_s|^2
+    const double rest = std::sqrt(rr) * sigma;  // iterated residual, true units
+    if (!std::isfinite(rr)) {
+      rc = QPHIX_AMD_ENOCONV;
+      break;
+    }
+    const bool reliable = mixed ? (rest < delta * rmax || rest * rest < target2)
+                                : (rest * rest < target2);
+    if (reliable) {
+      // fold the increment into x, recompute the true residual in FP64
+      blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
+      e = hipMemsetAsync(xs.p, 0, n * sbytes(px), st);
+      if (e != hipSuccess) break;
+      e = true_residual(&r2);
+      if (e != hipSuccess) break;
+      updates++;
+      mv += 2;
+      if (r2 < target2) {
+        converged = true;
+        break;
+      }
+      const double sig_new = mixed ? std::sqrt(r2 / (double)n) : 1.0;
+      // beta from the true residual against the previous iterated one
+      const double beta = r2 / (hsc->rr_prev * sigma * sigma);
+      blas_axpby(QPHIX_AMD_F64, r64.p, ps, rs.p, 1.0 / sig_new, 0.0, n, partial, nullptr, st);
+      e = set_scalars(r2 / (sig_new * sig_new), beta * sigma / sig_new);
+      if (e != hipSuccess) break;
+      blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+      sigma = sig_new;
+      rmax = std::sqrt(r2);
+    } else {
+      blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+      if (rest > rmax) rmax = rest;
+    }
+  }
+  if (e == hipSuccess && !converged && rc == QPHIX_AMD_OK) {
+    // report the true residual of the final iterate
+    blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
+    e = true_residual(&r2);
+    converged = r2 < target2;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  auto t1 = std::chrono::steady_clock::now();
+  res->iters = iters;
+  res->reliable_updates = updates;
+  res->rsd_final = b2 > 0 ? std::sqrt(r2 / b2) : 0.0;
+  res->seconds = std::chrono::duration<double>(t1 - t0).count();
+  res->mv_apps = mv;
+  if (e != hipSuccess) return QPHIX_AMD_EHIP;
+  if (rc != QPHIX_AMD_OK) return rc;
+  return converged ? QPHIX_AMD_OK : QPHIX_AMD_ENOCONV;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+"""GPU tests of the clover builder, the even-odd operator and the CG.
+
+* clover from links, its inverse and the twisted-mass blocks vs the FP64
+  oracle (1e-13 relative);
+* M and M^dag (FP64, FP32, FP16, BF16 storage) vs the oracle's composition
+  F_oo - 1/4 D_oe F_ee^-1 D_eo of its own dslash;
+* CG on M^dag M, FP64 and mixed precision (FP32/FP16/BF16 sloppy), to a
+  true relative residual of 1e-10 recomputed by the FP64 oracle on the CPU,
+  and mixed vs FP64 solution agreement.
+"""
+import numpy as np
+import pytest
+import torch
+
+from _util import cb_mask, on_cb
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+KAPPA = 0.1350
+ALPHA = 1.0 / (2 * KAPPA)
+CSW = 1.0
+
+
+@pytest.fixture(scope="module")
+def lat(qamd):
+    dims = (8, 8, 8, 8)
+    L = qamd.lattice
+    U = L.random_gauge(dims, 0.2, 2000)
+    b = L.random_spinor(dims, 2001)
+    return dims, U, b
+
+
+def oracle_M(orc, dims, U, A, psi, dagger, mu=0.0, lat=None):
+    isign = -1 if dagger else 1
+    if mu == 0.0:
+        inv = orc.clover_invert(A)
+        t = orc.dslash(dims, psi, U, isign, 0, kind=orc.CLOVER, clov=inv)
+        return orc.dslash(dims, t, U, isign, 1, kind=orc.CLOVER, achi=True, chi=psi, clov=A,
+                          coeff_s=0.25, coeff_t_f=0.25, coeff_t_b=0.25)
+    B = orc.tmclover_blocks(A, mu, isign, True)
+    F = orc.tmclover_blocks(A, mu, isign, False)
+    t = orc.dslash(dims, psi, U, isign, 0, kind=orc.TMCLOVER, clov=B)
+    return orc.dslash(dims, t, U, isign, 1, kind=orc.TMCLOVER, achi=True, chi=psi, clov=F,
+                      coeff_s=0.25, coeff_t_f=0.25, coeff_t_b=0.25)
+
+
+def test_clover_build_vs_oracle(qamd, orc, lat):
+    dims, U, _ = lat
+    geom = qamd.Geometry(dims)
+    A = qamd.clover_from_links(geom, U.to(DEV), ALPHA, CSW).cpu().numpy()
+    ref = orc.clover_from_links(dims, U.numpy(), ALPHA, CSW)
+    assert np.abs(A - ref).max() / np.abs(ref).max() < 1e-13
+    inv = qamd.clover_invert(geom, torch.from_numpy(ref).to(DEV)).cpu().numpy()
+    rinv = orc.clover_invert(ref)
+    assert np.abs(inv - rinv).max() / np.abs(rinv).max() < 1e-13
+    for sign in (1, -1):
+        for invert in (0, 1):
+            bl = qamd.tmclover_blocks(geom, torch.from_numpy(ref).to(DEV), 0.1, sign, invert).cpu().numpy()
+            rb = orc.tmclover_blocks(ref, 0.1, sign, invert)
+            assert np.abs(bl - rb).max() / np.abs(rb).max() < 1e-13
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.1], ids=["clover", "tmclover"])
+def test_eo_operator_vs_oracle(qamd, orc, lat, mu):
+    dims, U, b = lat
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    An = A.cpu().numpy()
+    psi = on_cb(b.numpy(), dims, 1)
+    m = cb_mask(dims, 1)
+    for prec_s, tol in ((qamd.F32, 1e-6), (qamd.F16, 5e-3), (qamd.BF16, 4e-2)):
+        op = qamd.EvenOddCloverOperator(geom, Ud, A, mu=mu, prec_sloppy=prec_s, recon_sloppy=12)
+        for prec, tl in ((qamd.F64, 1e-13), (prec_s, tol)):
+            for dagger in (False, True):
+                src = geom.pack_spinor(b.to(DEV), prec, 1)
+                out = geom.new_spinor(prec, cb=1)
+                op.apply(out, src, dagger)
+                got = geom.unpack_spinor(out).cpu().numpy()
+                ref = oracle_M(orc, dims, U.numpy(), An, psi, dagger, mu)
+                err = np.abs(got[m] - ref[m]).max() / np.abs(ref[m]).max()
+                assert err < tl, (prec, dagger, err)
+        op.close()
+
+
+def _true_residual(orc, dims, U, A, x, b, mu):
+    y = oracle_M(orc, dims, U, A, x, False, mu)
+    y = oracle_M(orc, dims, U, A, on_cb(y, dims, 1), True, mu)
+    m = cb_mask(dims, 1)
+    return np.linalg.norm((b - y)[m]) / np.linalg.norm(b[m])
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.1], ids=["clover", "tmclover"])
+def test_cg_fp64_and_mixed(qamd, orc, lat, mu):
+    dims, U, b = lat
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    An = A.cpu().numpy()
+    bn = on_cb(b.numpy(), dims, 1)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+    sols = {}
+    for prec_s in (qamd.F64, qamd.F32, qamd.F16, qamd.BF16):
+        op = qamd.EvenOddCloverOperator(geom, Ud, A, mu=mu, prec_sloppy=prec_s, recon_sloppy=12)
+        x = geom.new_spinor(qamd.F64, cb=1)
+        res = qamd.InvCG(op, max_iters=5000)(x, bf, 1e-10, mixed=prec_s != qamd.F64)
+        print(f"mu={mu} sloppy={qamd.PREC_NAME[prec_s]}: {res}", flush=True)
+        assert res["rsd_final"] < 1e-10
+        xn = geom.unpack_spinor(x).cpu().numpy()
+        r = _true_residual(orc, dims, U.numpy(), An, xn, bn, mu)
+        assert r < 1.5e-10, (prec_s, r)
+        sols[prec_s] = xn
+        op.close()
+    ref = sols[qamd.F64]
+    for prec_s, xs in sols.items():
+        d = np.linalg.norm(xs - ref) / np.linalg.norm(ref)
+        assert d < 1e-8, (prec_s, d)
+
+
+def test_cg_reports_nonconvergence(qamd, lat):
+    dims, U, b = lat
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    op = qamd.EvenOddCloverOperator(geom, Ud, A, prec_sloppy=qamd.F16)
+    x = geom.new_spinor(qamd.F64, cb=1)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+    res = qamd.InvCG(op, max_iters=5)(x, bf, 1e-10, mixed=True, raise_on_noconv=False)
+    assert res["status"] == -5 and res["iters"] == 5 and res["rsd_final"] > 1e-10
