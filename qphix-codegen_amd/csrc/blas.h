@@ -16,8 +16,8 @@ struct CGScalars {
 
 constexpr int kMaxPartials = 1024;
 
-int blas_grid(long ngroups);
-// Re<a, b> over n reals (n % 8 == 0); kind: 0 -> alpha = rr / result,
+int blas_grid(long nsites);
+// Re<a, b> over n = 24 * vcb reals (whole sites); kind: 0 -> alpha = rr / result,
 // 2 -> sc->tmp = result.
 void blas_dot(int prec, const void *a, const void *b, long n, double *partial, CGScalars *sc,
               int kind, hipStream_t st);

@@ -1,7 +1,8 @@
-// blas.hip -- streaming vector kernels for the solvers: every vector is a
-// native-layout spinor on one checkerboard, viewed as a flat array of
-// n = 24 * vcb reals in its storage type.  Each thread handles groups of 8
-// reals (one 16-byte load for FP16/BF16, two for FP32, four for FP64).
+// blas.hip -- streaming vector kernels for the solvers.  Every vector is a
+// native-layout spinor on one checkerboard (n = 24 * vcb reals); a thread
+// handles whole sites, loading each operand through its own layout, so the
+// storage types of the operands may differ (the native vector width depends
+// on the storage type).
 //
 // Reductions are FP64 and deterministic: every work-group writes one
 // partial sum, and a single work-group adds the partials in a fixed order.
@@ -15,37 +16,21 @@ namespace {
 
 constexpr int kBlasBlock = 256;
 
-// 8 consecutive reals starting at element 8*g, as 16-byte loads/stores
+// all 24 reals of one site, each operand in its own native layout
 template <typename S, typename R>
-__device__ __forceinline__ void ld8(const S *p, long g, R v[8])
+__device__ __forceinline__ void lds(const S *p, int vcb, int site, R v[24])
 {
-  constexpr int W = 16 / (int)sizeof(S) < 8 ? 16 / (int)sizeof(S) : 8;
-  const u32x4 *vp = reinterpret_cast<const u32x4 *>(p + g * 8);
-#pragma unroll
-  for (int c = 0; c < 8 / W; c++) {
-    u32x4 x = vp[c];
-    S tmp[W];
-    __builtin_memcpy(tmp, &x, 16);
-#pragma unroll
-    for (int e = 0; e < W; e++) v[c * W + e] = (R)to_real(tmp[e]);
-  }
+  load_reals<S, pick_w<S>(24), 0, 24>(p, vcb, site, v);
 }
 
 template <typename S, typename R>
-__device__ __forceinline__ void st8(S *p, long g, const R v[8])
+__device__ __forceinline__ void sts(S *p, int vcb, int site, const R v[24])
 {
-  constexpr int W = 16 / (int)sizeof(S) < 8 ? 16 / (int)sizeof(S) : 8;
-  u32x4 *vp = reinterpret_cast<u32x4 *>(p + g * 8);
-#pragma unroll
-  for (int c = 0; c < 8 / W; c++) {
-    S tmp[W];
-#pragma unroll
-    for (int e = 0; e < W; e++) tmp[e] = to_storage<S>(v[c * W + e]);
-    u32x4 x;
-    __builtin_memcpy(&x, tmp, 16);
-    vp[c] = x;
-  }
+  store_reals<S, pick_w<S>(24), 0, 24>(p, vcb, site, v);
 }
+
+#define QAMD_SITE_LOOP(site, vcb) \
+  for (int site = blockIdx.x * kBlasBlock + threadIdx.x; site < (vcb); site += gridDim.x * kBlasBlock)
 
 // work-group sum of a double, result valid in thread 0
 __device__ __forceinline__ double block_sum(double v)
@@ -62,19 +47,19 @@ __device__ __forceinline__ double block_sum(double v)
 }
 
 template <typename S>
-__global__ __launch_bounds__(kBlasBlock) void dot_k(const S *a, const S *b, long ngroups,
+__global__ __launch_bounds__(kBlasBlock) void dot_k(const S *a, const S *b, int vcb,
                                                     double *partial)
 {
   using R = typename Traits<S>::R;
   double acc = 0.0;
-  for (long g = (long)blockIdx.x * kBlasBlock + threadIdx.x; g < ngroups;
-       g += (long)gridDim.x * kBlasBlock) {
-    R x[8], y[8];
-    ld8(a, g, x);
-    ld8(b, g, y);
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    R x[24], y[24];
+    lds(a, vcb, site, x);
+    lds(b, vcb, site, y);
     R s = 0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) s += x[k] * y[k];
+    for (int k = 0; k < 24; k++) s += x[k] * y[k];
     acc += (double)s;
   }
   const double t = block_sum(acc);
@@ -84,26 +69,27 @@ __global__ __launch_bounds__(kBlasBlock) void dot_k(const S *a, const S *b, long
 // y = a*x + b*y (a, b from the host), optional |y|^2
 template <typename SX, typename SY>
 __global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *x, SY *y, double a, double b,
-                                                      long ngroups, double *partial)
+                                                      int vcb, double *partial)
 {
   using R = typename Traits<SY>::R;
   double acc = 0.0;
-  for (long g = (long)blockIdx.x * kBlasBlock + threadIdx.x; g < ngroups;
-       g += (long)gridDim.x * kBlasBlock) {
-    double xv[8], yv[8];
-    ld8(x, g, xv);
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    double xv[24], yv[24];
+    lds(x, vcb, site, xv);
     if (b != 0.0)
-      ld8(y, g, yv);
+      lds(y, vcb, site, yv);
     else
 #pragma unroll
-      for (int k = 0; k < 8; k++) yv[k] = 0.0;
-    R o[8];
+      for (int k = 0; k < 24; k++) yv[k] = 0.0;
+    R o[24];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      o[k] = (R)(a * xv[k] + b * yv[k]);
-      acc += (double)o[k] * (double)o[k];
-    }
-    st8(y, g, o);
+    for (int k = 0; k < 24; k++) o[k] = (R)(a * xv[k] + b * yv[k]);
+    sts(y, vcb, site, o);
+    // norm of the stored (rounded) values
+    lds(y, vcb, site, xv);
+#pragma unroll
+    for (int k = 0; k < 24; k++) acc += xv[k] * xv[k];
   }
   if (partial) {
     const double t = block_sum(acc);
@@ -114,32 +100,33 @@ __global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *x, SY *y, double
 // CG step: r -= alpha Ap; x += alpha p; partial |r|^2.  alpha = sc->alpha.
 template <typename S, typename SXS>
 __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *r, SXS *x, const S *p, const S *Ap,
-                                                          const CGScalars *sc, long ngroups,
+                                                          const CGScalars *sc, int vcb,
                                                           double *partial)
 {
   using R = typename Traits<S>::R;
+  using RX = typename Traits<SXS>::R;
   const R alpha = (R)sc->alpha;
   double acc = 0.0;
-  for (long g = (long)blockIdx.x * kBlasBlock + threadIdx.x; g < ngroups;
-       g += (long)gridDim.x * kBlasBlock) {
-    R rv[8], pv[8], av[8];
-    ld8(r, g, rv);
-    ld8(p, g, pv);
-    ld8(Ap, g, av);
-    typename Traits<SXS>::R xv[8];
-    ld8(x, g, xv);
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    R rv[24], pv[24], av[24];
+    RX xv[24];
+    lds(r, vcb, site, rv);
+    lds(p, vcb, site, pv);
+    lds(Ap, vcb, site, av);
+    lds(x, vcb, site, xv);
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+      rv[k] = rv[k] - alpha * av[k];
+      xv[k] = xv[k] + (RX)(alpha * pv[k]);
+    }
+    sts(r, vcb, site, rv);
+    sts(x, vcb, site, xv);
+    // norm of the stored (rounded) residual
+    lds(r, vcb, site, rv);
     R s = 0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      rv[k] = rv[k] - alpha * av[k];
-      xv[k] = xv[k] + (typename Traits<SXS>::R)(alpha * pv[k]);
-    }
-    st8(r, g, rv);
-    st8(x, g, xv);
-    // norm of the stored (rounded) residual
-    ld8(r, g, rv);
-#pragma unroll
-    for (int k = 0; k < 8; k++) s += rv[k] * rv[k];
+    for (int k = 0; k < 24; k++) s += rv[k] * rv[k];
     acc += (double)s;
   }
   const double t = block_sum(acc);
@@ -148,19 +135,18 @@ __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *r, SXS *x, const S 
 
 // p = r + beta p, beta = sc->beta
 template <typename S>
-__global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *p, const S *r, const CGScalars *sc,
-                                                        long ngroups)
+__global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *p, const S *r, const CGScalars *sc, int vcb)
 {
   using R = typename Traits<S>::R;
   const R beta = (R)sc->beta;
-  for (long g = (long)blockIdx.x * kBlasBlock + threadIdx.x; g < ngroups;
-       g += (long)gridDim.x * kBlasBlock) {
-    R rv[8], pv[8];
-    ld8(r, g, rv);
-    ld8(p, g, pv);
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    R rv[24], pv[24];
+    lds(r, vcb, site, rv);
+    lds(p, vcb, site, pv);
 #pragma unroll
-    for (int k = 0; k < 8; k++) pv[k] = rv[k] + beta * pv[k];
-    st8(p, g, pv);
+    for (int k = 0; k < 24; k++) pv[k] = rv[k] + beta * pv[k];
+    sts(p, vcb, site, pv);
   }
 }
 
@@ -193,9 +179,9 @@ template <typename S> S *mp(void *p) { return static_cast<S *>(p); }
 
 }  // namespace
 
-int blas_grid(long ngroups)
+int blas_grid(long nsites)
 {
-  long b = (ngroups + kBlasBlock - 1) / kBlasBlock;
+  long b = (nsites + kBlasBlock - 1) / kBlasBlock;
   return (int)(b < kMaxPartials ? b : kMaxPartials);
 }
 
@@ -210,7 +196,7 @@ int blas_grid(long ngroups)
 void blas_dot(int prec, const void *a, const void *b, long n, double *partial, CGScalars *sc,
               int kind, hipStream_t st)
 {
-  const long ng = n / 8;
+  const int ng = (int)(n / 24);  // sites
   const int grid = blas_grid(ng);
 #define QAMD_DOT(T) \
   hipLaunchKernelGGL(dot_k<T>, dim3(grid), dim3(kBlasBlock), 0, st, cp<T>(a), cp<T>(b), ng, partial)
@@ -222,7 +208,7 @@ void blas_dot(int prec, const void *a, const void *b, long n, double *partial, C
 void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double b, long n,
                 double *partial, CGScalars *sc, hipStream_t st)
 {
-  const long ng = n / 8;
+  const int ng = (int)(n / 24);  // sites
   const int grid = blas_grid(ng);
   double *pp = sc ? partial : nullptr;
 #define QAMD_AXPBY_Y(TY)                                                                    \
@@ -252,7 +238,7 @@ void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double
 void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const void *Ap, long n,
                     double *partial, CGScalars *sc, hipStream_t st)
 {
-  const long ng = n / 8;
+  const int ng = (int)(n / 24);  // sites
   const int grid = blas_grid(ng);
 #define QAMD_CGU(T)                                                                          \
   if (prec_x == QPHIX_AMD_F64)                                                               \
@@ -268,7 +254,7 @@ void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const
 
 void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st)
 {
-  const long ng = n / 8;
+  const int ng = (int)(n / 24);  // sites
   const int grid = blas_grid(ng);
 #define QAMD_XPAY(T) \
   hipLaunchKernelGGL(cg_xpay_k<T>, dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(p), cp<T>(r), sc, ng)
