@@ -10,8 +10,10 @@ clover blocks and their FP64-computed inverses) and resident in HBM before
 timing starts; two input/output sets alternate so the Infinity Cache cannot
 hold a step's working set.
 
-Prints one JSON line (metric, value = GFLOP/s of the step, roofline of the
-dominant kernel from HIP-event timing, CPU baseline from the FP64 oracle).
+Prints one JSON line: metric, value = GFLOP/s of the step, the roofline of
+the dominant kernel from HIP-event timing, "cg" = FP64 vs mixed-precision CG
+time to a true residual of 1e-10 (config 2), and the CPU baseline (the
+reference's own AVX-512/AVX2 kernel on the host cores).
 """
 import argparse
 import json
@@ -49,34 +51,104 @@ def time_kernel(fn, n):
     return s.elapsed_time(e) / n * 1e-3
 
 
+def _host_isa():
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return None
+    if "avx512f" in flags:
+        return "avx512_f32"
+    if "avx2" in flags and "fma" in flags:
+        return "avx2_f32"
+    return None
+
+
 def cpu_baseline(seconds=10.0):
-    """FP64 oracle (scalar C, one thread) on a bounded sample of the same
-    workload: the clover dslash at sampled sites of a 32^3x64 lattice."""
+    """The reference's own generated FP32 clover kernel (oracle/_ref, built
+    from the qphix-codegen sources) under an OpenMP block loop on the host
+    cores, on a bounded slab of the workload: 32^3 x 8 (same per-site work as
+    32^3 x 64).  Falls back to the FP64 C oracle (one thread) if the
+    reference build is absent."""
     import numpy as np
-    import torch
-    from oracle import oracle as orc
+    from oracle import ref
     import __graft_entry__ as ge
     q = ge.load_package()
     L = q.lattice
-    # a smaller lattice with identical per-site work keeps host memory small
+    cfg = _host_isa()
+    if cfg and ref.available(cfg):
+        dims = (32, 32, 32, 8)
+        U = L.random_gauge(dims, 0.3, 5).numpy()
+        psi = L.random_spinor(dims, 6).numpy()
+        inv = L.invert_clover(L.random_clover(dims, 7)).numpy()
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        vcb = U.shape[0] // 2
+        _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=3)
+        reps = max(3, int(seconds / max(sec, 1e-6)))
+        _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=reps)
+        return {"value": round(vcb * FLOP_CLOVER / sec / 1e9, 2), "unit": "GFLOP/s", "cores": nthr,
+                "kind": "reference",
+                "sample": f"reference {cfg} clov_dslash (VECLEN=SOALEN={ref.soalen(cfg)}, 12-real links) "
+                          f"on 32^3x8 ({vcb} sites/cb), {reps} calls x {sec * 1e3:.2f} ms, "
+                          f"OpenMP {nthr} threads"}
+    from oracle import oracle as orc
     dims = (16, 16, 16, 32)
     U = L.random_gauge(dims, 0.3, 5).numpy()
     psi = L.random_spinor(dims, 6).numpy()
     clov = L.invert_clover(L.random_clover(dims, 7)).numpy()
-    V = U.shape[0]
-    sites = np.arange(0, V, 2, dtype=np.int64)
-    done, t0 = 0, time.perf_counter()
-    n = 2048
+    sites = np.arange(0, U.shape[0], 2, dtype=np.int64)
+    done, t0, n = 0, time.perf_counter(), 2048
     while time.perf_counter() - t0 < seconds:
         chunk = sites[(done % len(sites)):(done % len(sites)) + n]
         orc.dslash_sites(dims, chunk, psi, U, 1, kind=orc.CLOVER, clov=clov)
         done += len(chunk)
     dt = time.perf_counter() - t0
-    del torch
     return {"value": round(done * FLOP_CLOVER / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
             "kind": "port",
-            "sample": f"FP64 C oracle, clover dslash at {done} sites of a 16^3x32 lattice "
-                      f"(same per-site work as 32^3x64), {dt:.1f} s, 1 thread"}
+            "sample": f"FP64 C oracle, clover dslash at {done} sites of a 16^3x32 lattice, "
+                      f"{dt:.1f} s, 1 thread"}
+
+
+def profile_traffic():
+    """HBM bytes per clov_dslash launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE x its copy-kernel calibration + WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", "clov_dslash_f32_32x64_pmc.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cg_compare(q, dev):
+    """Config 2: even-odd Wilson-clover M^dag M at 32^3x64, kappa = 0.1350,
+    csw = 1.0, links sigma = 0.2; FP64 CG vs mixed-precision reliable-update
+    CG (FP16 and BF16 storage, FP32 arithmetic, FP64 true residuals) to a
+    true relative residual of 1e-10 on the same Gaussian right-hand side."""
+    import torch
+    L = q.lattice
+    geom = q.Geometry(DIMS, t_boundary=-1.0)
+    U = L.random_gauge(DIMS, 0.2, 3000, device=dev)
+    b = L.random_spinor(DIMS, 3001, device=dev)
+    A = q.clover_from_links(geom, U, 1.0 / (2 * 0.1350), 1.0)
+    bf = geom.pack_spinor(b, q.F64, 1)
+    out, sols = {}, {}
+    for name, prec in (("f64", q.F64), ("f16", q.F16), ("bf16", q.BF16)):
+        op = q.EvenOddCloverOperator(geom, U, A, prec_sloppy=prec, recon_sloppy=12)
+        x = geom.new_spinor(q.F64, cb=1)
+        res = q.InvCG(op, max_iters=20000)(x, bf, 1e-10, mixed=prec != q.F64)
+        sols[name] = x.data
+        out[name] = {"iters": res["iters"], "seconds": round(res["seconds"], 4),
+                     "reliable_updates": res["reliable_updates"], "rsd_final": res["rsd_final"]}
+        op.close()
+    for name in ("f16", "bf16"):
+        out[name]["speedup_vs_f64"] = round(out["f64"]["seconds"] / out[name]["seconds"], 3)
+        out[name]["rel_err_vs_f64_solution"] = float(
+            ((sols[name] - sols["f64"]).norm() / sols["f64"].norm()).item())
+    del sols
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -85,6 +157,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="launch each operator kernel 10 times plus a 1 GiB device copy (for "
+                         "rocprofv3 --pmc runs and their copy-kernel calibration), print nothing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -123,6 +199,20 @@ def main():
         s = sets[i & 1]
         op.dslash(s["t0"], s["psi1"], s["g0"], s["inv0"], 1, 0)
         op.dslashAChiMinusBDPsi(s["out1"], s["t0"], s["psi1"], s["g1"], s["clov1"], beta, 1, 1)
+
+    if args.kernels_only:
+        for i in range(10):
+            op.dslash(sets[i & 1]["t0"], sets[i & 1]["psi1"], sets[i & 1]["g0"], sets[i & 1]["inv0"], 1, 0)
+        for i in range(10):
+            s = sets[i & 1]
+            op.dslashAChiMinusBDPsi(s["out1"], s["t0"], s["psi1"], s["g1"], s["clov1"], 0.25, 1, 1)
+        del sets
+        a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        return
 
     for i in range(args.warmup):
         step(i)
@@ -173,11 +263,17 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": profile_traffic(),
                      "kernel": "clov_dslash f32 recon12",
                      "algorithmic_bytes_per_launch": b_d},
         "cpu_baseline": None,
+        "cg": None,
     }
+    del sets
+    torch.cuda.empty_cache()
+    if not args.no_cg:
+        line["cg"] = cg_compare(q, dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
