@@ -176,6 +176,29 @@ int qphix_amd_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
                                         double coeff_t_f, double coeff_t_b,
                                         double prec_mass_rho, void *stream);
 
+/* ---- differentiation (no counterpart in the reference) ------------------
+   For L = Re<chi, D psi>, D = the hop above with output checkerboard cb:
+   dL/dpsi = D^dagger chi.  D_isign^dagger = D_{-isign} with coeff_t_f and
+   coeff_t_b exchanged (gamma_mu Hermitian), landing on checkerboard 1-cb;
+   `gauge` must be packed for checkerboard 1-cb.                         */
+int qphix_amd_dslash_adjoint(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb,
+                             void *grad_psi, const void *chi, const void *gauge,
+                             double coeff_s, double coeff_t_f, double coeff_t_b,
+                             void *stream);
+/* Link gradient of L: accumulates scale * (per link) into the FP64
+   lexicographic field force_lex [V][4][3][3][2]:
+     project = 1: TA(U G), the traceless anti-Hermitian su(3) force, with
+                  d/de L(exp(i e T) U) = Re tr(i T F) for Hermitian T;
+     project = 0: G^dagger, the gradient with respect to the real and
+                  imaginary parts of the link entries;
+   where L = sum over links Re tr(U G) (outer products of psi and the
+   projected chi).  chi, psi: native spinors (FP64 or FP32) on cb, 1-cb;
+   u_lex: FP64 lexicographic links.                                      */
+int qphix_amd_dslash_force(qphix_amd_ctx *ctx, int prec, int isign, int cb, double *force_lex,
+                           const void *chi, const void *psi, const double *u_lex,
+                           double coeff_s, double coeff_t_f, double coeff_t_b, double scale,
+                           int project, void *stream);
+
 /* ---- clover term from links (no counterpart in the reference, whose
    kernels take A or A^-1 as input) ------------------------------------
    A(x) = alpha + (csw/2) sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),

@@ -42,6 +42,7 @@ def lib():
         l.oracle_dslash_sites.argtypes = [_dp, ctypes.POINTER(ctypes.c_long), ctypes.c_long, _dp, _dp,
                                           _dp, _dp, i, i, i, i, i, i, i, d, d, d, d, _ip, _dp, d, d,
                                           d, d]
+        l.oracle_force.argtypes = [_dp, _dp, _dp, _dp, i, i, i, i, i, i, d, d, d, d, _ip, _dp, i]
         l.oracle_clover_apply.argtypes = [_dp, _dp, _dp, i, i, i, i, i, i]
         l.oracle_clover_from_links.argtypes = [_dp, _dp, i, i, i, i, d, d]
         l.oracle_clover_invert.argtypes = [_dp, _dp, ctypes.c_long]
@@ -104,6 +105,23 @@ def dslash_sites(dims, sites, psi, U, isign, kind=WILSON, achi=False, chi=None, 
     lib().oracle_dslash_sites(po, sites.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), len(sites),
                               pp, pc, pu, pcl, kind, int(achi), Lx, Ly, Lz, Lt, isign, coeff_s,
                               coeff_t_f, coeff_t_b, t_boundary, t, p, alpha, mu, muinv, rho)
+    return out
+
+
+def force(dims, chi, psi, U, isign, cb, coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.0, t_boundary=-1.0,
+          tbc=None, tbc_phases=None, project=True):
+    """Link gradient of L = Re<chi, hop psi> (chi read on cb, psi on 1-cb);
+    (V,4,3,3,2): TA(U G) if project else G^dagger."""
+    Lx, Ly, Lz, Lt = dims
+    V = Lx * Ly * Lz * Lt
+    chi, pc = _c(chi)
+    psi, pp = _c(psi)
+    U, pu = _c(U)
+    out = np.zeros((V, 4, 3, 3, 2))
+    out, po = _c(out)
+    t, p = _bc(tbc, tbc_phases)
+    lib().oracle_force(po, pc, pp, pu, Lx, Ly, Lz, Lt, isign, cb, coeff_s, coeff_t_f, coeff_t_b,
+                       t_boundary, t, p, int(bool(project)))
     return out
 
 

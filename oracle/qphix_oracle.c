@@ -568,3 +568,93 @@ int oracle_tmclover_blocks(double *out, const double *clov, long nsites,
     }
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Gauge-link gradient of L = Re<chi, hop psi> (chi on cb, psi on 1-cb).
+ * No counterpart in the reference.  For every hop of every output site x,
+ * the hop term  beta ph_fb chi(x)^+ (1 -/+ s g_mu) V psi(n)  with
+ * V = U_mu(x) (forward) or U_mu(x-mu)^+ (backward) contributes to
+ * Re tr(U G) of the link it uses; G is accumulated per link here by
+ * expanding each term entry by entry.  Output per link [V][4][3][3][2]:
+ *   project = 1: TA(U G) (traceless anti-Hermitian, d/de L(e^{ieT}U) =
+ *                Re tr(i T F)),  project = 0: G^dagger.
+ * ---------------------------------------------------------------------- */
+void oracle_force(double *F, const double *chi, const double *psi, const double *U,
+                  int Lx, int Ly, int Lz, int Lt, int isign, int cb, double coeff_s,
+                  double coeff_t_f, double coeff_t_b, double t_boundary,
+                  const int *tbc, const double *tbc_phases, int project)
+{
+  lat_t g;
+  lat_init(&g, Lx, Ly, Lz, Lt);
+  init_gamma();
+  cplx *G = (cplx *)calloc((size_t)g.V * 4 * 9, sizeof(cplx));
+  for (long x = 0; x < g.V; x++) {
+    int c[4];
+    site_coords(&g, x, c);
+    if (((c[0] + c[1] + c[2] + c[3]) & 1) != cb) continue;
+    for (int mu = 0; mu < 4; mu++) {
+      cplx ph = 1.0;
+      if (tbc && tbc[mu]) ph = tbc_phases[2 * mu] + I * tbc_phases[2 * mu + 1];
+      for (int fb = 0; fb < 2; fb++) {
+        long n = neighbour(&g, x, mu, fb ? +1 : -1);
+        double beta;
+        if (mu < 3)
+          beta = coeff_s;
+        else if (fb)
+          beta = coeff_t_f * (c[3] == Lt - 1 ? t_boundary : 1.0);
+        else
+          beta = coeff_t_b * (c[3] == 0 ? t_boundary : 1.0);
+        double sg = fb ? -(double)isign : (double)isign;
+        /* eta = (1 + sg g_mu)^+ chi(x) = (1 + sg g_mu) chi(x) (g Hermitian) */
+        cplx eta[4][3];
+        for (int sp = 0; sp < 4; sp++)
+          for (int i = 0; i < 3; i++) {
+            cplx v = spin(chi, x, sp, i);
+            for (int k = 0; k < 4; k++) v += sg * gamma_mat[mu][sp][k] * spin(chi, x, k, i);
+            eta[sp][i] = v;
+          }
+        /* term = beta * sum_s sum_ij conj(eta_si) V_ij psi(n)_sj with
+           V = ph U_mu(x) (fb = 1) or conj(ph) U_mu(n)^+ (fb = 0).
+           fb = 1: term = tr(U_mu(x) Gx), Gx[j][i] += beta ph psi_j conj(eta_i)
+           fb = 0: V_ij = conj(ph) conj(U_mu(n)_ji):
+                   Re term = Re tr(U_mu(n) Gn), Gn[i][j] += beta ph eta_i conj(psi_j) */
+        long l = fb ? x : n;
+        for (int sp = 0; sp < 4; sp++)
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+              if (fb)
+                G[(l * 4 + mu) * 9 + j * 3 + i] += beta * ph * spin(psi, n, sp, j) * conj(eta[sp][i]);
+              else
+                G[(l * 4 + mu) * 9 + i * 3 + j] += beta * ph * eta[sp][i] * conj(spin(psi, n, sp, j));
+            }
+      }
+    }
+  }
+  for (long l = 0; l < g.V * 4; l++) {
+    cplx *Gl = G + l * 9;
+    double *out = F + l * 18;
+    if (!project) {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) st(out + (i * 3 + j) * 2, conj(Gl[j * 3 + i]));
+      continue;
+    }
+    long s = l / 4;
+    int mu = (int)(l % 4);
+    cplx W[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        cplx v = 0;
+        for (int k = 0; k < 3; k++) v += link(U, s, mu, i, k) * Gl[k * 3 + j];
+        W[i][j] = v;
+      }
+    cplx tr = 0;
+    for (int i = 0; i < 3; i++) tr += W[i][i] - conj(W[i][i]);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        cplx v = 0.5 * (W[i][j] - conj(W[j][i]));
+        if (i == j) v -= tr / 6.0;
+        st(out + (i * 3 + j) * 2, v);
+      }
+  }
+  free(G);
+}

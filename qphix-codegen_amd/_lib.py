@@ -57,6 +57,10 @@ SIGNATURES = {
     "qphix_amd_tm_clov_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                                      _vp, _vp, _c_double, _c_double, _c_double, _c_double,
                                                      _vp]),
+    "qphix_amd_dslash_adjoint": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                          _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_dslash_force": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_double,
+                                        _c_double, _c_double, _c_double, _c_int, _vp]),
     "qphix_amd_clover_from_links": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _vp]),
     "qphix_amd_clover_invert": (_c_int, [_vp, _vp, _vp, _vp]),
     "qphix_amd_tmclover_blocks": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _vp]),

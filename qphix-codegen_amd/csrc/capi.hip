@@ -206,6 +206,17 @@ int qphix_amd_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb,
   return run(ctx, c, stream);
 }
 
+int qphix_amd_dslash_adjoint(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb,
+                             void *grad_psi, const void *chi, const void *gauge, double coeff_s,
+                             double coeff_t_f, double coeff_t_b, void *stream)
+{
+  if (cb != 0 && cb != 1) return QPHIX_AMD_EINVAL;
+  // (D_s)^dagger = D_{-s} on the other checkerboard, time coefficients swapped
+  DslashCall c = make_call(prec, recon, qamd::OP_WILSON, false, -isign, 1 - cb, grad_psi, chi,
+                           nullptr, gauge, nullptr, coeff_s, coeff_t_b, coeff_t_f);
+  return run(ctx, c, stream);
+}
+
 int qphix_amd_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb,
                                 void *out, const void *in, const void *chi, const void *gauge,
                                 double alpha, double coeff_s, double coeff_t_f, double coeff_t_b,

@@ -1,0 +1,177 @@
+"""GPU tests of the differentiation kernels (config 3 semantics).
+
+* the link-gradient kernel (qphix_amd_dslash_force) vs the FP64 oracle
+  (FP64 1e-13, FP32 1e-6 relative), projected and Euclidean, both isigns
+  and checkerboards, twisted boundary phases;
+* dL/dpsi = D^dagger chi (qphix_amd_dslash_adjoint) vs the oracle;
+* the torch.autograd binding: gradients of L = Re<chi, D psi> vs the oracle;
+* on 16^3 x 32, central finite differences (e = 1e-5, FP64) along 32 random
+  su(3) link perturbations of L = Re<chi, D_W psi> and of the pseudofermion
+  action S = phi^dag (M^dag M)^-1 phi (even-odd Wilson, FP64 CG), against
+  sum Re tr(i T F) from the kernels (1e-6 relative); the FP32 force against
+  the FP64 one.
+"""
+import numpy as np
+import pytest
+import torch
+
+from _util import cb_mask, cplx, on_cb, real
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+@pytest.fixture(scope="module")
+def small(qamd):
+    dims = (8, 8, 8, 8)
+    L = qamd.lattice
+    return dims, L.random_gauge(dims, 0.4, 61), L.random_spinor(dims, 62), L.random_spinor(dims, 63)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-6)], ids=["f64", "f32"])
+def test_force_kernel_vs_oracle(qamd, orc, small, prec, tol):
+    dims, U, chi, psi = small
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    ph = [np.exp(0.3j), 1.0, np.exp(-1.1j), np.exp(0.5j)]
+    geom.set_tbc([1, 0, 1, 1], ph)
+    Ud = U.to(DEV)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            c = geom.pack_spinor(chi.to(DEV), prec, cb)
+            p = geom.pack_spinor(psi.to(DEV), prec, 1 - cb)
+            for project in (True, False):
+                F = qamd.dslash_force(geom, torch.zeros_like(Ud), c, p, Ud, isign, cb, 1.0, project)
+                ref = orc.force(dims, on_cb(chi.numpy(), dims, cb), on_cb(psi.numpy(), dims, 1 - cb),
+                                U.numpy(), isign, cb, tbc=[1, 0, 1, 1], tbc_phases=ph, project=project)
+                err = np.abs(F.cpu().numpy() - ref).max() / np.abs(ref).max()
+                assert err < tol, (isign, cb, project, err)
+
+
+def test_adjoint_kernel_vs_oracle(qamd, orc, small):
+    dims, U, chi, _ = small
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    for isign in (1, -1):
+        for cb in (0, 1):
+            c = geom.pack_spinor(chi.to(DEV), 0, cb)
+            g = geom.pack_gauge(U.to(DEV), 0, 18, 1 - cb)
+            out = geom.new_spinor(0, cb=1 - cb)
+            lib = qamd.lib()
+            import ctypes
+            assert lib.qphix_amd_dslash_adjoint(geom.ctx, 0, 18, isign, cb, out.ptr(), c.ptr(), g.ptr(),
+                                                0.9, 1.3, 0.7, None) == 0
+            torch.cuda.synchronize()
+            got = geom.unpack_spinor(out).cpu().numpy()
+            # D^dag chi with D(coeff_t_f=1.3, coeff_t_b=0.7) = D_{-s}(coeff_t_f=0.7, coeff_t_b=1.3)
+            ref = orc.dslash(dims, on_cb(chi.numpy(), dims, cb), U.numpy(), -isign, 1 - cb,
+                             coeff_s=0.9, coeff_t_f=0.7, coeff_t_b=1.3)
+            m = cb_mask(dims, 1 - cb)
+            assert np.abs(got[m] - ref[m]).max() / np.abs(ref[m]).max() < 1e-13
+            # and it is the adjoint: <chi, D psi> = <D^dag chi, psi>
+            rng = np.random.default_rng(5 + isign + 3 * cb)
+            psi = on_cb(rng.normal(size=chi.shape), dims, 1 - cb)
+            d = orc.dslash(dims, psi, U.numpy(), isign, cb, coeff_s=0.9, coeff_t_f=1.3, coeff_t_b=0.7)
+            lhs = np.sum(cplx(on_cb(chi.numpy(), dims, cb)).conj() * cplx(d))
+            rhs = np.sum(cplx(got).conj() * cplx(psi))
+            assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+def test_autograd_dslash_vs_oracle(qamd, orc, small):
+    dims, U, chi, psi = small
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    cb, isign = 1, 1
+    Ud = U.to(DEV).requires_grad_(True)
+    pd = psi.to(DEV).requires_grad_(True)
+    out = qamd.DslashFunction.apply(pd, Ud, geom, isign, cb)
+    chid = torch.from_numpy(on_cb(chi.numpy(), dims, cb)).to(DEV)
+    L = (chid * out).sum()
+    L.backward()
+    ref_psi = orc.dslash(dims, on_cb(chi.numpy(), dims, cb), U.numpy(), -isign, 1 - cb)
+    ref_U = orc.force(dims, on_cb(chi.numpy(), dims, cb), on_cb(psi.numpy(), dims, 1 - cb), U.numpy(),
+                      isign, cb, project=False)
+    gp = pd.grad.cpu().numpy()
+    m = cb_mask(dims, 1 - cb)
+    assert np.abs(gp[m] - ref_psi[m]).max() / np.abs(ref_psi[m]).max() < 1e-13
+    assert np.abs(gp[~m]).max() == 0
+    assert np.abs(Ud.grad.cpu().numpy() - ref_U).max() / np.abs(ref_U).max() < 1e-13
+
+
+def _su3_dirs(qamd, V, n, seed):
+    g = torch.Generator().manual_seed(seed)
+    lam = qamd.lattice.gellmann(DEV)
+    for k in range(n):
+        w = torch.randn(V, 4, 8, generator=g, dtype=torch.float64).to(DEV)
+        yield torch.einsum("vma,aij->vmij", w.to(torch.complex128), lam) * 0.5
+
+
+def _rotate(U, T, eps):
+    Uc = torch.view_as_complex(U)
+    return torch.view_as_real((torch.linalg.matrix_exp(1j * eps * T) @ Uc).contiguous()).contiguous()
+
+
+def _directional(F, T):
+    Fc = torch.view_as_complex(F.contiguous())
+    return torch.real(1j * torch.einsum("vmij,vmji->", T, Fc)).item()
+
+
+def test_force_finite_differences_dslash_16x32(qamd):
+    dims = (16, 16, 16, 32)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    U = L.random_gauge(dims, 0.3, 71, DEV)
+    chi = L.random_spinor(dims, 72, DEV)
+    psi = L.random_spinor(dims, 73, DEV)
+    cb, isign = 0, 1
+    chic = geom.pack_spinor(chi, 0, cb)
+
+    def loss(Uv):
+        g = geom.pack_gauge(Uv, 0, 18, cb)
+        out = geom.new_spinor(0, cb=cb)
+        qamd.Dslash(geom).dslash(out, geom.pack_spinor(psi, 0, 1 - cb), g, isign, cb)
+        return torch.dot(chic.data, out.data).item()
+
+    F = qamd.su3_force_dslash(geom, chi, psi, U, isign, cb)
+    F32 = qamd.su3_force_dslash(geom, chi, psi, U, isign, cb, prec=1)
+    assert ((F32 - F).abs().max() / F.abs().max()).item() < 1e-6
+    eps = 1e-5
+    worst = 0.0
+    for T in _su3_dirs(qamd, U.shape[0], 32, 74):
+        fd = (loss(_rotate(U, T, eps)) - loss(_rotate(U, T, -eps))) / (2 * eps)
+        an = _directional(F, T)
+        worst = max(worst, abs(fd - an) / abs(an))
+    print("dslash force FD worst rel err", worst)
+    assert worst < 1e-6
+
+
+def test_force_finite_differences_pseudofermion_16x32(qamd):
+    dims = (16, 16, 16, 32)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    U = L.random_gauge(dims, 0.3, 81, DEV)
+    phi = L.random_spinor(dims, 82, DEV)
+    kappa, rsd = 0.12, 1e-14
+
+    def action(Uv):
+        return qamd.PseudofermionAction.apply(Uv, phi, geom, kappa, rsd).item()
+
+    F = qamd.su3_force_pseudofermion(geom, U, phi, kappa, rsd)
+    # the autograd (Euclidean) gradient is consistent with the projected force:
+    # TA(U G) with G^dag = dS/dU
+    Ur = U.clone().requires_grad_(True)
+    S = qamd.PseudofermionAction.apply(Ur, phi, geom, kappa, rsd)
+    S.backward()
+    Gd = torch.view_as_complex(Ur.grad.contiguous())
+    W = torch.view_as_complex(U) @ Gd.conj().transpose(-1, -2)
+    TA = 0.5 * (W - W.conj().transpose(-1, -2))
+    tr = torch.diagonal(TA, dim1=-2, dim2=-1).sum(-1) / 3
+    TA = TA - tr[..., None, None] * torch.eye(3, dtype=TA.dtype, device=DEV)
+    Fc = torch.view_as_complex(F.contiguous())
+    assert ((TA - Fc).abs().max() / Fc.abs().max()).item() < 1e-12
+    eps = 1e-5
+    worst = 0.0
+    for T in _su3_dirs(qamd, U.shape[0], 32, 84):
+        fd = (action(_rotate(U, T, eps)) - action(_rotate(U, T, -eps))) / (2 * eps)
+        an = _directional(F, T)
+        worst = max(worst, abs(fd - an) / abs(an))
+    print("pseudofermion force FD worst rel err", worst)
+    assert worst < 1e-6
