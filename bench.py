@@ -151,6 +151,83 @@ def cg_compare(q, dev):
     return out
 
 
+def tmclover48(args, q, dev):
+    """Config 4: 48^3x96 (5,308,416 sites/cb) twisted-mass clover, mu = 0.05,
+    kappa = 0.1355, csw = 1.5, links sigma = 0.2.
+      * dslash throughput in half storage (FP16 spinors, 12-real FP16 links,
+        FP16 full clover blocks, FP32 arithmetic) and in FP32;
+      * FP64 CG vs mixed-precision CG (FP16 storage) to a true residual of
+        1e-10 on M^dag M of the even-odd TM-clover operator;
+      * the hopping-term gauge force at the solution: from the mixed solve
+        vs from the FP64 solve (FP64 force kernels)."""
+    import torch
+    L = q.lattice
+    dims = (48, 48, 48, 96)
+    kappa, mu, csw = 0.1355, 0.05, 1.5
+    alpha = 1.0 / (2 * kappa)
+    geom = q.Geometry(dims, t_boundary=-1.0)
+    vcb = geom.vcb
+    U = L.random_gauge(dims, 0.2, 4000, device=dev)
+    A = q.clover_from_links(geom, U, alpha, csw)
+    phi = L.random_spinor(dims, 4001, device=dev)
+    out = {"workload": "48^3x96 twisted-mass clover, mu=0.05, kappa=0.1355, csw=1.5",
+           "sites_per_cb": vcb}
+    # --- dslash throughput
+    flop_d, flop_a = 1872, 1920
+    op_k = q.TMClovDslash(geom)
+    for name, prec in (("f16", q.F16), ("f32", q.F32)):
+        B = q.tmclover_blocks(geom, A, mu, 1, True)
+        F = q.tmclover_blocks(geom, A, mu, 1, False)
+        f = dict(g0=geom.pack_gauge(U, prec, 12, 0), g1=geom.pack_gauge(U, prec, 12, 1),
+                 B0=geom.pack_full_clover(B, prec, 0), F1=geom.pack_full_clover(F, prec, 1),
+                 psi=geom.pack_spinor(phi, prec, 1), t=geom.new_spinor(prec, cb=0),
+                 o=geom.new_spinor(prec, cb=1))
+        del B, F
+
+        def kd(i):
+            op_k.dslash(f["t"], f["psi"], f["g0"], f["B0"], 1, 0)
+
+        def ka(i):
+            op_k.dslashAChiMinusBDPsi(f["o"], f["t"], f["psi"], f["g1"], f["F1"], 0.25, 1, 1)
+
+        for k in (kd, ka):
+            for i in range(3):
+                k(i)
+        td, ta = time_kernel(kd, 50), time_kernel(ka, 50)
+        sb = 2 if prec == q.F16 else 4
+        bd = (24 + 96 + 144 + 24) * sb * vcb
+        ba = (24 + 96 + 144 + 24 + 24) * sb * vcb
+        out[f"dslash_{name}"] = {
+            "tm_clov_dslash_us": round(td * 1e6, 1), "tm_clov_dslash_gflops": round(vcb * flop_d / td / 1e9, 1),
+            "tm_clov_dslash_hbm_frac": round(bd / td / 1e9 / PEAK_HBM_GBS, 4),
+            "tm_clov_achimbdpsi_us": round(ta * 1e6, 1),
+            "tm_clov_achimbdpsi_gflops": round(vcb * flop_a / ta / 1e9, 1),
+            "tm_clov_achimbdpsi_hbm_frac": round(ba / ta / 1e9 / PEAK_HBM_GBS, 4)}
+        del f
+        torch.cuda.empty_cache()
+    # --- CG: FP64 vs mixed (FP16 storage), then the force at each solution
+    b = geom.pack_spinor(phi, q.F64, 1)
+    forces = {}
+    for name, prec in (("f64", q.F64), ("f16", q.F16)):
+        op = q.EvenOddCloverOperator(geom, U, A, mu=mu, prec_sloppy=prec, recon_sloppy=12)
+        x = geom.new_spinor(q.F64, cb=1)
+        res = q.InvCG(op, max_iters=20000)(x, b, 1e-10, mixed=prec != q.F64)
+        out[f"cg_{name}"] = {"iters": res["iters"], "seconds": round(res["seconds"], 4),
+                             "reliable_updates": res["reliable_updates"], "rsd_final": res["rsd_final"]}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        forces[name] = op.force(torch.zeros_like(U), x, U, project=True)
+        torch.cuda.synchronize()
+        out[f"cg_{name}"]["force_seconds"] = round(time.perf_counter() - t0, 4)
+        op.close()
+        del x
+        torch.cuda.empty_cache()
+    out["cg_f16"]["speedup_vs_f64"] = round(out["cg_f64"]["seconds"] / out["cg_f16"]["seconds"], 3)
+    out["force_rel_err_mixed_vs_f64"] = float(
+        ((forces["f16"] - forces["f64"]).abs().max() / forces["f64"].abs().max()).item())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +235,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--workload", default="clover32", choices=["clover32", "tmclover48"],
+                    help="clover32: the 32^3x64 clover operator (the headline line); "
+                         "tmclover48: the 48^3x96 twisted-mass clover case")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch each operator kernel 10 times plus a 1 GiB device copy (for "
                          "rocprofv3 --pmc runs and their copy-kernel calibration), print nothing")
@@ -173,6 +253,9 @@ def main():
     L = q.lattice
     dev = "cuda:0"
     torch.cuda.set_device(0)
+    if args.workload == "tmclover48":
+        print(json.dumps(tmclover48(args, q, dev)), flush=True)
+        return
     prec, recon = q.F32, 12
     geom = q.Geometry(DIMS, t_boundary=-1.0)
     vcb = geom.vcb

@@ -199,6 +199,11 @@ int qphix_amd_dslash_force(qphix_amd_ctx *ctx, int prec, int isign, int cb, doub
                            double coeff_s, double coeff_t_f, double coeff_t_b, double scale,
                            int project, void *stream);
 
+/* ---- link reunitarisation (FP64, in place, lexicographic links) --------
+   row 0 normalised, row 1 Gram-Schmidt against row 0 and normalised,
+   row 2 = conj(row0 x row1); no counterpart in the reference.           */
+int qphix_amd_reunitarize(qphix_amd_ctx *ctx, double *u_lex, void *stream);
+
 /* ---- clover term from links (no counterpart in the reference, whose
    kernels take A or A^-1 as input) ------------------------------------
    A(x) = alpha + (csw/2) sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
@@ -244,6 +249,17 @@ typedef struct {
   double seconds;       /* wall time of the solve (host clock)            */
   long mv_apps;         /* applications of M or M^dagger                 */
 } qphix_amd_cg_result;
+
+/* Hopping-term part of the gradient of S = phi^dag (M^dag M)^-1 phi at
+   x = (M^dag M)^-1 phi (FP64 native, cb 1):
+     dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>],  Y = M X,
+     Z = F_ee^-1 D_eo X,  W = (F_ee^-1)^dag D_oe^dag Y,
+   accumulated as scale * (per link) into force_lex like
+   qphix_amd_dslash_force (project = 1: su(3) force, 0: link gradient).
+   This is the whole gradient when F does not depend on the links (csw = 0);
+   the clover-term derivative is not included.                           */
+int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
+                          const double *u_lex, double scale, int project, void *stream);
 
 /* Solves M^dag M x = b; x, b are FP64 native spinors on cb 1.  Returns
    QPHIX_AMD_ENOCONV (with res filled) when max_iters is reached.         */

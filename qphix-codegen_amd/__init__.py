@@ -8,6 +8,6 @@ from .dslash import (PREC_NAME, TORCH_DTYPE, ClovDslash, Dslash, Field, Geometry
                      TMClovDslash, TMDslash)
 from . import lattice  # noqa: F401
 from .solver import (EvenOddCloverOperator, InvCG, clover_from_links, clover_invert,  # noqa: F401
-                     tmclover_blocks)
+                     reunitarize, tmclover_blocks)
 from .autograd import (DslashFunction, PseudofermionAction, dslash_adjoint, dslash_force,  # noqa: F401
                        su3_force_dslash, su3_force_pseudofermion)

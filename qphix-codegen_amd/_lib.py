@@ -61,12 +61,14 @@ SIGNATURES = {
                                           _c_double, _c_double, _c_double, _vp]),
     "qphix_amd_dslash_force": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_double,
                                         _c_double, _c_double, _c_double, _c_int, _vp]),
+    "qphix_amd_reunitarize": (_c_int, [_vp, _vp, _vp]),
     "qphix_amd_clover_from_links": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _vp]),
     "qphix_amd_clover_invert": (_c_int, [_vp, _vp, _vp, _vp]),
     "qphix_amd_tmclover_blocks": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _vp]),
     "qphix_amd_eo_op_create": (_c_int, [ctypes.POINTER(_vp), _vp, _vp, _vp, _c_double, _c_int, _c_int, _vp]),
     "qphix_amd_eo_op_destroy": (_c_int, [_vp]),
     "qphix_amd_eo_op_apply": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
+    "qphix_amd_eo_op_force": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _c_int, _vp]),
     "qphix_amd_cg_solve": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(CGParams), ctypes.POINTER(CGResult), _vp]),
 }
 

@@ -7,19 +7,18 @@ PseudofermionAction  S = phi^dagger (M^dagger M)^-1 phi with the even-odd Wilson
                      operator M = alpha - 1/(4 alpha) D_oe D_eo on cb 1,
                      alpha = 1/(2 kappa).  backward recomputes the solve
                      (nothing solver-sized is kept between forward and
-                     backward) and assembles
-                       dS = (1/(2 alpha)) Re[<Y, dD_oe Z> + <W, dD_eo X>],
-                     X = (M^dag M)^-1 phi, Y = M X, Z = D_eo X, W = D_oe^dag Y,
-                     from two force-kernel launches.
+                     backward) and calls qphix_amd_eo_op_force:
+                       dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>],
+                     X = (M^dag M)^-1 phi, Y = M X, Z = D_eo X / alpha,
+                     W = D_oe^dag Y / alpha.
 su3_force(...)       the traceless anti-Hermitian su(3) force of either loss,
                      d/de L(exp(i e T) U) = sum_links Re tr(i T F).
 """
-import ctypes
 
 import torch
 
 from ._lib import F64, check, lib
-from .dslash import Dslash, _ptr, _stream
+from .dslash import Dslash, _ptr, _stream  # noqa: F401
 from .solver import EvenOddCloverOperator, InvCG, clover_from_links
 
 
@@ -90,18 +89,7 @@ def _pf_solution(geom, U, phi, kappa, rsd):
 
 def _pf_force(geom, U, phi, kappa, rsd, scale, project):
     op, alpha, b, X = _pf_solution(geom, U, phi, kappa, rsd)
-    Y = geom.new_spinor(F64, cb=1)
-    op.apply(Y, X, dagger=False)
-    g0 = geom.pack_gauge(U, F64, 12, 0)
-    Z = geom.new_spinor(F64, cb=0)
-    W = geom.new_spinor(F64, cb=0)
-    D = Dslash(geom)
-    D.dslash(Z, X, g0, 1, 0)
-    dslash_adjoint(geom, W, Y, g0, 1, 1)  # W = D_oe^dag Y on cb 0
-    F = torch.zeros_like(U)
-    s = scale / (2.0 * alpha)
-    dslash_force(geom, F, Y, Z, U, 1, 1, scale=s, project=project)
-    dslash_force(geom, F, W, X, U, 1, 0, scale=s, project=project)
+    F = op.force(torch.zeros_like(U), X, U, scale=scale, project=project)
     op.close()
     return F
 

@@ -101,6 +101,25 @@ hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const v
   return run_dslash_any(op->ctx, c, st);
 }
 
+// first stage of eo_apply only: out (cb 0) = F_ee^-1 D_eo in (dagger: the
+// dagger blocks and isign -1)
+hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
+                     hipStream_t st)
+{
+  DslashCall c{};
+  c.prec = level_prec(op, lvl);
+  c.recon = level_recon(op, lvl);
+  c.op = op->tm ? OP_TMCLOVER : OP_CLOVER;
+  c.isign = dagger ? -1 : 1;
+  c.coeff_s = c.coeff_tf = c.coeff_tb = 1.0;
+  c.cb = 0;
+  c.out = out;
+  c.in = in;
+  c.gauge = op->gauge[lvl][0].p;
+  c.clov = op->dsl[lvl][dagger].p;
+  return run_dslash_any(op->ctx, c, st);
+}
+
 // out = M^dag M in, via the cb-1 temporary mid
 hipError_t eo_normal(qphix_amd_eo_op *op, int lvl, void *out, void *mid, const void *in,
                      hipStream_t st)
@@ -233,6 +252,30 @@ int qphix_amd_eo_op_apply(qphix_amd_eo_op *op, int prec, int dagger, void *out, 
     return QPHIX_AMD_EINVAL;
   return eo_apply(op, lvl, dagger ? 1 : 0, out, in, (hipStream_t)stream) == hipSuccess ? QPHIX_AMD_OK
                                                                                        : QPHIX_AMD_EHIP;
+}
+
+int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
+                          const double *u_lex, double scale, int project, void *stream)
+{
+  if (!op || !force_lex || !x || !u_lex) return QPHIX_AMD_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const long n = 24L * op->ctx->g.vcb;
+  DevBuf y, z, w;
+  if (y.alloc(n * 8) != hipSuccess || z.alloc(n * 8) != hipSuccess || w.alloc(n * 8) != hipSuccess)
+    return QPHIX_AMD_ENOMEM;
+  // Y = M X, Z = F_ee^-1 D_eo X, W = (F_ee^-1)^dag D_oe^dag Y
+  hipError_t e = eo_apply(op, 0, 0, y.p, x, st);
+  if (e == hipSuccess) e = eo_stage1(op, 0, 0, z.p, x, st);
+  if (e == hipSuccess) e = eo_stage1(op, 0, 1, w.p, y.p, st);
+  if (e != hipSuccess) return QPHIX_AMD_EHIP;
+  // dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>]
+  int rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 1, force_lex, y.p, z.p, u_lex, 1.0, 1.0,
+                                  1.0, 0.5 * scale, project, stream);
+  if (rc == QPHIX_AMD_OK)
+    rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 0, force_lex, w.p, x, u_lex, 1.0, 1.0,
+                                1.0, 0.5 * scale, project, stream);
+  if (rc == QPHIX_AMD_OK && hipStreamSynchronize(st) != hipSuccess) rc = QPHIX_AMD_EHIP;
+  return rc;
 }
 
 int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_amd_cg_params *prm,

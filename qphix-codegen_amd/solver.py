@@ -15,6 +15,12 @@ from ._lib import F64, CGParams, CGResult, QPhiXAMDError, check, lib
 from .dslash import Field, _ptr, _stream
 
 
+def reunitarize(geom, u_lex):
+    """Project every link of u_lex (lex FP64, in place) back onto SU(3)."""
+    check(lib().qphix_amd_reunitarize(geom.ctx, _ptr(u_lex), _stream()), "qphix_amd_reunitarize")
+    return u_lex
+
+
 def clover_from_links(geom, u_lex, alpha, csw):
     """Packed Hermitian clover blocks A(x) (lex FP64, (V,2,36)) from links."""
     out = torch.empty(geom.vcb * 2, 2, 36, dtype=torch.float64, device=u_lex.device)
@@ -72,6 +78,15 @@ class EvenOddCloverOperator:
                                           _stream()),
               "qphix_amd_eo_op_apply")
         out.cb = 1
+
+
+    def force(self, force, x, u_lex, scale=1.0, project=True):
+        """Accumulate the hopping-term gradient of phi^dag (M^dag M)^-1 phi at
+        x = (M^dag M)^-1 phi (FP64, cb 1) into force (lex FP64 (V,4,3,3,2))."""
+        check(lib().qphix_amd_eo_op_force(self._op, _ptr(force), x.ptr(), _ptr(u_lex.contiguous()),
+                                          float(scale), int(bool(project)), _stream()),
+              "qphix_amd_eo_op_force")
+        return force
 
 
 class InvCG:

@@ -128,3 +128,22 @@ def test_cg_reports_nonconvergence(qamd, lat):
     bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
     res = qamd.InvCG(op, max_iters=5)(x, bf, 1e-10, mixed=True, raise_on_noconv=False)
     assert res["status"] == -5 and res["iters"] == 5 and res["rsd_final"] > 1e-10
+
+
+def test_reunitarize(qamd):
+    """Perturbed links projected back onto SU(3): unitary, det 1, equal to the
+    host Gram-Schmidt projection, and exact SU(3) links are left unchanged."""
+    dims = (8, 8, 8, 8)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims)
+    U = L.random_gauge(dims, 0.4, 91)
+    noisy = U + 1e-3 * torch.randn(U.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(3))
+    got = qamd.reunitarize(geom, noisy.clone().to(DEV)).cpu()
+    ref = L.cplx_to_real(L.reunitarize(L.real_to_cplx(noisy)))
+    assert (got - ref).abs().max().item() < 1e-14
+    z = L.real_to_cplx(got)
+    eye = torch.eye(3, dtype=torch.complex128)
+    assert (z @ z.conj().transpose(-1, -2) - eye).abs().max().item() < 1e-14
+    assert (torch.linalg.det(z) - 1).abs().max().item() < 1e-14
+    same = qamd.reunitarize(geom, U.clone().to(DEV)).cpu()
+    assert (same - U).abs().max().item() < 1e-14
