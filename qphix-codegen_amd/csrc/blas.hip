@@ -20,13 +20,13 @@ constexpr int kBlasBlock = 256;
 template <typename S, typename R>
 __device__ __forceinline__ void lds(const S *p, int vcb, int site, R v[24])
 {
-  load_reals<S, pick_w<S>(24), 0, 24>(p, vcb, site, v);
+  load_reals<S, spinor_w<S>(), 0, 24>(p, vcb, site, v);
 }
 
 template <typename S, typename R>
 __device__ __forceinline__ void sts(S *p, int vcb, int site, const R v[24])
 {
-  store_reals<S, pick_w<S>(24), 0, 24>(p, vcb, site, v);
+  store_reals<S, spinor_w<S>(), 0, 24>(p, vcb, site, v);
 }
 
 #define QAMD_SITE_LOOP(site, vcb) \

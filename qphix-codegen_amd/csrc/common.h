@@ -71,6 +71,15 @@ template <typename S> constexpr int pick_w(int N)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// Unit widths of each field.  The two links of one direction mu (back and
+// forward) form one unit, as do the two packed clover blocks of a site, so
+// that FP16/BF16 12-real links (48 B per pair) and packed clover (144 B)
+// load as 16-byte vectors.
+template <typename S> constexpr int spinor_w() { return pick_w<S>(24); }
+template <typename S, int RECON> constexpr int gauge_w() { return pick_w<S>(2 * RECON); }
+template <typename S> constexpr int clover_w() { return pick_w<S>(72); }
+template <typename S> constexpr int fclover_w() { return pick_w<S>(144); }
+
 template <int BYTES> struct RawVec;
 template <> struct RawVec<16> { using T = u32x4; };
 template <> struct RawVec<8> { using T = u32x2; };

@@ -20,6 +20,19 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.g = ctx->g;
   p.cb = c.cb;
   p.nblocks = blocks_for(ctx->g.vcb, kDslashBlock);
+  // z-slab traversal needs whole work-groups per (z,t) plane and Lz % 8 == 0
+  p.order = 0;
+  p.zs = p.ts = p.bpp = 1;
+  if (ctx->g.sz % kDslashBlock == 0) {
+    p.bpp = ctx->g.sz / kDslashBlock;
+    if (QAMD_BLOCK_ORDER == 1 && ctx->g.Lz % kNumXcd == 0) {
+      p.order = 1;
+      p.zs = ctx->g.Lz / kNumXcd;
+    } else if (QAMD_BLOCK_ORDER == 2 && ctx->g.Lt % kNumXcd == 0) {
+      p.order = 2;
+      p.ts = ctx->g.Lt / kNumXcd;
+    }
+  }
   p.coeff_s = (R)c.coeff_s;
   p.coeff_tf = (R)c.coeff_tf;
   p.coeff_tb = (R)c.coeff_tb;

@@ -28,12 +28,22 @@ template <typename R> struct DslashParams {
   Geom g;
   int cb;
   int nblocks;  // real (unpadded) work-groups
+  int order;    // work-group traversal: 0 t-slabs per XCD (natural order),
+                // 1 z-slabs per XCD visited t-major, 2 t-slabs visited z-major
+  int zs, ts, bpp;  // z-planes / t-slices per XCD slab, work-groups per (z,t) plane
   R coeff_s, coeff_tf, coeff_tb, tbound;
   int tbc_mask;
   R tbc_re[4], tbc_im[4];
   R alpha, mu, muinv, rho;
 };
 
+#ifndef QAMD_XCD_REMAP
+#define QAMD_XCD_REMAP 1
+#endif
+// default work-group traversal (see DslashParams::order)
+#ifndef QAMD_BLOCK_ORDER
+#define QAMD_BLOCK_ORDER 0
+#endif
 #ifndef QAMD_DSLASH_BLOCK
 #define QAMD_DSLASH_BLOCK 256
 #endif
@@ -93,39 +103,43 @@ __device__ __forceinline__ void reconstruct(cx<R> acc[4][3], const cx<R> w[2][3]
   }
 }
 
-// Load hop matrix d of a site: rows 0,1 stored, row 2 stored (18) or
-// rebuilt as conj(row0 x row1) (12; codegen/dslash_common.cc:1664-1674).
-template <typename S, int RECON, int D, typename R>
-__device__ __forceinline__ void load_link(const S *gauge, int vcb, int site, cx<R> u[3][3])
+// Load the two hop matrices of direction MU at a site (d = 2 MU back,
+// 2 MU + 1 forward): rows 0,1 stored, row 2 stored (18) or rebuilt as
+// conj(row0 x row1) (12; codegen/dslash_common.cc:1664-1674).
+template <typename S, int RECON, int MU, typename R>
+__device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site, cx<R> ub[3][3],
+                                               cx<R> uf[3][3])
 {
-  constexpr int WL = pick_w<S>(RECON);
-  R lk[RECON];
-  load_reals<S, WL, D * RECON, RECON, R, true>(gauge, vcb, site, lk);
+  constexpr int WL = gauge_w<S, RECON>();
+  R lk[2 * RECON];
+  load_reals<S, WL, 2 * MU * RECON, 2 * RECON, R, true>(gauge, vcb, site, lk);
 #pragma unroll
-  for (int r = 0; r < RECON / 6; r++)
+  for (int fb = 0; fb < 2; fb++) {
+    cx<R>(*u)[3] = fb ? uf : ub;
+    const R *l = lk + fb * RECON;
 #pragma unroll
-    for (int c = 0; c < 3; c++) u[r][c] = {lk[(r * 3 + c) * 2], lk[(r * 3 + c) * 2 + 1]};
-  if (RECON == 12) {
+    for (int r = 0; r < RECON / 6; r++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-      u[2][c] = conj(u[0][c1] * u[1][c2] - u[0][c2] * u[1][c1]);
+      for (int c = 0; c < 3; c++) u[r][c] = {l[(r * 3 + c) * 2], l[(r * 3 + c) * 2 + 1]};
+    if (RECON == 12) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        u[2][c] = conj(u[0][c1] * u[1][c2] - u[0][c2] * u[1][c1]);
+      }
     }
   }
 }
 
-template <typename S, int RECON, int MU, int FB, int ISIGN, typename R>
+template <typename S, int MU, int FB, int ISIGN, typename R>
 __device__ __forceinline__ void hop_dir(cx<R> acc[4][3], const S *__restrict__ in,
-                                        const S *__restrict__ gauge, int vcb,
-                                        int nbr, int site, R beta, bool tbc,
+                                        const cx<R> u[3][3], int vcb, int nbr, R beta, bool tbc,
                                         cx<R> phase)
 {
   constexpr int SGN = FB == 0 ? ISIGN : -ISIGN;
-  constexpr int WS = pick_w<S>(24);
+  constexpr int WS = spinor_w<S>();
   R psi[24];
   load_reals<S, WS, 0, 24>(in, vcb, nbr, psi);
-  cx<R> u[3][3];
-  load_link<S, RECON, 2 * MU + FB>(gauge, vcb, site, u);
   cx<R> h[2][3];
   project<MU, SGN>(psi, h);
   cx<R> w[2][3];
@@ -148,20 +162,28 @@ __device__ __forceinline__ void hop_dir(cx<R> acc[4][3], const S *__restrict__ i
   reconstruct<MU, SGN>(acc, w);
 }
 
+template <typename S, int RECON, int MU, int ISIGN, typename R>
+__device__ __forceinline__ void hop_mu(cx<R> acc[4][3], const S *__restrict__ in,
+                                       const S *__restrict__ gauge, int vcb, int site, int nb,
+                                       int nf, R bb, R bf, bool tbc, cx<R> phase)
+{
+  cx<R> ub[3][3], uf[3][3];
+  load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf);
+  hop_dir<S, MU, 0, ISIGN>(acc, in, ub, vcb, nb, bb, tbc, conj(phase));
+  hop_dir<S, MU, 1, ISIGN>(acc, in, uf, vcb, nf, bf, tbc, phase);
+}
+
 // out = A in, A = two packed Hermitian 6x6 blocks
 // (codegen/dslash_common.cc:1098-1186).
 template <typename S, typename R>
 __device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
                                               const cx<R> in[4][3], cx<R> out[4][3])
 {
-  constexpr int WC = pick_w<S>(36);
+  R all[72];
+  load_reals<S, clover_w<S>(), 0, 72, R, true>(clov, vcb, site, all);
 #pragma unroll
   for (int b = 0; b < 2; b++) {
-    R blk[36];
-    if (b == 0)
-      load_reals<S, WC, 0, 36, R, true>(clov, vcb, site, blk);
-    else
-      load_reals<S, WC, 36, 36, R, true>(clov, vcb, site, blk);
+    const R *blk = all + 36 * b;
     cx<R> v[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
@@ -191,7 +213,7 @@ template <typename S, typename R>
 __device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
                                             const cx<R> in[4][3], cx<R> out[4][3])
 {
-  constexpr int WF = pick_w<S>(72);
+  constexpr int WF = fclover_w<S>();
 #pragma unroll
   for (int b = 0; b < 2; b++) {
     R blk[72];
@@ -240,10 +262,33 @@ template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
 __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {
   using R = typename Traits<S>::R;
+#if QAMD_XCD_REMAP
   const int blk = xcd_block();
+#else
+  const int blk = (int)blockIdx.x;
+#endif
   if (blk >= p.nblocks) return;
-  const int idx = blk * kDslashBlock + (int)threadIdx.x;
   const Geom &g = p.g;
+  int idx;
+  if (p.order == 1) {
+    // XCD k owns z-planes [k zs, (k+1) zs) for every t, visited t-major, so
+    // t-neighbours are zs*bpp work-groups apart and z-neighbours bpp apart
+    const int per = p.zs * g.Lt * p.bpp;
+    const int k = blk / per, r = blk - k * per;
+    const int t = r / (p.zs * p.bpp), r2 = r - t * (p.zs * p.bpp);
+    const int z = k * p.zs + r2 / p.bpp, h = r2 % p.bpp;
+    idx = (t * g.Lz + z) * g.sz + h * kDslashBlock + (int)threadIdx.x;
+  } else if (p.order == 2) {
+    // XCD k owns t-slices [k ts, (k+1) ts), visited z-major: t-neighbours
+    // are bpp work-groups apart, z-neighbours ts*bpp
+    const int per = p.ts * g.Lz * p.bpp;
+    const int k = blk / per, r = blk - k * per;
+    const int z = r / (p.ts * p.bpp), r2 = r - z * (p.ts * p.bpp);
+    const int t = k * p.ts + r2 / p.bpp, h = r2 % p.bpp;
+    idx = (t * g.Lz + z) * g.sz + h * kDslashBlock + (int)threadIdx.x;
+  } else {
+    idx = blk * kDslashBlock + (int)threadIdx.x;
+  }
   if (idx >= g.vcb) return;
   const SiteCoord c = decode(g, idx, p.cb);
   const S *__restrict__ in = static_cast<const S *>(p.in);
@@ -274,14 +319,10 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const int tm = p.tbc_mask;
   const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
   const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
-  hop_dir<S, RECON, 0, 0, ISIGN>(acc, in, gauge, vcb, n_xb, idx, bs, tm & 1, conj(ph0));
-  hop_dir<S, RECON, 0, 1, ISIGN>(acc, in, gauge, vcb, n_xf, idx, bs, tm & 1, ph0);
-  hop_dir<S, RECON, 1, 0, ISIGN>(acc, in, gauge, vcb, n_yb, idx, bs, tm & 2, conj(ph1));
-  hop_dir<S, RECON, 1, 1, ISIGN>(acc, in, gauge, vcb, n_yf, idx, bs, tm & 2, ph1);
-  hop_dir<S, RECON, 2, 0, ISIGN>(acc, in, gauge, vcb, n_zb, idx, bs, tm & 4, conj(ph2));
-  hop_dir<S, RECON, 2, 1, ISIGN>(acc, in, gauge, vcb, n_zf, idx, bs, tm & 4, ph2);
-  hop_dir<S, RECON, 3, 0, ISIGN>(acc, in, gauge, vcb, n_tb, idx, btb, tm & 8, conj(ph3));
-  hop_dir<S, RECON, 3, 1, ISIGN>(acc, in, gauge, vcb, n_tf, idx, btf, tm & 8, ph3);
+  hop_mu<S, RECON, 0, ISIGN>(acc, in, gauge, vcb, idx, n_xb, n_xf, bs, bs, tm & 1, ph0);
+  hop_mu<S, RECON, 1, ISIGN>(acc, in, gauge, vcb, idx, n_yb, n_yf, bs, bs, tm & 2, ph1);
+  hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bs, bs, tm & 4, ph2);
+  hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3);
 
   cx<R> res[4][3];
   if (!ACHI) {
@@ -299,7 +340,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     }
   } else {
     R chr[24];
-    load_reals<S, pick_w<S>(24), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
+    load_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
     cx<R> chi[4][3];
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -340,7 +381,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       o[(s * 3 + k) * 2] = res[s][k].re;
       o[(s * 3 + k) * 2 + 1] = res[s][k].im;
     }
-  store_reals<S, pick_w<S>(24), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
+  store_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
 }
 
 // Host-side launch for one instantiation.

@@ -93,7 +93,7 @@ __device__ __forceinline__ void force_dir(const ForceParams &p, long y, const in
   const bool fwd = par == p.cb;  // y on chi's checkerboard: forward hop at y
   const S *chi = static_cast<const S *>(p.chi);
   const S *psi = static_cast<const S *>(p.psi);
-  constexpr int WS = pick_w<S>(24);
+  constexpr int WS = spinor_w<S>();
   R a[24], b[24];
   cx<R> va[4][3], vb[4][3], pr[4][3];
   double beta;

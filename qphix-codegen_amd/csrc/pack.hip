@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kPackBlock) void pack_spinor_k(S *dst, const double
   double v[24];
 #pragma unroll
   for (int k = 0; k < 24; k++) v[k] = src[lex * 24 + k];
-  store_reals<S, pick_w<S>(24), 0, 24>(dst, g.vcb, idx, v);
+  store_reals<S, spinor_w<S>(), 0, 24>(dst, g.vcb, idx, v);
 }
 
 template <typename S>
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kPackBlock) void unpack_spinor_k(double *dst, const
   const SiteCoord c = decode(g, idx, cb);
   const long lex = lex_index(g, c);
   double v[24];
-  load_reals<S, pick_w<S>(24), 0, 24>(src, g.vcb, idx, v);
+  load_reals<S, spinor_w<S>(), 0, 24>(src, g.vcb, idx, v);
 #pragma unroll
   for (int k = 0; k < 24; k++) dst[lex * 24 + k] = v[k];
 }
@@ -49,27 +49,31 @@ __device__ __forceinline__ long lex_shift(const Geom &g, SiteCoord c, int mu, in
   return lex_index(g, c);
 }
 
-template <typename S, int RECON, int D>
-__device__ __forceinline__ void pack_one_link(S *dst, const double *U, const Geom &g,
-                                              const SiteCoord &c, int idx)
+// Both links of direction MU form one storage unit (d = 2 MU, 2 MU + 1).
+template <typename S, int RECON, int MU>
+__device__ __forceinline__ void pack_link_pair(S *dst, const double *U, const Geom &g,
+                                               const SiteCoord &c, int idx)
 {
-  constexpr int MU = D / 2, FB = D % 2;
-  const long s = FB ? lex_index(g, c) : lex_shift(g, c, MU, -1);
-  const double *m = U + (s * 4 + MU) * 18;
-  double v[RECON];
+  double v[2 * RECON];
 #pragma unroll
-  for (int r = 0; r < RECON / 6; r++)
+  for (int FB = 0; FB < 2; FB++) {
+    const long s = FB ? lex_index(g, c) : lex_shift(g, c, MU, -1);
+    const double *m = U + (s * 4 + MU) * 18;
+    double *o = v + FB * RECON;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      if (FB) {
-        v[(r * 3 + k) * 2] = m[(r * 3 + k) * 2];
-        v[(r * 3 + k) * 2 + 1] = m[(r * 3 + k) * 2 + 1];
-      } else {  // dagger: (U^+)[r][k] = conj(U[k][r])
-        v[(r * 3 + k) * 2] = m[(k * 3 + r) * 2];
-        v[(r * 3 + k) * 2 + 1] = -m[(k * 3 + r) * 2 + 1];
+    for (int r = 0; r < RECON / 6; r++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (FB) {
+          o[(r * 3 + k) * 2] = m[(r * 3 + k) * 2];
+          o[(r * 3 + k) * 2 + 1] = m[(r * 3 + k) * 2 + 1];
+        } else {  // dagger: (U^+)[r][k] = conj(U[k][r])
+          o[(r * 3 + k) * 2] = m[(k * 3 + r) * 2];
+          o[(r * 3 + k) * 2 + 1] = -m[(k * 3 + r) * 2 + 1];
+        }
       }
-    }
-  store_reals<S, pick_w<S>(RECON), D * RECON, RECON>(dst, g.vcb, idx, v);
+  }
+  store_reals<S, gauge_w<S, RECON>(), 2 * MU * RECON, 2 * RECON>(dst, g.vcb, idx, v);
 }
 
 template <typename S, int RECON>
@@ -78,17 +82,13 @@ __global__ __launch_bounds__(kPackBlock) void pack_gauge_k(S *dst, const double 
   const int idx = blockIdx.x * kPackBlock + threadIdx.x;
   if (idx >= g.vcb) return;
   const SiteCoord c = decode(g, idx, cb);
-  pack_one_link<S, RECON, 0>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 1>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 2>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 3>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 4>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 5>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 6>(dst, U, g, c, idx);
-  pack_one_link<S, RECON, 7>(dst, U, g, c, idx);
+  pack_link_pair<S, RECON, 0>(dst, U, g, c, idx);
+  pack_link_pair<S, RECON, 1>(dst, U, g, c, idx);
+  pack_link_pair<S, RECON, 2>(dst, U, g, c, idx);
+  pack_link_pair<S, RECON, 3>(dst, U, g, c, idx);
 }
 
-// Clover (72 reals) or full clover (144 reals): two blocks of NB reals.
+// Clover (72 reals) or full clover (144 reals) of a site as one unit.
 template <typename S, int NB>
 __global__ __launch_bounds__(kPackBlock) void pack_clover_k(S *dst, const double *src, Geom g, int cb)
 {
@@ -96,16 +96,11 @@ __global__ __launch_bounds__(kPackBlock) void pack_clover_k(S *dst, const double
   if (idx >= g.vcb) return;
   const SiteCoord c = decode(g, idx, cb);
   const long lex = lex_index(g, c);
-  double v[NB];
+  double v[2 * NB];
 #pragma unroll
-  for (int b = 0; b < 2; b++) {
-#pragma unroll
-    for (int k = 0; k < NB; k++) v[k] = src[lex * 2 * NB + b * NB + k];
-    if (b == 0)
-      store_reals<S, pick_w<S>(NB), 0, NB>(dst, g.vcb, idx, v);
-    else
-      store_reals<S, pick_w<S>(NB), NB, NB>(dst, g.vcb, idx, v);
-  }
+  for (int k = 0; k < 2 * NB; k++) v[k] = src[lex * 2 * NB + k];
+  constexpr int W = NB == 36 ? clover_w<S>() : fclover_w<S>();
+  store_reals<S, W, 0, 2 * NB>(dst, g.vcb, idx, v);
 }
 
 template <typename S>
