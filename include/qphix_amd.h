@@ -238,7 +238,8 @@ int qphix_amd_eo_op_apply(qphix_amd_eo_op *op, int prec, int dagger, void *out, 
 typedef struct {
   double rsd_target; /* stop at |b - M^dag M x| / |b| < rsd_target (FP64)   */
   int max_iters;
-  double delta;      /* reliable update when |r| < delta * max|r| (def 0.1) */
+  double delta;      /* reliable update when |r| < delta * max|r|; <= 0 picks
+                        0.01 (FP32 sloppy), 0.05 (FP16) or 0.1 (BF16)       */
   int mixed;         /* 0: CG in FP64; 1: sloppy iterations + reliable updates */
 } qphix_amd_cg_params;
 

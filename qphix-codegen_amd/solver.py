@@ -97,7 +97,7 @@ class InvCG:
         self.op = op
         self.max_iters = int(max_iters)
 
-    def __call__(self, x, b, rsd_target, mixed=False, delta=0.1, raise_on_noconv=True):
+    def __call__(self, x, b, rsd_target, mixed=False, delta=0.0, raise_on_noconv=True):
         if x.prec != F64 or b.prec != F64:
             raise QPhiXAMDError("x and b are FP64 native spinors on cb 1")
         p = CGParams(float(rsd_target), self.max_iters, float(delta), int(bool(mixed)))
