@@ -101,6 +101,70 @@ int qphix_amd_pack_clover(qphix_amd_ctx *ctx, void *dst, int prec,
 int qphix_amd_pack_full_clover(qphix_amd_ctx *ctx, void *dst, int prec,
                                const double *fclov_lex, int cb, void *stream);
 
+/* ---- QPhiX block layouts (device QPhiX arrays <-> device native) ------
+   The per-checkerboard arrays a QPhiX host hands to the reference's kernels
+   (codegen/include/decl_common.h:9-62), with FT given by `ft`:
+     FourSpinorBlock  FT[3 colour][4 spin][2][soalen]
+     SU3MatrixBlock   FT[8 dir][2|3 rows][3][2][veclen]; entry [d][i][j] is
+                      M_d[j][i] (stored transposed, applied as U h forward
+                      and U^dagger h back, codegen/dslash_common.cc:1678-1727),
+                      with M_{2mu} = U_mu(x - mu) and M_{2mu+1} = U_mu(x);
+                      compress12 keeps rows i = 0, 1 and the kernel rebuilds
+                      row 2 = conj(row0 x row1) (codegen/dslash_common.cc:
+                      1657-1676)
+     CloverBlock      {diag1[6][V], off_diag1[15][2][V], diag2, off_diag2}
+     FullCloverBlock  {block1[6][6][2][V], block2[6][6][2][V]}
+   Block order (QPhiX's checkerboarded geometry) for the site
+   x = 2 xh + ((cb + y + z + t) & 1) of checkerboard cb:
+     nvecs = (Lx/2) / soalen,  ngy = veclen / soalen,
+     Pxy = Ly nvecs + pad_xy,  Pxyz = Lz Pxy + pad_xyz,
+     spinor blocks        t Pxyz + z Pxy + y nvecs + xh/soalen,
+                          lane xh % soalen  (Lt Pxyz blocks)
+     gauge/clover blocks  (t Pxyz + z Pxy)/ngy + (y/ngy) nvecs + xh/soalen,
+                          lane (y % ngy) soalen + xh % soalen
+                          (Lt Pxyz / ngy blocks)
+   so with soalen < veclen one gauge/clover vector spans ngy rows of y, as
+   the reference's kernels read it (packed gauges, codegen/data_types.cc:
+   218-241).  Requirements: soalen divides Lx/2 and veclen, ngy divides Ly,
+   pad_xy and pad_xyz are non-negative multiples of ngy; ft is F64, F32 or
+   F16 (IEEE binary16, the reference's `half`, decl_common.h:7).  Padding
+   blocks are neither read nor written.  The gauge conversion rebuilds row 2
+   of compressed blocks in FP64 before storing the applied matrices.    */
+typedef struct {
+  int ft;         /* storage of the QPhiX arrays: QPHIX_AMD_F64/F32/F16   */
+  int veclen;     /* VECLEN, lanes of SU3MatrixBlock / CloverBlock        */
+  int soalen;     /* SOALEN, lanes of FourSpinorBlock                     */
+  int compress12; /* SU3MatrixBlock holds 2 rows (1) or 3 rows (0)        */
+  int pad_xy;     /* spinor blocks of padding after each xy plane         */
+  int pad_xyz;    /* spinor blocks of padding after each xyz volume       */
+} qphix_amd_qphix_layout;
+
+/* Bytes of one checkerboard array of `field` in this QPhiX layout (0 if the
+   layout is invalid for the context's lattice).                          */
+size_t qphix_amd_qphix_field_bytes(const qphix_amd_ctx *ctx,
+                                   const qphix_amd_qphix_layout *lay,
+                                   int field);
+int qphix_amd_spinor_from_qphix(qphix_amd_ctx *ctx,
+                                const qphix_amd_qphix_layout *lay, void *dst,
+                                int prec, const void *src, int cb,
+                                void *stream);
+int qphix_amd_spinor_to_qphix(qphix_amd_ctx *ctx,
+                              const qphix_amd_qphix_layout *lay, void *dst,
+                              const void *src, int prec, int cb,
+                              void *stream);
+int qphix_amd_gauge_from_qphix(qphix_amd_ctx *ctx,
+                               const qphix_amd_qphix_layout *lay, void *dst,
+                               int prec, int recon, const void *src, int cb,
+                               void *stream);
+int qphix_amd_clover_from_qphix(qphix_amd_ctx *ctx,
+                                const qphix_amd_qphix_layout *lay, void *dst,
+                                int prec, const void *src, int cb,
+                                void *stream);
+int qphix_amd_full_clover_from_qphix(qphix_amd_ctx *ctx,
+                                     const qphix_amd_qphix_layout *lay,
+                                     void *dst, int prec, const void *src,
+                                     int cb, void *stream);
+
 /* ---- operators ----------------------------------------------------------
    out (checkerboard cb) from in (checkerboard 1-cb).  hop(x) =
      sum_mu b_mu(x) (1 + s g_mu) U_mu(x-mu)^+ psi(x-mu)

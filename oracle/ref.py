@@ -32,6 +32,9 @@ def lib(config):
                                 d, d, d, i, i, _dp]
         l.ref_apply.restype = i
         l.ref_veclen.restype = i
+        l.ref_precision.restype = i
+        l.ref_pack_field.argtypes = [i, i, i, i, i, i, i, _dp, ctypes.c_void_p]
+        l.ref_pack_field.restype = ctypes.c_long
         l.ref_soalen.restype = i
         _libs[config] = l
     return _libs[config]
@@ -64,3 +67,17 @@ def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=No
     if rc != 0:
         raise ValueError(f"reference kernel rejected the call (config {config}, dims {dims})")
     return out, (sec.value if reps > 0 else None)
+
+
+def pack(config, field, dims, src, cb, recon=18):
+    """The block array (flat, FT = float32 or float64) that ref_apply hands
+    to the reference's bodies for a lexicographic FP64 field: field 0 spinor,
+    1 gauge (recon 12 | 18), 2 clover, 3 full clover."""
+    l = lib(config)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    n = l.ref_pack_field(field, recon, cb, *dims, _p(src), None)
+    if n < 0:
+        raise ValueError(f"unsupported geometry {dims} for {config}")
+    out = np.empty(n, dtype=np.float32 if l.ref_precision() == 1 else np.float64)
+    l.ref_pack_field(field, recon, cb, *dims, _p(src), out.ctypes.data_as(ctypes.c_void_p))
+    return out

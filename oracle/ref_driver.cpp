@@ -221,6 +221,35 @@ void pack_full_clover(FT *dst, const double *cl, const Lat &g, int cb)
 
 extern "C" {
 
+// Packs a lexicographic FP64 field into the block arrays ref_apply hands to
+// the bodies (field 0 spinor, 1 gauge with recon 12|18, 2 clover, 3 full
+// clover) for checkerboard cb.  Returns the number of FT elements; with
+// dst == NULL only the count.  -1 for an unsupported geometry.
+long ref_pack_field(int field, int recon, int cb, int Lx, int Ly, int Lz, int Lt, const double *src,
+                    void *dst)
+{
+  if ((Lx | Ly | Lz | Lt) & 1 || (Lx / 2) % S || field < 0 || field > 3) return -1;
+  Lat g;
+  g.L[0] = Lx;
+  g.L[1] = Ly;
+  g.L[2] = Lz;
+  g.L[3] = Lt;
+  g.Lxh = Lx / 2;
+  g.nxb = g.Lxh / S;
+  const long nblk = (long)Lt * Lz * Ly * g.nxb;
+  const int nrow = recon == 12 ? 2 : 3;
+  const long per[4] = {BLK, 8L * nrow * 3 * 2 * VECLEN, 72L * VECLEN, 144L * VECLEN};
+  const long n = nblk * per[field];
+  if (!dst) return n;
+  FT *d = (FT *)dst;
+  memset(d, 0, n * sizeof(FT));
+  if (field == 0) pack_spinor(d, src, g, cb);
+  if (field == 1) pack_gauge(d, src, g, cb, nrow);
+  if (field == 2) pack_clover(d, src, g, cb);
+  if (field == 3) pack_full_clover(d, src, g, cb);
+  return n;
+}
+
 const char *ref_config(void) { return REF_CONFIG_NAME; }
 int ref_veclen(void) { return VECLEN; }
 int ref_soalen(void) { return SOALEN; }

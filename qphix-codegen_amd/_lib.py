@@ -27,6 +27,14 @@ class CGResult(ctypes.Structure):
                 ("rsd_final", ctypes.c_double), ("seconds", ctypes.c_double),
                 ("mv_apps", ctypes.c_long)]
 
+class QPhiXLayout(ctypes.Structure):
+    """qphix_amd_qphix_layout: storage and block geometry of QPhiX arrays."""
+    _fields_ = [("ft", ctypes.c_int), ("veclen", ctypes.c_int), ("soalen", ctypes.c_int),
+                ("compress12", ctypes.c_int), ("pad_xy", ctypes.c_int), ("pad_xyz", ctypes.c_int)]
+
+
+_lay = ctypes.POINTER(QPhiXLayout)
+
 # name -> (restype, argtypes); the order follows include/qphix_amd.h
 SIGNATURES = {
     "qphix_amd_ctx_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_double]),
@@ -40,6 +48,12 @@ SIGNATURES = {
     "qphix_amd_pack_gauge": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "qphix_amd_pack_clover": (_c_int, [_vp, _vp, _c_int, _vp, _c_int, _vp]),
     "qphix_amd_pack_full_clover": (_c_int, [_vp, _vp, _c_int, _vp, _c_int, _vp]),
+    "qphix_amd_qphix_field_bytes": (ctypes.c_size_t, [_vp, _lay, _c_int]),
+    "qphix_amd_spinor_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _vp, _c_int, _vp]),
+    "qphix_amd_spinor_to_qphix": (_c_int, [_vp, _lay, _vp, _vp, _c_int, _c_int, _vp]),
+    "qphix_amd_gauge_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _c_int, _vp, _c_int, _vp]),
+    "qphix_amd_clover_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _vp, _c_int, _vp]),
+    "qphix_amd_full_clover_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _vp, _c_int, _vp]),
     "qphix_amd_dslash": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                   _c_double, _c_double, _c_double, _vp]),
     "qphix_amd_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,

@@ -24,7 +24,7 @@ import ctypes
 import torch
 
 from ._lib import (BF16, F16, F32, F64, FIELD_CLOVER, FIELD_FULLCLOVER, FIELD_GAUGE,
-                   FIELD_SPINOR, QPhiXAMDError, check, lib)
+                   FIELD_SPINOR, QPhiXAMDError, QPhiXLayout, check, lib)
 
 TORCH_DTYPE = {F64: torch.float64, F32: torch.float32, F16: torch.float16, BF16: torch.bfloat16}
 PREC_NAME = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16"}
@@ -146,6 +146,76 @@ class Geometry:
         check(lib().qphix_amd_pack_full_clover(self._ctx, f.ptr(), prec, _ptr(fclov_lex), cb, _stream()),
               "qphix_amd_pack_full_clover")
         return f
+
+    # ---- converters (device QPhiX block arrays <-> native) ---------------
+    # A layout is a QPhiXLayout (ft, veclen, soalen, compress12, pad_xy,
+    # pad_xyz); QPhiX arrays are flat device tensors of the layout's FT
+    # (float64, float32, or float16 holding the reference's IEEE `half`).
+    def qphix_field_bytes(self, layout, kind):
+        n = int(lib().qphix_amd_qphix_field_bytes(self._ctx, ctypes.byref(layout), kind))
+        if n == 0:
+            raise QPhiXAMDError(f"layout {_layout_str(layout)} does not fit lattice {self.dims}")
+        return n
+
+    def new_qphix_field(self, layout, kind, device="cuda"):
+        dt = TORCH_DTYPE[layout.ft]
+        n = self.qphix_field_bytes(layout, kind) // torch.tensor([], dtype=dt).element_size()
+        return torch.zeros(n, dtype=dt, device=device)
+
+    def _qphix_in(self, arr, layout, kind):
+        if arr.dtype != TORCH_DTYPE[layout.ft] or not arr.is_cuda:
+            raise QPhiXAMDError("QPhiX arrays are device tensors of the layout's FT")
+        if arr.numel() * arr.element_size() < self.qphix_field_bytes(layout, kind):
+            raise QPhiXAMDError("QPhiX array smaller than the layout requires")
+        return arr.contiguous()
+
+    def spinor_from_qphix(self, arr, layout, prec, cb):
+        arr = self._qphix_in(arr, layout, FIELD_SPINOR)
+        f = self.new_spinor(prec, device=arr.device, cb=cb)
+        check(lib().qphix_amd_spinor_from_qphix(self._ctx, ctypes.byref(layout), f.ptr(), prec, _ptr(arr),
+                                                cb, _stream()), "qphix_amd_spinor_from_qphix")
+        return f
+
+    def spinor_to_qphix(self, f, layout, out=None):
+        """Native spinor -> QPhiX FourSpinorBlocks (padding blocks untouched)."""
+        if out is None:
+            out = self.new_qphix_field(layout, FIELD_SPINOR, device=f.data.device)
+        out = self._qphix_in(out, layout, FIELD_SPINOR)
+        check(lib().qphix_amd_spinor_to_qphix(self._ctx, ctypes.byref(layout), _ptr(out), f.ptr(), f.prec,
+                                              f.cb, _stream()), "qphix_amd_spinor_to_qphix")
+        return out
+
+    def gauge_from_qphix(self, arr, layout, prec, recon, cb):
+        arr = self._qphix_in(arr, layout, FIELD_GAUGE)
+        f = self.new_field(FIELD_GAUGE, prec, recon, device=arr.device, cb=cb)
+        check(lib().qphix_amd_gauge_from_qphix(self._ctx, ctypes.byref(layout), f.ptr(), prec, recon,
+                                               _ptr(arr), cb, _stream()), "qphix_amd_gauge_from_qphix")
+        return f
+
+    def clover_from_qphix(self, arr, layout, prec, cb):
+        arr = self._qphix_in(arr, layout, FIELD_CLOVER)
+        f = self.new_field(FIELD_CLOVER, prec, device=arr.device, cb=cb)
+        check(lib().qphix_amd_clover_from_qphix(self._ctx, ctypes.byref(layout), f.ptr(), prec, _ptr(arr),
+                                                cb, _stream()), "qphix_amd_clover_from_qphix")
+        return f
+
+    def full_clover_from_qphix(self, arr, layout, prec, cb):
+        arr = self._qphix_in(arr, layout, FIELD_FULLCLOVER)
+        f = self.new_field(FIELD_FULLCLOVER, prec, device=arr.device, cb=cb)
+        check(lib().qphix_amd_full_clover_from_qphix(self._ctx, ctypes.byref(layout), f.ptr(), prec,
+                                                     _ptr(arr), cb, _stream()),
+              "qphix_amd_full_clover_from_qphix")
+        return f
+
+
+def qphix_layout(ft, veclen, soalen, compress12=True, pad_xy=0, pad_xyz=0):
+    """QPhiXLayout for arrays of storage ft (F64 / F32 / F16)."""
+    return QPhiXLayout(int(ft), int(veclen), int(soalen), int(bool(compress12)), int(pad_xy), int(pad_xyz))
+
+
+def _layout_str(l):
+    return (f"(ft={PREC_NAME.get(l.ft, l.ft)}, veclen={l.veclen}, soalen={l.soalen}, "
+            f"compress12={l.compress12}, pad_xy={l.pad_xy}, pad_xyz={l.pad_xyz})")
 
 
 def _check_args(res, psi, u, isign, cb):
