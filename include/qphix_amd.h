@@ -240,6 +240,67 @@ int qphix_amd_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
                                         double coeff_t_f, double coeff_t_b,
                                         double prec_mass_rho, void *stream);
 
+/* ---- faces: the lattice split over several GPUs --------------------------
+   qphix_amd_set_comm marks split dimensions: comm[mu] != 0 drops, from every
+   operator call above, the hops that wrap around the local lattice in mu
+   (they belong to neighbouring GPUs), which gives the interior part the
+   reference computes with accumulate[d] = 0 on boundary blocks
+   (codegen/dslash.cc:248,368; the time-boundary sign then no longer applies
+   inside the call).  The faces complete it:
+     face_pack   half-spinors of the input (checkerboard 1-cb) on the slice
+                 x_dim = 0 (fb 0, for the backward neighbour) or L_dim - 1
+                 (fb 1, forward), projected with the receiving hop's
+                 projector, (1 - isign g_dim) for fb 0 and (1 + isign g_dim)
+                 for fb 1 (face_proj_dir_{plus,minus},
+                 codegen/jinja/dslash_decl.h.j2:91-100, codegen/dslash.cc:
+                 433-458);
+     *_finish    out sites (checkerboard cb) on x_dim = 0 (fb 0, buffer from
+                 the backward neighbour, U_dim(x - dim)^dagger) or L_dim - 1
+                 (fb 1, from the forward neighbour, U_dim(x)): r = reconstructed
+                 beta * U h (twisted-BC phase of the context included), then
+                   face_finish          out -= r
+                   clov_face_finish     out += invclover r
+                   tm_face_finish       out -= muinv (1 - isign i mu g5) r
+                   tm_clov_face_finish  out += fclover r
+                 (face_finish_dir_*, face_clov_finish_dir_*,
+                 tm_face_finish_dir_*, tm_clov_face_finish_dir_*:
+                 codegen/jinja/{,clov_,tm_,tm_clov_}dslash_decl.h.j2, codegen/dslash.cc:463-553,
+                 codegen/dslash_common.cc:1947-1993).  So completing a plain
+                 or twisted-mass dslash passes beta = -coeff, the clover ones
+                 beta = +coeff, and any achimbdpsi uses face_finish with
+                 beta = +coeff; the caller folds the time-boundary sign into
+                 beta on the GPUs at the global time boundary.
+   A face buffer holds qphix_amd_face_sites(ctx, dim) half-spinors of 12
+   reals ([spin][colour][re|im], codegen/data_types.cc:508-543) in the
+   storage precision, site-fastest in 16-byte vectors (bytes = sites * 12 *
+   sizeof(storage)).  Face site f runs over the other coordinates:
+     x: (t Lz + z) Ly/2 + y/2   y: (t Lz + z) Lx/2 + x/2
+     z: (t Ly + y) Lx/2 + x/2   t: (z Ly + y) Lx/2 + x/2
+   identical on both sides of a cut.  Operators built on a context
+   (qphix_amd_eo_op_create, qphix_amd_dslash_force) need comm all zero.   */
+int qphix_amd_set_comm(qphix_amd_ctx *ctx, const int comm[4]);
+long qphix_amd_face_sites(const qphix_amd_ctx *ctx, int dim);
+int qphix_amd_face_pack(qphix_amd_ctx *ctx, int prec, int isign, int cb,
+                        int dim, int fb, void *buf, const void *in,
+                        void *stream);
+int qphix_amd_face_finish(qphix_amd_ctx *ctx, int prec, int recon, int isign,
+                          int cb, int dim, int fb, void *out, const void *buf,
+                          const void *gauge, double beta, void *stream);
+int qphix_amd_clov_face_finish(qphix_amd_ctx *ctx, int prec, int recon,
+                               int isign, int cb, int dim, int fb, void *out,
+                               const void *buf, const void *gauge,
+                               const void *invclover, double beta,
+                               void *stream);
+int qphix_amd_tm_face_finish(qphix_amd_ctx *ctx, int prec, int recon,
+                             int isign, int cb, int dim, int fb, void *out,
+                             const void *buf, const void *gauge, double beta,
+                             double mu, double muinv, void *stream);
+int qphix_amd_tm_clov_face_finish(qphix_amd_ctx *ctx, int prec, int recon,
+                                  int isign, int cb, int dim, int fb,
+                                  void *out, const void *buf,
+                                  const void *gauge, const void *fclover,
+                                  double beta, void *stream);
+
 /* ---- differentiation (no counterpart in the reference) ------------------
    For L = Re<chi, D psi>, D = the hop above with output checkerboard cb:
    dL/dpsi = D^dagger chi.  D_isign^dagger = D_{-isign} with coeff_t_f and

@@ -91,6 +91,7 @@ int qphix_amd_ctx_create(qphix_amd_ctx **ctx, int Lx, int Ly, int Lz, int Lt, do
   c->g.sz = Lx / 2 * Ly;
   c->g.st = Lx / 2 * Ly * Lz;
   c->tbound = t_boundary;
+  c->comm_mask = 0;
   c->tbc_mask = 0;
   for (int m = 0; m < 4; m++) {
     c->tbc_re[m] = 1.0;

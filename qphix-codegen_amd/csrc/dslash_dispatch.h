@@ -37,6 +37,7 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.coeff_tf = (R)c.coeff_tf;
   p.coeff_tb = (R)c.coeff_tb;
   p.tbound = (R)ctx->tbound;
+  p.comm = ctx->comm_mask;
   p.tbc_mask = ctx->tbc_mask;
   for (int m = 0; m < 4; m++) {
     p.tbc_re[m] = (R)ctx->tbc_re[m];

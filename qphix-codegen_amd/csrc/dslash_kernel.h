@@ -32,6 +32,7 @@ template <typename R> struct DslashParams {
                 // 1 z-slabs per XCD visited t-major, 2 t-slabs visited z-major
   int zs, ts, bpp;  // z-planes / t-slices per XCD slab, work-groups per (z,t) plane
   R coeff_s, coeff_tf, coeff_tb, tbound;
+  int comm;  // bit mu: dimension mu is split; hops across its wrap-around are dropped
   int tbc_mask;
   R tbc_re[4], tbc_im[4];
   R alpha, mu, muinv, rho;
@@ -306,9 +307,19 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const int n_zf = c.z == g.Lz - 1 ? idx - (g.Lz - 1) * g.sz : idx + g.sz;
   const int n_tb = c.t == 0 ? idx + (g.Lt - 1) * g.st : idx - g.st;
   const int n_tf = c.t == g.Lt - 1 ? idx - (g.Lt - 1) * g.st : idx + g.st;
+  // Direction coefficients; in a split dimension the hops that wrap around
+  // the local lattice belong to the faces and are dropped here (the
+  // reference's accumulate[d] = 0 on boundary blocks, codegen/dslash.cc:248).
   const R bs = p.coeff_s;
-  const R btb = c.t == 0 ? p.coeff_tb * p.tbound : p.coeff_tb;
-  const R btf = c.t == g.Lt - 1 ? p.coeff_tf * p.tbound : p.coeff_tf;
+  const int cm = p.comm;
+  const R bxb = ((cm & 1) && c.x == 0) ? R(0) : bs;
+  const R bxf = ((cm & 1) && c.x == g.Lx - 1) ? R(0) : bs;
+  const R byb = ((cm & 2) && c.y == 0) ? R(0) : bs;
+  const R byf = ((cm & 2) && c.y == g.Ly - 1) ? R(0) : bs;
+  const R bzb = ((cm & 4) && c.z == 0) ? R(0) : bs;
+  const R bzf = ((cm & 4) && c.z == g.Lz - 1) ? R(0) : bs;
+  const R btb = c.t == 0 ? ((cm & 8) ? R(0) : p.coeff_tb * p.tbound) : p.coeff_tb;
+  const R btf = c.t == g.Lt - 1 ? ((cm & 8) ? R(0) : p.coeff_tf * p.tbound) : p.coeff_tf;
 
   cx<R> acc[4][3];
 #pragma unroll
@@ -319,9 +330,9 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const int tm = p.tbc_mask;
   const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
   const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
-  hop_mu<S, RECON, 0, ISIGN>(acc, in, gauge, vcb, idx, n_xb, n_xf, bs, bs, tm & 1, ph0);
-  hop_mu<S, RECON, 1, ISIGN>(acc, in, gauge, vcb, idx, n_yb, n_yf, bs, bs, tm & 2, ph1);
-  hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bs, bs, tm & 4, ph2);
+  hop_mu<S, RECON, 0, ISIGN>(acc, in, gauge, vcb, idx, n_xb, n_xf, bxb, bxf, tm & 1, ph0);
+  hop_mu<S, RECON, 1, ISIGN>(acc, in, gauge, vcb, idx, n_yb, n_yf, byb, byf, tm & 2, ph1);
+  hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bzb, bzf, tm & 4, ph2);
   hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3);
 
   cx<R> res[4][3];

@@ -221,6 +221,7 @@ extern "C" int qphix_amd_dslash_force(qphix_amd_ctx *ctx, int prec, int isign, i
                                       double coeff_t_b, double scale, int project, void *stream)
 {
   if (!ctx || !force_lex || !chi || !psi || !u_lex) return QPHIX_AMD_EINVAL;
+  if (ctx->comm_mask) return QPHIX_AMD_EINVAL;  // needs the whole lattice
   if ((isign != 1 && isign != -1) || (cb != 0 && cb != 1)) return QPHIX_AMD_EINVAL;
   if (prec != QPHIX_AMD_F64 && prec != QPHIX_AMD_F32) return QPHIX_AMD_EINVAL;
   ForceParams p{};

@@ -6,6 +6,7 @@
 struct qphix_amd_ctx {
   qamd::Geom g;
   double tbound;
+  int comm_mask;  // split dimensions (qphix_amd_set_comm)
   int tbc_mask;
   double tbc_re[4], tbc_im[4];
 };

@@ -165,6 +165,7 @@ int qphix_amd_eo_op_create(qphix_amd_eo_op **out_op, qphix_amd_ctx *ctx, const d
                            void *stream)
 {
   if (!out_op || !ctx || !u_lex || !clov_lex) return QPHIX_AMD_EINVAL;
+  if (ctx->comm_mask) return QPHIX_AMD_EINVAL;  // the operator needs the whole lattice
   if (prec_sloppy < QPHIX_AMD_F64 || prec_sloppy > QPHIX_AMD_BF16) return QPHIX_AMD_EINVAL;
   if (recon_sloppy != 12 && recon_sloppy != 18) return QPHIX_AMD_EINVAL;
   *out_op = nullptr;

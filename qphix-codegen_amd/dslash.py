@@ -88,6 +88,18 @@ class Geometry:
             p[2 * m], p[2 * m + 1] = complex(z).real, complex(z).imag
         check(lib().qphix_amd_set_tbc(self._ctx, t, p), "qphix_amd_set_tbc")
 
+    def set_comm(self, comm):
+        """comm: 4 flags, split dimensions (their wrap-around hops belong to
+        the faces, see face_pack / face_finish)."""
+        c = (ctypes.c_int * 4)(*[int(bool(b)) for b in comm])
+        check(lib().qphix_amd_set_comm(self._ctx, c), "qphix_amd_set_comm")
+
+    def face_sites(self, dim):
+        return int(lib().qphix_amd_face_sites(self._ctx, dim))
+
+    def new_face(self, prec, dim, device="cuda"):
+        return torch.empty(self.face_sites(dim) * 12, dtype=TORCH_DTYPE[prec], device=device)
+
     # ---- allocation ---------------------------------------------------
     def field_bytes(self, kind, prec, recon=18):
         return int(lib().qphix_amd_field_bytes(self._ctx, kind, prec, recon))
@@ -206,6 +218,36 @@ class Geometry:
                                                      _ptr(arr), cb, _stream()),
               "qphix_amd_full_clover_from_qphix")
         return f
+
+
+def face_pack(geom, buf, psi, isign, cb, dim, fb):
+    """buf <- projected half-spinors of psi (checkerboard 1-cb) on the slice
+    x_dim = 0 (fb 0, sent backwards) or L_dim - 1 (fb 1, sent forwards)."""
+    if buf.dtype != TORCH_DTYPE[psi.prec] or buf.numel() < geom.face_sites(dim) * 12:
+        raise QPhiXAMDError("face buffer of the wrong type or size")
+    check(lib().qphix_amd_face_pack(geom.ctx, psi.prec, isign, cb, dim, fb, _ptr(buf), psi.ptr(), _stream()),
+          "qphix_amd_face_pack")
+    return buf
+
+
+def face_finish(geom, kind, out, buf, u, isign, cb, dim, fb, beta, clov=None, mu=0.0, muinv=0.0):
+    """Complete the boundary sites of out (checkerboard cb) from a received
+    face; kind 0 Wilson (out -= r), 1 clover (out += clov r), 2 twisted mass
+    (out -= muinv (1 - isign i mu g5) r), 3 twisted-mass clover (out += clov r)
+    with r the reconstructed beta * U h."""
+    if buf.dtype != TORCH_DTYPE[out.prec] or buf.numel() < geom.face_sites(dim) * 12:
+        raise QPhiXAMDError("face buffer of the wrong type or size")
+    l, a = lib(), (geom.ctx, out.prec, u.recon, isign, cb, dim, fb, out.ptr(), _ptr(buf), u.ptr())
+    if kind == 0:
+        st = l.qphix_amd_face_finish(*a, float(beta), _stream())
+    elif kind == 1:
+        st = l.qphix_amd_clov_face_finish(*a, clov.ptr(), float(beta), _stream())
+    elif kind == 2:
+        st = l.qphix_amd_tm_face_finish(*a, float(beta), float(mu), float(muinv), _stream())
+    else:
+        st = l.qphix_amd_tm_clov_face_finish(*a, clov.ptr(), float(beta), _stream())
+    check(st, "face finish")
+    return out
 
 
 def qphix_layout(ft, veclen, soalen, compress12=True, pad_xy=0, pad_xyz=0):

@@ -84,21 +84,25 @@ def unpack_spinor(arr, lay, dims, cb):
     return out
 
 
-def pack_gauge(U, lay, dims, cb):
-    """U: lex (V,4,3,3,2) FP64, U_mu(x) row-major."""
-    Lx, Ly, Lz, Lt = dims
+def pack_gauge(U, lay, dims, cb, origin=(0, 0, 0, 0), gdims=None):
+    """U: lex (V,4,3,3,2) FP64, U_mu(x) row-major, on the lattice gdims
+    (default dims).  The blocks cover the sub-lattice dims at `origin` (even
+    coordinate sum); back links of its boundary sites come from across the
+    cut, as a host with a gauge halo provides them."""
+    gdims = tuple(dims) if gdims is None else tuple(gdims)
+    GX, GY, GZ, GT = gdims
     blk, lane, lex = matrix_slots(lay, dims, cb)
-    _, _, _, _, _, x = cb_sites(dims, cb)
+    _, y, z, t, _, x = cb_sites(dims, cb)
     nrow = 2 if lay.compress12 else 3
     arr = np.zeros((lay.n_matrix_blocks(dims), 8, nrow, 3, 2, lay.veclen), dtype=NP_FT[lay.ft])
-    c = np.stack([x, (lex // Lx) % Ly, (lex // (Lx * Ly)) % Lz, lex // (Lx * Ly * Lz)], axis=1)
-    L = np.array(dims)
+    c = np.stack([x, y, z, t], axis=1) + np.asarray(origin)[None, :]
+    G = np.array(gdims)
     for d in range(8):
         mu, fb = d // 2, d % 2
         n = c.copy()
         if not fb:
-            n[:, mu] = (n[:, mu] - 1) % L[mu]
-        s = ((n[:, 3] * Lz + n[:, 2]) * Ly + n[:, 1]) * Lx + n[:, 0]
+            n[:, mu] = (n[:, mu] - 1) % G[mu]
+        s = ((n[:, 3] * GZ + n[:, 2]) * GY + n[:, 1]) * GX + n[:, 0]
         M = U[s, mu]  # [site][row][col][2]
         ST = np.transpose(M, (0, 2, 1, 3))  # S[i][j] = M[j][i]
         arr[blk, d, :, :, :, lane] = ST[:, :nrow]

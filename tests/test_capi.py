@@ -63,3 +63,35 @@ def test_ctx_validation(qamd):
         assert lib.qphix_amd_error_string(-2).startswith(b"unsupported lattice")
     finally:
         lib.qphix_amd_ctx_destroy(ctx)
+
+
+def test_face_entry_points_validate(qamd):
+    lib = qamd.lib()
+    ctx = ctypes.c_void_p()
+    assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 8, 4, 6, 10, -1.0) == 0
+    try:
+        # half-spinors per face: one checkerboard of the slice
+        assert [lib.qphix_amd_face_sites(ctx, d) for d in range(4)] == [10 * 6 * 2, 10 * 6 * 4,
+                                                                       10 * 4 * 4, 6 * 4 * 4]
+        assert lib.qphix_amd_face_sites(ctx, 4) == 0
+        assert lib.qphix_amd_set_comm(ctx, None) == -1
+        buf = ctypes.create_string_buffer(256)
+        p = ctypes.c_void_p((ctypes.addressof(buf) + 15) & ~15)
+        q = ctypes.c_void_p(p.value + 64)
+        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 4, 0, p, q, None) == -1   # dim
+        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 2, p, q, None) == -1   # fb
+        assert lib.qphix_amd_face_pack(ctx, 1, 0, 0, 0, 0, p, q, None) == -1   # isign
+        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 0, p, p, None) == -1   # buf == in
+        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 0, ctypes.c_void_p(p.value + 4), q, None) == -6
+        assert lib.qphix_amd_face_finish(ctx, 1, 13, 1, 0, 0, 0, p, q, q, 1.0, None) == -1
+        assert lib.qphix_amd_clov_face_finish(ctx, 1, 12, 1, 0, 0, 0, p, q, q, None, 1.0, None) == -1
+        assert lib.qphix_amd_tm_clov_face_finish(ctx, 1, 12, 1, 0, 3, 1, p, p, q, q, 1.0, None) == -1
+        assert lib.qphix_amd_tm_face_finish(ctx, 7, 12, 1, 0, 3, 1, p, q, q, 1.0, 0.1, 0.2, None) == -1
+        # operators that need the whole lattice refuse a split context
+        comm = (ctypes.c_int * 4)(0, 0, 0, 1)
+        assert lib.qphix_amd_set_comm(ctx, comm) == 0
+        op = ctypes.c_void_p()
+        assert lib.qphix_amd_eo_op_create(ctypes.byref(op), ctx, p, p, 0.0, 1, 12, None) == -1
+        assert lib.qphix_amd_dslash_force(ctx, 0, 1, 0, p, q, q, p, 1.0, 1.0, 1.0, 1.0, 1, None) == -1
+    finally:
+        lib.qphix_amd_ctx_destroy(ctx)
