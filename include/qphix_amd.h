@@ -86,9 +86,11 @@ const char *qphix_amd_error_string(int code);
      clover      [V][2 block][6 diag + 15x2 lower-triangle]
      full clover [V][2 block][6][6][2]
    pack_gauge stores, per site x of checkerboard cb, the 8 hop matrices in
-   the reference's direction order d = 2*mu + (0 back | 1 forward):
-   U_mu(x - mu)^dagger and U_mu(x) (codegen/include/decl_common.h:13,
-   8 links per site); recon 12 keeps two rows and rebuilds the third in the
+   the reference's direction order d = 2*mu + (0 back | 1 forward), as the
+   rows of U_mu(x - mu)^dagger and of U_mu(x)^T (codegen/include/
+   decl_common.h:13, 8 links per site; the reference stores every link
+   transposed, codegen/dslash_common.cc:1678-1727); recon 12 keeps rows 0, 1
+   -- the reference's two stored rows -- and rebuilds the third in the
    kernel (codegen/dslash_common.cc:1657-1676).                          */
 int qphix_amd_pack_spinor(qphix_amd_ctx *ctx, void *dst, int prec,
                           const double *src_lex, int cb, void *stream);

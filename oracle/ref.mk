@@ -20,6 +20,8 @@
 #   avx512_f32  avx512  FP32  16
 #   avx2_f64    avx2    FP64   4
 #   avx2_f32    avx2    FP32   8
+#   avx512_f16  avx512  FP16 storage, FP32 arithmetic (ENABLE_LOW_PRECISION,
+#               codegen/CMakeLists.txt:78-83)  16
 REF ?= /root/reference/codegen
 OUT := $(abspath _ref)
 CXX ?= g++
@@ -38,31 +40,37 @@ DEFS_avx512_f64 := $(YES) -DAVX512 -DPRECISION=2 -DVECLEN=8 -DSOALEN=8
 DEFS_avx512_f32 := $(YES) -DAVX512 -DPRECISION=1 -DVECLEN=16 -DSOALEN=16
 DEFS_avx2_f64 := $(YES) -DNO_HW_MASKING -DAVX2 -DPRECISION=2 -DVECLEN=4 -DSOALEN=4
 DEFS_avx2_f32 := $(YES) -DNO_HW_MASKING -DAVX2 -DPRECISION=1 -DVECLEN=8 -DSOALEN=8
+DEFS_avx512_f16 := $(YES) -DAVX512 -DPRECISION=1 -DVECLEN=16 -DSOALEN=16 -DENABLE_LOW_PRECISION \
+                   -DUSE_LP_SPINOR -DUSE_LP_GAUGE -DUSE_LP_CLOVER
 
 ARCH_scalar_f64 := scalar
 ARCH_avx512_f64 := avx512
 ARCH_avx512_f32 := avx512
 ARCH_avx2_f64 := avx2
 ARCH_avx2_f32 := avx2
+ARCH_avx512_f16 := avx512
 
 MARCH_scalar_f64 := -march=x86-64
 MARCH_avx512_f64 := -mavx512f -mfma
 MARCH_avx512_f32 := -mavx512f -mfma
 MARCH_avx2_f64 := -mavx2 -mfma -mf16c
 MARCH_avx2_f32 := -mavx2 -mfma -mf16c
+MARCH_avx512_f16 := -mavx512f -mfma -mf16c
 
 FT_scalar_f64 := double
 FT_avx512_f64 := double
 FT_avx512_f32 := float
 FT_avx2_f64 := double
 FT_avx2_f32 := float
+FT_avx512_f16 := half
 VL_scalar_f64 := 1
 VL_avx512_f64 := 8
 VL_avx512_f32 := 16
 VL_avx2_f64 := 4
 VL_avx2_f32 := 8
+VL_avx512_f16 := 16
 
-CONFIGS := scalar_f64 avx512_f64 avx512_f32 avx2_f64 avx2_f32
+CONFIGS := scalar_f64 avx512_f64 avx512_f32 avx2_f64 avx2_f32 avx512_f16
 
 all: $(foreach c,$(CONFIGS),$(OUT)/libref_$(c).so)
 

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-CONFIGS = ("scalar_f64", "avx512_f64", "avx512_f32", "avx2_f64", "avx2_f32")
+CONFIGS = ("scalar_f64", "avx512_f64", "avx512_f32", "avx2_f64", "avx2_f32", "avx512_f16")
 _libs = {}
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -33,6 +33,7 @@ def lib(config):
         l.ref_apply.restype = i
         l.ref_veclen.restype = i
         l.ref_precision.restype = i
+        l.ref_lowprec.restype = i
         l.ref_pack_field.argtypes = [i, i, i, i, i, i, i, _dp, ctypes.c_void_p]
         l.ref_pack_field.restype = ctypes.c_long
         l.ref_soalen.restype = i
@@ -78,6 +79,10 @@ def pack(config, field, dims, src, cb, recon=18):
     n = l.ref_pack_field(field, recon, cb, *dims, _p(src), None)
     if n < 0:
         raise ValueError(f"unsupported geometry {dims} for {config}")
-    out = np.empty(n, dtype=np.float32 if l.ref_precision() == 1 else np.float64)
+    if l.ref_lowprec():
+        dt = np.float16  # IEEE binary16 bits, the reference's `half`
+    else:
+        dt = np.float32 if l.ref_precision() == 1 else np.float64
+    out = np.empty(n, dtype=dt)
     l.ref_pack_field(field, recon, cb, *dims, _p(src), out.ctypes.data_as(ctypes.c_void_p))
     return out

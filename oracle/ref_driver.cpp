@@ -39,10 +39,22 @@
   }
 #endif
 
-#if PRECISION == 1
+#if defined(USE_LP_SPINOR)
+// low-precision build: every array and every scalar is IEEE binary16
+// (codegen/data_types.h:7-39; the bodies broadcast scalars from half,
+// codegen/inst_sp_vec16.cc:190-213), converted round-to-nearest-even as the
+// kernels' own stores do (codegen/inst_sp_vec16.cc:5-7,129-135)
+typedef QPhiX::half FT;
+static inline FT to_ft(double v) { return (FT)_cvtss_sh((float)v, _MM_FROUND_TO_NEAREST_INT); }
+static inline double from_ft(FT v) { return (double)_cvtsh_ss(v); }
+#elif PRECISION == 1
 typedef float FT;
 #else
 typedef double FT;
+#endif
+#if !defined(USE_LP_SPINOR)
+static inline FT to_ft(double v) { return (FT)v; }
+static inline double from_ft(FT v) { return (double)v; }
 #endif
 
 #if SOALEN != VECLEN
@@ -128,7 +140,7 @@ void pack_spinor(FT *dst, const double *src, const Lat &g, int cb)
             for (int c = 0; c < 3; c++)
               for (int sp = 0; sp < 4; sp++)
                 for (int r = 0; r < 2; r++)
-                  b[((c * 4 + sp) * 2 + r) * S + i] = (FT)src[((s * 4 + sp) * 3 + c) * 2 + r];
+                  b[((c * 4 + sp) * 2 + r) * S + i] = to_ft(src[((s * 4 + sp) * 3 + c) * 2 + r]);
           }
         }
 }
@@ -145,7 +157,7 @@ void unpack_spinor(double *dst, const FT *src, const Lat &g, int cb)
             for (int c = 0; c < 3; c++)
               for (int sp = 0; sp < 4; sp++)
                 for (int r = 0; r < 2; r++)
-                  dst[((s * 4 + sp) * 3 + c) * 2 + r] = (double)b[((c * 4 + sp) * 2 + r) * S + i];
+                  dst[((s * 4 + sp) * 3 + c) * 2 + r] = from_ft(b[((c * 4 + sp) * 2 + r) * S + i]);
           }
         }
 }
@@ -172,7 +184,7 @@ void pack_gauge(FT *dst, const double *U, const Lat &g, int cb, int nrow)
                 for (int col = 0; col < 3; col++)
                   for (int ri = 0; ri < 2; ri++)
                     b[(((d * nrow + r) * 3 + col) * 2 + ri) * VECLEN + i] =
-                        (FT)m[(col * 3 + r) * 2 + ri];
+                        to_ft(m[(col * 3 + r) * 2 + ri]);
             }
           }
         }
@@ -193,9 +205,9 @@ void pack_clover(FT *dst, const double *cl, const Lat &g, int cb)
               const double *src = cl + s * 72 + blk * 36;
               FT *diag = b + blk * 36 * VECLEN;
               FT *off = diag + 6 * VECLEN;
-              for (int k = 0; k < 6; k++) diag[k * VECLEN + i] = (FT)src[k];
+              for (int k = 0; k < 6; k++) diag[k * VECLEN + i] = to_ft(src[k]);
               for (int k = 0; k < 15; k++)
-                for (int ri = 0; ri < 2; ri++) off[(k * 2 + ri) * VECLEN + i] = (FT)src[6 + 2 * k + ri];
+                for (int ri = 0; ri < 2; ri++) off[(k * 2 + ri) * VECLEN + i] = to_ft(src[6 + 2 * k + ri]);
             }
           }
         }
@@ -212,7 +224,7 @@ void pack_full_clover(FT *dst, const double *cl, const Lat &g, int cb)
           FT *b = dst + g.block(t, z, y, xb) * cblk;
           for (int i = 0; i < S; i++) {
             long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
-            for (int k = 0; k < 144; k++) b[k * VECLEN + i] = (FT)cl[s * 144 + k];
+            for (int k = 0; k < 144; k++) b[k * VECLEN + i] = to_ft(cl[s * 144 + k]);
           }
         }
 }
@@ -254,6 +266,14 @@ const char *ref_config(void) { return REF_CONFIG_NAME; }
 int ref_veclen(void) { return VECLEN; }
 int ref_soalen(void) { return SOALEN; }
 int ref_precision(void) { return PRECISION; }
+int ref_lowprec(void)
+{
+#if defined(USE_LP_SPINOR)
+  return 1;
+#else
+  return 0;
+#endif
+}
 
 // kind: 0 wilson, 1 clover, 2 tm, 3 tm-clover.  Inputs are lexicographic
 // FP64 arrays; psi is read on checkerboard 1-cb, chi on cb, and out is
@@ -332,13 +352,13 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
       A.gBase = gb + bi * gblk;
       A.clBase = cl ? cl + bi * cblk : nullptr;
       for (int d = 0; d < 8; d++) A.accumulate[d] = all;
-      A.alpha = (FT)alpha;
-      A.coeff_s = (FT)coeff_s;
-      A.coeff_t_b = (FT)(t == 0 ? coeff_t_b * t_boundary : coeff_t_b);
-      A.coeff_t_f = (FT)(t == Lt - 1 ? coeff_t_f * t_boundary : coeff_t_f);
-      A.mu = (FT)mu;
-      A.muinv = (FT)muinv;
-      A.prec_mass_rho = (FT)rho;
+      A.alpha = to_ft(alpha);
+      A.coeff_s = to_ft(coeff_s);
+      A.coeff_t_b = to_ft(t == 0 ? coeff_t_b * t_boundary : coeff_t_b);
+      A.coeff_t_f = to_ft(t == Lt - 1 ? coeff_t_f * t_boundary : coeff_t_f);
+      A.mu = to_ft(mu);
+      A.muinv = to_ft(muinv);
+      A.prec_mass_rho = to_ft(rho);
       fn(A);
     }
   };

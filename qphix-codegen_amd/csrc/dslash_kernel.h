@@ -105,8 +105,12 @@ __device__ __forceinline__ void reconstruct(cx<R> acc[4][3], const cx<R> w[2][3]
 }
 
 // Load the two hop matrices of direction MU at a site (d = 2 MU back,
-// 2 MU + 1 forward): rows 0,1 stored, row 2 stored (18) or rebuilt as
-// conj(row0 x row1) (12; codegen/dslash_common.cc:1664-1674).
+// 2 MU + 1 forward).  Each is stored as the rows of X, X = U_mu(x - mu)^dagger
+// for the back hop and X = U_mu(x)^T for the forward hop, so that the kept
+// rows are the reference's stored rows (it stores the transpose of each link,
+// codegen/dslash_common.cc:1678-1727): rows 0,1 stored, row 2 stored (18) or
+// rebuilt as conj(row0 x row1) (12; codegen/dslash_common.cc:1664-1674).
+// Returns the applied matrices: ub = X_back, uf = X_fwd^T.
 template <typename S, int RECON, int MU, typename R>
 __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site, cx<R> ub[3][3],
                                                cx<R> uf[3][3])
@@ -116,19 +120,28 @@ __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site
   load_reals<S, WL, 2 * MU * RECON, 2 * RECON, R, true>(gauge, vcb, site, lk);
 #pragma unroll
   for (int fb = 0; fb < 2; fb++) {
-    cx<R>(*u)[3] = fb ? uf : ub;
+    cx<R> x[3][3];
     const R *l = lk + fb * RECON;
 #pragma unroll
     for (int r = 0; r < RECON / 6; r++)
 #pragma unroll
-      for (int c = 0; c < 3; c++) u[r][c] = {l[(r * 3 + c) * 2], l[(r * 3 + c) * 2 + 1]};
+      for (int c = 0; c < 3; c++) x[r][c] = {l[(r * 3 + c) * 2], l[(r * 3 + c) * 2 + 1]};
     if (RECON == 12) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-        u[2][c] = conj(u[0][c1] * u[1][c2] - u[0][c2] * u[1][c1]);
+        x[2][c] = conj(x[0][c1] * x[1][c2] - x[0][c2] * x[1][c1]);
       }
     }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (fb)
+          uf[r][c] = x[c][r];
+        else
+          ub[r][c] = x[r][c];
+      }
   }
 }
 

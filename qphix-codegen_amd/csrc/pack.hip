@@ -3,11 +3,13 @@
 //
 // The gauge packer stores, at each site x of the checkerboard, the 8 hop
 // matrices in the reference's direction order d = 2*mu + fb
-// (codegen/dslash.cc:60-61,232-234): fb = 0 holds U_mu(x - mu)^dagger,
-// fb = 1 holds U_mu(x).  The reference keeps 8 links per site as well
-// (codegen/include/decl_common.h:13) but stores the transpose of U and
-// applies the dagger in the kernel (codegen/dslash_common.cc:1678-1727);
-// here the stored matrix is the one applied, so every hop is a plain U*h.
+// (codegen/dslash.cc:60-61,232-234): fb = 0 holds the rows of
+// U_mu(x - mu)^dagger (the matrix the back hop applies), fb = 1 the rows of
+// U_mu(x)^T (the kernel applies its transpose).  The reference keeps 8 links
+// per site as well (codegen/include/decl_common.h:13) and stores the
+// transpose of every link (codegen/dslash_common.cc:1678-1727), so with 12
+// reals both keep the same two rows -- columns 0, 1 of each link -- and
+// rebuild the same third one.
 #include "internal.h"
 
 namespace qamd {
@@ -64,9 +66,9 @@ __device__ __forceinline__ void pack_link_pair(S *dst, const double *U, const Ge
     for (int r = 0; r < RECON / 6; r++)
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        if (FB) {
-          o[(r * 3 + k) * 2] = m[(r * 3 + k) * 2];
-          o[(r * 3 + k) * 2 + 1] = m[(r * 3 + k) * 2 + 1];
+        if (FB) {  // transpose: (U^T)[r][k] = U[k][r]
+          o[(r * 3 + k) * 2] = m[(k * 3 + r) * 2];
+          o[(r * 3 + k) * 2 + 1] = m[(k * 3 + r) * 2 + 1];
         } else {  // dagger: (U^+)[r][k] = conj(U[k][r])
           o[(r * 3 + k) * 2] = m[(k * 3 + r) * 2];
           o[(r * 3 + k) * 2 + 1] = -m[(k * 3 + r) * 2 + 1];

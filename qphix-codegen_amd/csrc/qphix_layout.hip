@@ -81,9 +81,9 @@ __global__ __launch_bounds__(kLayBlock) void spinor_to_qphix_k(T *dst, const S *
         b[((col * 4 + sp) * 2 + ri) * q.soalen] = to_storage<T>(v[(sp * 3 + col) * 2 + ri]);
 }
 
-// Direction MU of the native gauge unit: the applied matrices
-// U_mu(x - mu)^dagger = conj(S_{2mu}) and U_mu(x) = S_{2mu+1}^T, where S is
-// the stored block entry (S[i][j] = M[j][i]).
+// Direction MU of the native gauge unit, the rows of U_mu(x - mu)^dagger =
+// conj(S_{2mu}) and of U_mu(x)^T = S_{2mu+1}, where S is the stored block
+// entry (S[i][j] = M[j][i]).
 template <typename S, int RECON, int NROW, int MU, typename T>
 __device__ __forceinline__ void gauge_dir(S *dst, const T *b, int V, const Geom &g, int idx)
 {
@@ -116,9 +116,9 @@ __device__ __forceinline__ void gauge_dir(S *dst, const T *b, int V, const Geom 
     for (int r = 0; r < RECON / 6; r++)
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        if (FB) {  // U = S^T
-          o[(r * 3 + k) * 2] = sr[k][r];
-          o[(r * 3 + k) * 2 + 1] = si[k][r];
+        if (FB) {  // U^T = S
+          o[(r * 3 + k) * 2] = sr[r][k];
+          o[(r * 3 + k) * 2 + 1] = si[r][k];
         } else {  // U^dagger = conj(S)
           o[(r * 3 + k) * 2] = sr[r][k];
           o[(r * 3 + k) * 2 + 1] = -si[r][k];

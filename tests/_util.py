@@ -56,3 +56,23 @@ def on_cb(a, dims, cb):
 
 def rel_err(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def f16_ulps(a, b):
+    """Distance in FP16 units in the last place between two float16 arrays
+    (sign-magnitude bit patterns mapped to a monotone integer line)."""
+    def line(x):
+        i = np.asarray(x, dtype=np.float16).view(np.int16).astype(np.int32)
+        return np.where(i < 0, -(i & 0x7FFF), i)
+    return np.abs(line(a) - line(b))
+
+
+def recon12_links(U):
+    """Links as 12-real kernels apply them: the reference keeps columns 0, 1
+    of every link (rows of its transposed storage) and rebuilds column 2 =
+    conj(col0 x col1) (codegen/dslash_common.cc:1664-1674)."""
+    z = U[..., 0] + 1j * U[..., 1]
+    c0, c1 = z[..., :, 0], z[..., :, 1]
+    z = z.copy()
+    z[..., :, 2] = np.conj(np.cross(c0, c1))
+    return np.stack([z.real, z.imag], axis=-1)

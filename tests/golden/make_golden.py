@@ -2,7 +2,7 @@
 generated kernels (qphix-codegen bodies built by oracle/ref.mk, run through
 oracle/ref_driver.cpp).  Run in a container that has /root/reference:
 
-    make -C oracle -f ref.mk && python tests/golden/make_golden.py
+    make -C oracle -f ref.mk && python tests/golden/make_golden.py [f64] [f32] [f16]
 
 ref_scalar_f64_4x4.npz   scalar FP64 kernels on a 4^4 lattice: every operator
                          family x {dslash, achimbdpsi} x two (isign, cb, recon)
@@ -10,6 +10,11 @@ ref_scalar_f64_4x4.npz   scalar FP64 kernels on a 4^4 lattice: every operator
 ref_avx512_f32_32x2.npz  AVX-512 FP32 kernels (VECLEN = SOALEN = 16) on
                          32x2x2x2, clover family, outputs on the checkerboard;
                          inputs stored as the FP32 values the kernels consumed.
+ref_avx512_f16_32x2.npz  AVX-512 low-precision kernels (FP16 storage of every
+                         field and scalar, FP32 arithmetic; ENABLE_LOW_PRECISION)
+                         on 32x2x2x2: every family x {dslash, achimbdpsi} x
+                         {(+1, cb 0, 18 reals), (-1, cb 1, 12 reals)}; inputs,
+                         scalars (exact in FP16) and outputs as FP16.
 Each file holds the inputs, the scalar parameters and the outputs.
 """
 import os
@@ -58,7 +63,48 @@ def run(config, dims, lat, kind, achi, isign, cb, recon):
                      **PARAMS)[0]
 
 
-def main():
+F16_PARAMS = dict(alpha=3.75, mu=0.03125, muinv=0.25, rho=0.125, coeff_s=1.0, coeff_t_f=0.5,
+                  coeff_t_b=0.75, t_boundary=-1.0)
+
+
+def write_f16():
+    """Fixture of the reference's FP16 (low-precision) kernels."""
+    import __graft_entry__ as ge
+    L = ge.load_package().lattice
+    dims = (32, 2, 2, 2)
+    h = lambda a: np.asarray(a).astype(np.float16)
+    clov = L.random_clover(dims, 803)
+    g = np.random.default_rng(804)
+    lat = dict(U=h(L.random_gauge(dims, 0.3, 800).numpy()), psi=h(L.random_spinor(dims, 801).numpy()),
+               chi=h(L.random_spinor(dims, 802).numpy()), clov=h(clov.numpy()),
+               inv=h(L.invert_clover(clov).numpy()),
+               fB=h(0.3 * g.normal(size=(int(np.prod(dims)), 2, 6, 6, 2)) + np.eye(6)[None, None, :, :, None]
+                    * np.array([1.0, 0.0])),
+               fF=h(0.3 * g.normal(size=(int(np.prod(dims)), 2, 6, 6, 2)) + np.eye(6)[None, None, :, :, None]
+                    * np.array([1.0, 0.0])))
+    par = np.array([(x + y + z + t) & 1 for t in range(2) for z in range(2) for y in range(2)
+                    for x in range(32)])
+    cases, outs = [], []
+    for kind in range(4):
+        for achi in (0, 1):
+            ep = {1: "clov" if achi else "inv", 3: "fF" if achi else "fB"}.get(kind)
+            for isign, cb, recon in ((1, 0, 18), (-1, 1, 12)):
+                out = ref.apply("avx512_f16", dims, lat["psi"], lat["U"], isign, cb, kind=kind, achi=achi,
+                                chi=lat["chi"], clov=None if ep is None else lat[ep], recon=recon,
+                                **F16_PARAMS)[0]
+                cases.append((kind, achi, isign, cb, recon))
+                outs.append(out[par == cb].astype(np.float16))
+    np.savez_compressed(os.path.join(HERE, "ref_avx512_f16_32x2.npz"), dims=np.array(dims),
+                        cases=np.array(cases), outs=np.array(outs),
+                        params=np.array([F16_PARAMS[k] for k in sorted(F16_PARAMS)]),
+                        param_names=np.array(sorted(F16_PARAMS)), **lat)
+
+
+def main(which=("f64", "f32", "f16")):
+    if "f16" in which:
+        write_f16()
+    if "f64" not in which and "f32" not in which:
+        return
     rng = np.random.default_rng(2024)
     dims = (4, 4, 4, 4)
     lat = lattice(dims, 500)
@@ -100,4 +146,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]) or ("f64", "f32", "f16"))

@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from _util import cb_mask
+from _util import cb_mask, f16_ulps, recon12_links
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -112,3 +112,31 @@ def test_oracle_matches_reference_kernels_live(qamd, orc, config):
                         b, _ = ref.apply(config, dims, psi, U, isign, cb, recon=recon, **kw)
                         err = np.abs(a[m] - b[m]).max() / np.abs(a[m]).max()
                         assert err < tol, (kind, achi, isign, cb, recon, err)
+
+
+def _f16_fixture_case(orc, f, case):
+    """The FP64 oracle's result for one case of the FP16 fixture, rounded to
+    FP16, with the links the 12-real kernels apply."""
+    kind, achi, isign, cb, recon = (int(c) for c in case)
+    p = params(f)
+    dims = tuple(int(d) for d in f["dims"])
+    d = {k: np.asarray(f[k], dtype=np.float64) for k in ("U", "psi", "chi", "clov", "inv", "fB", "fF")}
+    U = recon12_links(d["U"]) if recon == 12 else d["U"]
+    clov = {1: d["clov"] if achi else d["inv"], 3: d["fF"] if achi else d["fB"]}.get(kind)
+    out = orc.dslash(dims, d["psi"], U, isign, cb, kind=kind, achi=bool(achi), chi=d["chi"], clov=clov,
+                     **{k: p[k] for k in ("coeff_s", "coeff_t_f", "coeff_t_b", "t_boundary", "alpha",
+                                          "mu", "muinv", "rho")})
+    return out[cb_mask(dims, cb)]
+
+
+def test_oracle_matches_reference_avx512_f16_fixture(qamd, orc):
+    """The reference's low-precision kernels (FP16 storage of every field and
+    scalar, FP32 arithmetic) equal the exact operator on their FP16 inputs,
+    rounded to FP16: at most 1 unit in the last place anywhere, and the same
+    bits in >= 99% of the elements."""
+    f = load("ref_avx512_f16_32x2.npz")
+    for case, ref_out in zip(f["cases"], f["outs"]):
+        exact = _f16_fixture_case(orc, f, case)
+        ulps = f16_ulps(exact.astype(np.float16), ref_out)
+        assert ulps.max() <= 1, (case, ulps.max())
+        assert np.mean(ulps == 0) >= 0.99, (case, np.mean(ulps == 0))

@@ -343,3 +343,50 @@ def test_gpu_f32_matches_reference_avx512_fixture(qamd):
         m = cb_mask(dims, cb)
         err = np.abs(out[m] - ref_out).max() / np.abs(ref_out).max()
         assert err < 1e-6, (case, err)
+
+
+def test_gpu_f16_matches_reference_lowprec_fixture(qamd):
+    """FP16 storage, FP32 arithmetic, every family and kernel, 12 and 18
+    reals, against the reference's own low-precision AVX-512 kernels
+    (golden fixture): at most 1 FP16 unit in the last place, and the same
+    bits in >= 99% of the elements."""
+    from _util import f16_ulps
+    f = _fixture("ref_avx512_f16_32x2.npz")
+    dims = tuple(int(d) for d in f["dims"])
+    p = {str(k): float(v) for k, v in zip(f["param_names"], f["params"])}
+    geom = qamd.Geometry(dims, t_boundary=p["t_boundary"])
+    lex = {k: torch.from_numpy(np.asarray(f[k], dtype=np.float64)).to(DEV)
+           for k in ("U", "psi", "chi", "clov", "inv", "fB", "fF")}
+    F16 = qamd.F16
+    cs, ctf, ctb = p["coeff_s"], p["coeff_t_f"], p["coeff_t_b"]
+    lib = qamd.lib()
+    for case, ref_out in zip(f["cases"], f["outs"]):
+        kind, achi, isign, cb, recon = (int(c) for c in case)
+        g = geom.pack_gauge(lex["U"], F16, recon, cb)
+        psi = geom.pack_spinor(lex["psi"], F16, 1 - cb)
+        chi = geom.pack_spinor(lex["chi"], F16, cb)
+        res = geom.new_spinor(F16, cb=cb)
+        a = (geom.ctx, F16, recon, isign, cb, res.ptr(), psi.ptr())
+        if kind == 0:
+            rc = (lib.qphix_amd_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), p["alpha"], cs, ctf, ctb, None)
+                  if achi else lib.qphix_amd_dslash(*a, g.ptr(), cs, ctf, ctb, None))
+        elif kind == 1:
+            cl = geom.pack_clover(lex["clov" if achi else "inv"], F16, cb)
+            rc = (lib.qphix_amd_clov_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), cl.ptr(), cs, ctf, ctb,
+                                                       p["rho"], None)
+                  if achi else lib.qphix_amd_clov_dslash(*a, g.ptr(), cl.ptr(), cs, ctf, ctb, None))
+        elif kind == 2:
+            rc = (lib.qphix_amd_tm_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), p["alpha"], cs, ctf, ctb,
+                                                     p["mu"], None)
+                  if achi else lib.qphix_amd_tm_dslash(*a, g.ptr(), cs, ctf, ctb, p["mu"], p["muinv"], None))
+        else:
+            cl = geom.pack_full_clover(lex["fF" if achi else "fB"], F16, cb)
+            rc = (lib.qphix_amd_tm_clov_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), cl.ptr(), cs, ctf, ctb,
+                                                          p["rho"], None)
+                  if achi else lib.qphix_amd_tm_clov_dslash(*a, g.ptr(), cl.ptr(), cs, ctf, ctb, None))
+        assert rc == 0
+        out = geom.unpack_spinor(res).cpu().numpy()[cb_mask(dims, cb)]
+        ulps = f16_ulps(out.astype(np.float16), ref_out)
+        same = float(np.mean(ulps == 0))
+        print(f"f16 case {tuple(case)}: max {ulps.max()} ulp, identical {same:.4f}", flush=True)
+        assert ulps.max() <= 1 and same >= 0.99, (case, ulps.max(), same)
