@@ -242,6 +242,38 @@ int qphix_amd_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
                                         double coeff_t_f, double coeff_t_b,
                                         double prec_mass_rho, void *stream);
 
+/* ---- non-degenerate twisted mass -----------------------------------------
+   The reference declares an ndtm_ family (ndtm_dslash, ndtm_clov_dslash;
+   codegen/twisted_mass_enum.h:19-41, codegen/jinja/kernels-inactive.js:1-4)
+   but leaves it out of the generated variants (twisted_mass_enum.h:31-33),
+   and its bodies are copies of the degenerate ones (codegen/
+   dslash_common.cc:1599-1603,1643-1650).  These entry points give exactly
+   that: the twisted-mass / twisted-mass-clover operators under ndtm_
+   names, same arguments.                                                 */
+int qphix_amd_ndtm_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign,
+                          int cb, void *out, const void *in, const void *gauge,
+                          double coeff_s, double coeff_t_f, double coeff_t_b,
+                          double mu, double muinv, void *stream);
+int qphix_amd_ndtm_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon,
+                                     int isign, int cb, void *out,
+                                     const void *in, const void *chi,
+                                     const void *gauge, double alpha,
+                                     double coeff_s, double coeff_t_f,
+                                     double coeff_t_b, double mu,
+                                     void *stream);
+int qphix_amd_ndtm_clov_dslash(qphix_amd_ctx *ctx, int prec, int recon,
+                               int isign, int cb, void *out, const void *in,
+                               const void *gauge, const void *fclover,
+                               double coeff_s, double coeff_t_f,
+                               double coeff_t_b, void *stream);
+int qphix_amd_ndtm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
+                                          int recon, int isign, int cb,
+                                          void *out, const void *in,
+                                          const void *chi, const void *gauge,
+                                          const void *fclover, double coeff_s,
+                                          double coeff_t_f, double coeff_t_b,
+                                          double prec_mass_rho, void *stream);
+
 /* ---- faces: the lattice split over several GPUs --------------------------
    qphix_amd_set_comm marks split dimensions: comm[mu] != 0 drops, from every
    operator call above, the hops that wrap around the local lattice in mu

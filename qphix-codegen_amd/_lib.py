@@ -71,6 +71,15 @@ SIGNATURES = {
     "qphix_amd_tm_clov_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                                      _vp, _vp, _c_double, _c_double, _c_double, _c_double,
                                                      _vp]),
+    "qphix_amd_ndtm_dslash": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                       _c_double, _c_double, _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_ndtm_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                                  _c_double, _c_double, _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_ndtm_clov_dslash": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                            _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_ndtm_clov_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                                       _vp, _vp, _c_double, _c_double, _c_double, _c_double,
+                                                       _vp]),
     "qphix_amd_set_comm": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
     "qphix_amd_face_sites": (ctypes.c_long, [_vp, _c_int]),
     "qphix_amd_face_pack": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),

@@ -294,4 +294,44 @@ int qphix_amd_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon,
   return run(ctx, c, stream);
 }
 
+// Non-degenerate twisted mass: the reference declares the ndtm_ family but
+// does not generate it (codegen/twisted_mass_enum.h:19-41,
+// codegen/jinja/kernels-inactive.js:1-4), and its bodies are the degenerate
+// ones (codegen/dslash_common.cc:1599-1603,1643-1650), so these entry points
+// are the twisted-mass kernels under the ndtm_ names.
+int qphix_amd_ndtm_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb, void *out,
+                          const void *in, const void *gauge, double coeff_s, double coeff_t_f,
+                          double coeff_t_b, double mu, double muinv, void *stream)
+{
+  return qphix_amd_tm_dslash(ctx, prec, recon, isign, cb, out, in, gauge, coeff_s, coeff_t_f, coeff_t_b,
+                             mu, muinv, stream);
+}
+
+int qphix_amd_ndtm_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb,
+                                     void *out, const void *in, const void *chi, const void *gauge,
+                                     double alpha, double coeff_s, double coeff_t_f,
+                                     double coeff_t_b, double mu, void *stream)
+{
+  return qphix_amd_tm_dslash_achimbdpsi(ctx, prec, recon, isign, cb, out, in, chi, gauge, alpha, coeff_s,
+                                        coeff_t_f, coeff_t_b, mu, stream);
+}
+
+int qphix_amd_ndtm_clov_dslash(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb,
+                               void *out, const void *in, const void *gauge, const void *fclover,
+                               double coeff_s, double coeff_t_f, double coeff_t_b, void *stream)
+{
+  return qphix_amd_tm_clov_dslash(ctx, prec, recon, isign, cb, out, in, gauge, fclover, coeff_s,
+                                  coeff_t_f, coeff_t_b, stream);
+}
+
+int qphix_amd_ndtm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int recon, int isign,
+                                          int cb, void *out, const void *in, const void *chi,
+                                          const void *gauge, const void *fclover, double coeff_s,
+                                          double coeff_t_f, double coeff_t_b,
+                                          double prec_mass_rho, void *stream)
+{
+  return qphix_amd_tm_clov_dslash_achimbdpsi(ctx, prec, recon, isign, cb, out, in, chi, gauge, fclover,
+                                             coeff_s, coeff_t_f, coeff_t_b, prec_mass_rho, stream);
+}
+
 }  // extern "C"

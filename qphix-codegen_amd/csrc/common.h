@@ -154,6 +154,46 @@ template <typename R> __device__ __forceinline__ cx<R> cmul_conj(cx<R> a, cx<R> 
 {
   return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }
+// acc + a*b and acc + conj(a)*b as four fused multiply-adds (sums of complex
+// products otherwise compile to a multiply and an add per term)
+template <typename R> __device__ __forceinline__ cx<R> cmac(cx<R> acc, cx<R> a, cx<R> b)
+{
+  R re = fma(a.re, b.re, acc.re);
+  re = fma(-a.im, b.im, re);
+  R im = fma(a.re, b.im, acc.im);
+  im = fma(a.im, b.re, im);
+  return {re, im};
+}
+template <typename R> __device__ __forceinline__ cx<R> cmac_conj(cx<R> acc, cx<R> a, cx<R> b)
+{
+  R re = fma(a.re, b.re, acc.re);
+  re = fma(a.im, b.im, re);
+  R im = fma(a.re, b.im, acc.im);
+  im = fma(-a.im, b.re, im);
+  return {re, im};
+}
+// row . vec of three complex entries: u0 h0 + u1 h1 + u2 h2
+template <typename R>
+__device__ __forceinline__ cx<R> cdot3(cx<R> u0, cx<R> u1, cx<R> u2, cx<R> h0, cx<R> h1, cx<R> h2)
+{
+  cx<R> r = {u0.re * h0.re, u0.re * h0.im};
+  r.re = fma(-u0.im, h0.im, r.re);
+  r.im = fma(u0.im, h0.re, r.im);
+  return cmac(cmac(r, u1, h1), u2, h2);
+}
+// conj(p q - r s) (the third row of an SU(3) matrix from its first two)
+template <typename R> __device__ __forceinline__ cx<R> conj_diff(cx<R> p, cx<R> q, cx<R> r, cx<R> s)
+{
+  R re = p.re * q.re;
+  re = fma(-p.im, q.im, re);
+  re = fma(-r.re, s.re, re);
+  re = fma(r.im, s.im, re);
+  R im = -p.re * q.im;
+  im = fma(-p.im, q.re, im);
+  im = fma(r.re, s.im, im);
+  im = fma(r.im, s.re, im);
+  return {re, im};
+}
 
 // ---- checkerboard geometry ------------------------------------------------
 // cb index i = ((t*Lz + z)*Ly + y)*Lxh + xh,  x = 2 xh + ((cb+y+z+t) & 1).

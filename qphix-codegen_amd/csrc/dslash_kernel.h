@@ -80,26 +80,28 @@ __device__ __forceinline__ void project(const R *psi, cx<R> h[2][3])
   }
 }
 
-// Accumulate the reconstructed 4-spinor of (1 + SGN*gamma_MU) from w = U h.
+// Accumulate b times the reconstructed 4-spinor of (1 + SGN*gamma_MU) from
+// w = U h, b folded into fused multiply-adds.
 template <int MU, int SGN, typename R>
-__device__ __forceinline__ void reconstruct(cx<R> acc[4][3], const cx<R> w[2][3])
+__device__ __forceinline__ void reconstruct(cx<R> acc[4][3], const cx<R> w[2][3], R b)
 {
+  const R sb = SGN > 0 ? b : -b;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    acc[0][c] = acc[0][c] + w[0][c];
-    acc[1][c] = acc[1][c] + w[1][c];
+    acc[0][c] = {fma(b, w[0][c].re, acc[0][c].re), fma(b, w[0][c].im, acc[0][c].im)};
+    acc[1][c] = {fma(b, w[1][c].re, acc[1][c].re), fma(b, w[1][c].im, acc[1][c].im)};
     if (MU == 0) {  // r2 = -s i w1, r3 = -s i w0
-      acc[2][c] = SGN > 0 ? acc[2][c] - timesI(w[1][c]) : acc[2][c] + timesI(w[1][c]);
-      acc[3][c] = SGN > 0 ? acc[3][c] - timesI(w[0][c]) : acc[3][c] + timesI(w[0][c]);
+      acc[2][c] = {fma(sb, w[1][c].im, acc[2][c].re), fma(-sb, w[1][c].re, acc[2][c].im)};
+      acc[3][c] = {fma(sb, w[0][c].im, acc[3][c].re), fma(-sb, w[0][c].re, acc[3][c].im)};
     } else if (MU == 1) {  // r2 = s w1, r3 = -s w0
-      acc[2][c] = SGN > 0 ? acc[2][c] + w[1][c] : acc[2][c] - w[1][c];
-      acc[3][c] = SGN > 0 ? acc[3][c] - w[0][c] : acc[3][c] + w[0][c];
+      acc[2][c] = {fma(sb, w[1][c].re, acc[2][c].re), fma(sb, w[1][c].im, acc[2][c].im)};
+      acc[3][c] = {fma(-sb, w[0][c].re, acc[3][c].re), fma(-sb, w[0][c].im, acc[3][c].im)};
     } else if (MU == 2) {  // r2 = -s i w0, r3 = s i w1
-      acc[2][c] = SGN > 0 ? acc[2][c] - timesI(w[0][c]) : acc[2][c] + timesI(w[0][c]);
-      acc[3][c] = SGN > 0 ? acc[3][c] + timesI(w[1][c]) : acc[3][c] - timesI(w[1][c]);
+      acc[2][c] = {fma(sb, w[0][c].im, acc[2][c].re), fma(-sb, w[0][c].re, acc[2][c].im)};
+      acc[3][c] = {fma(-sb, w[1][c].im, acc[3][c].re), fma(sb, w[1][c].re, acc[3][c].im)};
     } else {  // r2 = s w0, r3 = s w1
-      acc[2][c] = SGN > 0 ? acc[2][c] + w[0][c] : acc[2][c] - w[0][c];
-      acc[3][c] = SGN > 0 ? acc[3][c] + w[1][c] : acc[3][c] - w[1][c];
+      acc[2][c] = {fma(sb, w[0][c].re, acc[2][c].re), fma(sb, w[0][c].im, acc[2][c].im)};
+      acc[3][c] = {fma(sb, w[1][c].re, acc[3][c].re), fma(sb, w[1][c].im, acc[3][c].im)};
     }
   }
 }
@@ -130,7 +132,7 @@ __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-        x[2][c] = conj(x[0][c1] * x[1][c2] - x[0][c2] * x[1][c1]);
+        x[2][c] = conj_diff(x[0][c1], x[1][c2], x[0][c2], x[1][c1]);
       }
     }
 #pragma unroll
@@ -160,20 +162,14 @@ __device__ __forceinline__ void hop_dir(cx<R> acc[4][3], const S *__restrict__ i
 #pragma unroll
   for (int k = 0; k < 2; k++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) w[k][i] = u[i][0] * h[k][0] + u[i][1] * h[k][1] + u[i][2] * h[k][2];
+    for (int i = 0; i < 3; i++) w[k][i] = cdot3(u[i][0], u[i][1], u[i][2], h[k][0], h[k][1], h[k][2]);
   if (tbc) {
-    cx<R> f = {beta * phase.re, beta * phase.im};
 #pragma unroll
     for (int k = 0; k < 2; k++)
 #pragma unroll
-      for (int i = 0; i < 3; i++) w[k][i] = f * w[k][i];
-  } else {
-#pragma unroll
-    for (int k = 0; k < 2; k++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) w[k][i] = beta * w[k][i];
+      for (int i = 0; i < 3; i++) w[k][i] = phase * w[k][i];
   }
-  reconstruct<MU, SGN>(acc, w);
+  reconstruct<MU, SGN>(acc, w, beta);
 }
 
 template <typename S, int RECON, int MU, int ISIGN, typename R>
@@ -209,11 +205,11 @@ __device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
         if (c < r) {
           const int od = r * (r - 1) / 2 + c;
           cx<R> a = {blk[6 + 2 * od], blk[6 + 2 * od + 1]};
-          res = res + a * v[c];
+          res = cmac(res, a, v[c]);
         } else if (c > r) {
           const int od = c * (c - 1) / 2 + r;
           cx<R> a = {blk[6 + 2 * od], blk[6 + 2 * od + 1]};
-          res = res + cmul_conj(a, v[c]);
+          res = cmac_conj(res, a, v[c]);
         }
       }
       out[2 * b + r / 3][r % 3] = res;
@@ -240,11 +236,12 @@ __device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
     for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
 #pragma unroll
     for (int r = 0; r < 6; r++) {
-      cx<R> res = {R(0), R(0)};
+      cx<R> a0 = {blk[(r * 6) * 2], blk[(r * 6) * 2 + 1]};
+      cx<R> res = a0 * v[0];
 #pragma unroll
-      for (int c = 0; c < 6; c++) {
+      for (int c = 1; c < 6; c++) {
         cx<R> a = {blk[(r * 6 + c) * 2], blk[(r * 6 + c) * 2 + 1]};
-        res = res + a * v[c];
+        res = cmac(res, a, v[c]);
       }
       out[2 * b + r / 3][r % 3] = res;
     }

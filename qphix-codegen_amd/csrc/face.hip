@@ -94,15 +94,16 @@ __device__ __forceinline__ void project_rt(int mu, int sgn, const R *psi, cx<R> 
 template <typename R>
 __device__ __forceinline__ void reconstruct_rt(int mu, int sgn, cx<R> acc[4][3], const cx<R> w[2][3])
 {
+  const R one = R(1);
   switch (mu * 2 + (sgn > 0)) {
-    case 0: reconstruct<0, -1>(acc, w); break;
-    case 1: reconstruct<0, 1>(acc, w); break;
-    case 2: reconstruct<1, -1>(acc, w); break;
-    case 3: reconstruct<1, 1>(acc, w); break;
-    case 4: reconstruct<2, -1>(acc, w); break;
-    case 5: reconstruct<2, 1>(acc, w); break;
-    case 6: reconstruct<3, -1>(acc, w); break;
-    default: reconstruct<3, 1>(acc, w); break;
+    case 0: reconstruct<0, -1>(acc, w, one); break;
+    case 1: reconstruct<0, 1>(acc, w, one); break;
+    case 2: reconstruct<1, -1>(acc, w, one); break;
+    case 3: reconstruct<1, 1>(acc, w, one); break;
+    case 4: reconstruct<2, -1>(acc, w, one); break;
+    case 5: reconstruct<2, 1>(acc, w, one); break;
+    case 6: reconstruct<3, -1>(acc, w, one); break;
+    default: reconstruct<3, 1>(acc, w, one); break;
   }
 }
 

@@ -390,3 +390,38 @@ def test_gpu_f16_matches_reference_lowprec_fixture(qamd):
         same = float(np.mean(ulps == 0))
         print(f"f16 case {tuple(case)}: max {ulps.max()} ulp, identical {same:.4f}", flush=True)
         assert ulps.max() <= 1 and same >= 0.99, (case, ulps.max(), same)
+
+
+def test_ndtm_entry_points_are_the_degenerate_kernels(qamd):
+    """The reference's ndtm_ bodies are its degenerate twisted-mass bodies
+    (codegen/dslash_common.cc:1599-1603,1643-1650): the ndtm_ entry points
+    give bit-identical results to the tm_ ones."""
+    dims = (8, 4, 4, 4)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims)
+    U = L.random_gauge(dims, 0.3, 901).to(DEV)
+    psi_l, chi_l = L.random_spinor(dims, 902).to(DEV), L.random_spinor(dims, 903).to(DEV)
+    fcl = L.tmclover_blocks(L.random_clover(dims, 904), 0.1, 1, True).to(DEV)
+    lib = qamd.lib()
+    for prec in (qamd.F64, qamd.F16):
+        for isign, cb in ((1, 0), (-1, 1)):
+            g = geom.pack_gauge(U, prec, 12, cb)
+            psi, chi = geom.pack_spinor(psi_l, prec, 1 - cb), geom.pack_spinor(chi_l, prec, cb)
+            cl = geom.pack_full_clover(fcl, prec, cb)
+            outs = []
+            for nd in ("tm", "ndtm"):
+                a = (geom.ctx, prec, 12, isign, cb)
+                r = [geom.new_spinor(prec, cb=cb) for _ in range(4)]
+                assert getattr(lib, f"qphix_amd_{nd}_dslash")(*a, r[0].ptr(), psi.ptr(), g.ptr(), 0.9, 1.1, 1.2,
+                                                             0.03, 0.26, None) == 0
+                assert getattr(lib, f"qphix_amd_{nd}_dslash_achimbdpsi")(*a, r[1].ptr(), psi.ptr(), chi.ptr(),
+                                                                         g.ptr(), 3.7, 0.9, 1.1, 1.2, 0.03,
+                                                                         None) == 0
+                assert getattr(lib, f"qphix_amd_{nd}_clov_dslash")(*a, r[2].ptr(), psi.ptr(), g.ptr(), cl.ptr(),
+                                                                  0.9, 1.1, 1.2, None) == 0
+                assert getattr(lib, f"qphix_amd_{nd}_clov_dslash_achimbdpsi")(*a, r[3].ptr(), psi.ptr(),
+                                                                              chi.ptr(), g.ptr(), cl.ptr(),
+                                                                              0.9, 1.1, 1.2, 0.05, None) == 0
+                outs.append([x.data.cpu() for x in r])
+            for a_, b_ in zip(*outs):
+                assert torch.equal(a_, b_)
