@@ -121,6 +121,60 @@ def profile_traffic():
     return None
 
 
+def copy_bandwidth(dev, nbytes=1 << 30, reps=10):
+    """Device-to-device copy of nbytes (read + write), GB/s: the box's
+    practical ceiling for a stream that reads and writes."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    t = time_kernel(lambda i: b.copy_(a), reps)
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2 * nbytes / t / 1e9, 1)
+
+
+def sloppy_kernels(q, dev):
+    """The CG's sloppy operator kernels (FP16 and BF16 storage, FP32
+    arithmetic, 12-real links) at 32^3x64, HIP-event averages."""
+    import torch
+    L = q.lattice
+    geom = q.Geometry(DIMS, t_boundary=-1.0)
+    vcb = geom.vcb
+    U = L.random_gauge(DIMS, 0.3, 1000, device=dev)
+    clov = L.random_clover(DIMS, 1003, device=dev)
+    inv = L.invert_clover(clov)
+    psi = L.random_spinor(DIMS, 1001, device=dev)
+    op = q.ClovDslash(geom)
+    out = {}
+    for name, prec in (("f16", q.F16), ("bf16", q.BF16)):
+        sets = [dict(g0=geom.pack_gauge(U, prec, 12, 0), g1=geom.pack_gauge(U, prec, 12, 1),
+                     inv0=geom.pack_clover(inv, prec, 0), clov1=geom.pack_clover(clov, prec, 1),
+                     psi1=geom.pack_spinor(psi, prec, 1), t0=geom.new_spinor(prec, cb=0),
+                     out1=geom.new_spinor(prec, cb=1)) for _ in range(2)]
+
+        def kd(i):
+            s = sets[i & 1]
+            op.dslash(s["t0"], s["psi1"], s["g0"], s["inv0"], 1, 0)
+
+        def ka(i):
+            s = sets[i & 1]
+            op.dslashAChiMinusBDPsi(s["out1"], s["t0"], s["psi1"], s["g1"], s["clov1"], 0.25, 1, 1)
+
+        for k in (kd, ka):
+            for i in range(5):
+                k(i)
+        td, ta = time_kernel(kd, 100), time_kernel(ka, 100)
+        bd, ba = algorithmic_bytes(12, 72, False, 2) * vcb, algorithmic_bytes(12, 72, True, 2) * vcb
+        out[name] = {"clov_dslash_us": round(td * 1e6, 2),
+                     "clov_dslash_hbm_frac": round(bd / td / 1e9 / PEAK_HBM_GBS, 4),
+                     "clov_dslash_achimbdpsi_us": round(ta * 1e6, 2),
+                     "clov_dslash_achimbdpsi_hbm_frac": round(ba / ta / 1e9 / PEAK_HBM_GBS, 4)}
+        del sets
+    del U, clov, inv, psi
+    torch.cuda.empty_cache()
+    return out
+
+
 def cg_compare(q, dev):
     """Config 2: even-odd Wilson-clover M^dag M at 32^3x64, kappa = 0.1350,
     csw = 1.0, links sigma = 0.2; FP64 CG vs mixed-precision reliable-update
@@ -313,6 +367,8 @@ def main():
 
     t_d = time_kernel(k_dslash, args.steps)
     t_a = time_kernel(k_achi, args.steps)
+    # naive count (every neighbour spinor read from memory), as often quoted
+    b_naive = (8 * 24 + 8 * recon + 72 + 24) * BYTES_F32 * vcb
     b_d = algorithmic_bytes(recon, 72, False, BYTES_F32) * vcb
     b_a = algorithmic_bytes(recon, 72, True, BYTES_F32) * vcb
     flops = vcb * (FLOP_CLOVER + FLOP_CLOVER_ACHI)
@@ -349,14 +405,18 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": profile_traffic(),
                      "kernel": "clov_dslash f32 recon12",
-                     "algorithmic_bytes_per_launch": b_d},
+                     "algorithmic_bytes_per_launch": b_d,
+                     "naive_bytes_per_launch": b_naive,
+                     "naive_gbs": round(b_naive / t_d / 1e9, 1)},
         "cpu_baseline": None,
         "cg": None,
     }
     del sets
     torch.cuda.empty_cache()
+    line["roofline"]["measured_copy_gbs"] = copy_bandwidth(dev)
     if not args.no_cg:
         line["cg"] = cg_compare(q, dev)
+        line["cg"]["sloppy_kernels"] = sloppy_kernels(q, dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

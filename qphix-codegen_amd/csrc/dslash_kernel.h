@@ -46,7 +46,7 @@ template <typename R> struct DslashParams {
 #define QAMD_BLOCK_ORDER 0
 #endif
 #ifndef QAMD_DSLASH_BLOCK
-#define QAMD_DSLASH_BLOCK 256
+#define QAMD_DSLASH_BLOCK 128
 #endif
 constexpr int kDslashBlock = QAMD_DSLASH_BLOCK;
 // Minimum waves per SIMD requested from the register allocator (0 = none).
