@@ -147,3 +147,30 @@ def test_reunitarize(qamd):
     assert (torch.linalg.det(z) - 1).abs().max().item() < 1e-14
     same = qamd.reunitarize(geom, U.clone().to(DEV)).cpu()
     assert (same - U).abs().max().item() < 1e-14
+
+
+def test_block_inverses_of_indefinite_hermitian_blocks(qamd):
+    """The block inverses take no positive-definiteness for granted:
+    Hermitian blocks with eigenvalues of both signs invert to FP64 accuracy,
+    plain and twisted (the kernels form (A - i m)(A^2 + m^2)^-1, whose error
+    grows like cond(A)^2 eps; these blocks have cond(A) = 4, clover blocks
+    near alpha 1 about 1.5-3)."""
+    dims = (4, 4, 4, 4)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims)
+    g = torch.Generator().manual_seed(77)
+    V = 256
+    q, _ = torch.linalg.qr(torch.randn(V, 2, 6, 6, dtype=torch.complex128, generator=g))
+    ev = torch.tensor([3.0, -2.0, 1.0, -1.5, 2.5, -4.0], dtype=torch.float64)
+    dense = q @ torch.diag_embed(ev.to(torch.complex128)) @ q.conj().transpose(-1, -2)
+    clov = L.clover_pack(0.5 * (dense + dense.conj().transpose(-1, -2)))
+    ref = torch.linalg.inv(L.clover_dense(clov))
+    got = L.clover_dense(qamd.clover_invert(geom, clov.to(DEV)).cpu())
+    assert (got - ref).abs().max().item() / ref.abs().max().item() < 1e-13
+    for sign in (1, -1):
+        blk = qamd.tmclover_blocks(geom, clov.to(DEV), 0.3, sign, 1).cpu()
+        g5 = torch.tensor([1.0, -1.0], dtype=torch.float64).view(1, 2, 1, 1)
+        twisted = L.clover_dense(clov) + 1j * sign * 0.3 * g5 * torch.eye(6, dtype=torch.complex128)
+        rf = torch.linalg.inv(twisted)
+        gz = torch.complex(blk[..., 0], blk[..., 1]).reshape(V, 2, 6, 6)
+        assert (gz - rf).abs().max().item() / rf.abs().max().item() < 1e-13
