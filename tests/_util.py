@@ -76,3 +76,16 @@ def recon12_links(U):
     z = z.copy()
     z[..., :, 2] = np.conj(np.cross(c0, c1))
     return np.stack([z.real, z.imag], axis=-1)
+
+
+def f16_within(got, want):
+    """Elementwise agreement of two FP16 results of the same FP32-arithmetic
+    operator: within one FP16 unit in the last place of `want`, or within
+    2^-20 of the largest |want| where cancellation leaves a small value (two
+    summation orders in FP32 differ by about 2^-24 of the summands there,
+    more than one FP16 ulp of a near-zero result)."""
+    w = np.asarray(want, dtype=np.float16)
+    g = np.asarray(got, dtype=np.float16)
+    diff = np.abs(g.astype(np.float64) - w.astype(np.float64))
+    ulp = np.spacing(np.abs(w)).astype(np.float64)
+    return diff <= np.maximum(ulp, 2.0 ** -20 * np.abs(w.astype(np.float64)).max())
