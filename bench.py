@@ -121,6 +121,49 @@ def profile_traffic():
     return None
 
 
+def tm_kernel(q, geom, s, dev, n):
+    """Config 1's twisted-mass variant: tm_dslash, FP32, 12-real links,
+    mu = 0.1, kappa = 0.1350 (alpha = 1/(2 kappa); the kernels take mu/alpha
+    and alpha/(alpha^2 + mu^2))."""
+    alpha, mu = 1.0 / (2 * 0.1350), 0.1
+    op = q.TMDslash(geom, alpha - 4.0, mu)
+    vcb = geom.vcb
+
+    def k(i):
+        op.dslash(s["t0"], s["psi1"], s["g0"], 1, 0)
+
+    for i in range(5):
+        k(i)
+    t = time_kernel(k, n)
+    b = algorithmic_bytes(12, 0, False, BYTES_F32) * vcb
+    return {"us": round(t * 1e6, 2), "gflops": round(vcb * 1392 / t / 1e9, 1),
+            "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4), "flop_per_site": 1392}
+
+
+def f32_error_vs_f64(q, geom, s, dev):
+    """Per-site max-abs difference of the FP32 clov_dslash from the FP64 one
+    on the same inputs (the FP64 kernels equal the FP64 oracle to 1e-13 in
+    the tests), absolute and relative to the largest output element."""
+    import torch
+    L = q.lattice
+    U = L.random_gauge(DIMS, 0.3, 1000, device=dev)
+    clov = L.random_clover(DIMS, 1003, device=dev)
+    inv = L.invert_clover(clov)
+    psi = L.random_spinor(DIMS, 1001, device=dev)
+    op = q.ClovDslash(geom)
+    outs = []
+    for prec in (q.F64, q.F32):
+        t = geom.new_spinor(prec, cb=0)
+        op.dslash(t, geom.pack_spinor(psi, prec, 1), geom.pack_gauge(U, prec, 12, 0),
+                  geom.pack_clover(inv, prec, 0), 1, 0)
+        outs.append(geom.unpack_spinor(t))
+    d = (outs[1] - outs[0]).abs().max().item()
+    m = outs[0].abs().max().item()
+    del U, clov, inv, psi, outs
+    torch.cuda.empty_cache()
+    return {"max_abs": d, "max_abs_rel": d / m}
+
+
 def copy_bandwidth(dev, nbytes=1 << 30, reps=10):
     """Device-to-device copy of nbytes (read + write), GB/s: the box's
     practical ceiling for a stream that reads and writes."""
@@ -367,6 +410,8 @@ def main():
 
     t_d = time_kernel(k_dslash, args.steps)
     t_a = time_kernel(k_achi, args.steps)
+    tm = tm_kernel(q, geom, sets[0], dev, args.steps)
+    err = f32_error_vs_f64(q, geom, sets[0], dev)
     # naive count (every neighbour spinor read from memory), as often quoted
     b_naive = (8 * 24 + 8 * recon + 72 + 24) * BYTES_F32 * vcb
     b_d = algorithmic_bytes(recon, 72, False, BYTES_F32) * vcb
@@ -400,6 +445,8 @@ def main():
             "kernel_hbm_frac": {"clov_dslash": round(b_d / t_d / 1e9 / PEAK_HBM_GBS, 4),
                                 "clov_dslash_achimbdpsi": round(b_a / t_a / 1e9 / PEAK_HBM_GBS, 4)},
             "parallelism": "single GPU",
+            "tm_dslash": tm,
+            "clov_dslash_f32_error_vs_f64": err,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
