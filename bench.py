@@ -255,8 +255,8 @@ def tmclover48(args, q, dev):
         FP16 full clover blocks, FP32 arithmetic) and in FP32;
       * FP64 CG vs mixed-precision CG (FP16 storage) to a true residual of
         1e-10 on M^dag M of the even-odd TM-clover operator;
-      * the hopping-term gauge force at the solution: from the mixed solve
-        vs from the FP64 solve (FP64 force kernels)."""
+      * the pseudofermion force at the solution (hopping and clover terms):
+        from the mixed solve vs from the FP64 solve (FP64 force kernels)."""
     import torch
     L = q.lattice
     dims = (48, 48, 48, 96)
@@ -313,7 +313,7 @@ def tmclover48(args, q, dev):
                              "reliable_updates": res["reliable_updates"], "rsd_final": res["rsd_final"]}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        forces[name] = op.force(torch.zeros_like(U), x, U, project=True)
+        forces[name] = op.force(torch.zeros_like(U), x, U, project=True, csw=csw)
         torch.cuda.synchronize()
         out[f"cg_{name}"]["force_seconds"] = round(time.perf_counter() - t0, 4)
         op.close()

@@ -417,9 +417,31 @@ typedef struct {
    accumulated as scale * (per link) into force_lex like
    qphix_amd_dslash_force (project = 1: su(3) force, 0: link gradient).
    This is the whole gradient when F does not depend on the links (csw = 0);
-   the clover-term derivative is not included.                           */
+   qphix_amd_eo_op_force_clover adds the clover-term derivative.         */
 int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
                           const double *u_lex, double scale, int project, void *stream);
+
+/* The whole pseudofermion force of the operator when its clover field is
+   A = alpha + (csw/2) sum sigma F from qphix_amd_clover_from_links of the
+   same links (any alpha; the twisted mass of the operator does not depend
+   on the links): the hopping term of qphix_amd_eo_op_force plus the clover
+   term dS = -2 Re <Y, dA_oo X> - 1/2 Re <W, dA_ee Z>.                   */
+int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex,
+                                 const void *x, const double *u_lex,
+                                 double csw, double scale, int project,
+                                 void *stream);
+
+/* Link derivative of the clover term on its own: accumulates into
+   force_lex the gradient of L = Re sum_x w(x) chi(x)^dag A(x) psi(x), A as
+   built by qphix_amd_clover_from_links (csw; alpha drops out), w(x) =
+   scale_even / scale_odd by the parity of x; chi, psi lexicographic FP64
+   spinors on the whole lattice.  project as for qphix_amd_dslash_force.
+   Every link lies on 24 plaquette leaves (2 squares in each of its 3
+   planes, each a leaf of its 4 corners), all gathered by one thread.    */
+int qphix_amd_clover_force(qphix_amd_ctx *ctx, double *force_lex,
+                           const double *chi_lex, const double *psi_lex,
+                           const double *u_lex, double csw, double scale_even,
+                           double scale_odd, int project, void *stream);
 
 /* Solves M^dag M x = b; x, b are FP64 native spinors on cb 1.  Returns
    QPHIX_AMD_ENOCONV (with res filled) when max_iters is reached.         */

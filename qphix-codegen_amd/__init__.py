@@ -7,7 +7,7 @@ from ._lib import (BF16, F16, F32, F64, FIELD_CLOVER, FIELD_FULLCLOVER, FIELD_GA
 from .dslash import (PREC_NAME, TORCH_DTYPE, ClovDslash, Dslash, Field, Geometry,  # noqa: F401
                      TMClovDslash, TMDslash, face_finish, face_pack, qphix_layout)
 from . import lattice  # noqa: F401
-from .solver import (EvenOddCloverOperator, InvCG, clover_from_links, clover_invert,  # noqa: F401
+from .solver import (EvenOddCloverOperator, InvCG, clover_force, clover_from_links, clover_invert,  # noqa: F401
                      reunitarize, tmclover_blocks)
 from .autograd import (DslashFunction, PseudofermionAction, dslash_adjoint, dslash_force,  # noqa: F401
                        su3_force_dslash, su3_force_pseudofermion)

@@ -3,14 +3,17 @@
 DslashFunction       out = D psi (Wilson hop, lexicographic FP64 in/out).
                      backward: dL/dpsi = D^dagger grad (qphix_amd_dslash_adjoint),
                      dL/dU = G^dagger (qphix_amd_dslash_force, project = 0).
-PseudofermionAction  S = phi^dagger (M^dagger M)^-1 phi with the even-odd Wilson
-                     operator M = alpha - 1/(4 alpha) D_oe D_eo on cb 1,
-                     alpha = 1/(2 kappa).  backward recomputes the solve
-                     (nothing solver-sized is kept between forward and
-                     backward) and calls qphix_amd_eo_op_force:
-                       dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>],
-                     X = (M^dag M)^-1 phi, Y = M X, Z = D_eo X / alpha,
-                     W = D_oe^dag Y / alpha.
+PseudofermionAction  S = phi^dagger (M^dagger M)^-1 phi with the even-odd
+                     operator M = F_oo - 1/4 D_oe F_ee^-1 D_eo on cb 1,
+                     F = A + i mu g5, A = clover_from_links(1/(2 kappa), csw)
+                     (csw = mu = 0: M = alpha - 1/(4 alpha) D_oe D_eo).
+                     backward recomputes the solve (nothing solver-sized is
+                     kept between forward and backward) and calls
+                     qphix_amd_eo_op_force(_clover):
+                       dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>]
+                            - 2 Re <Y, dA_oo X> - 1/2 Re <W, dA_ee Z>,
+                     X = (M^dag M)^-1 phi, Y = M X, Z = F_ee^-1 D_eo X,
+                     W = (F_ee^-1)^dag D_oe^dag Y.
 su3_force(...)       the traceless anti-Hermitian su(3) force of either loss,
                      d/de L(exp(i e T) U) = sum_links Re tr(i T F).
 """
@@ -73,46 +76,48 @@ class DslashFunction(torch.autograd.Function):
         return grad_psi, grad_U, None, None, None, None, None
 
 
-def _wilson_eo_operator(geom, U, kappa):
+def _eo_operator(geom, U, kappa, csw=0.0, mu=0.0):
     alpha = 1.0 / (2.0 * kappa)
-    A = clover_from_links(geom, U, alpha, 0.0)  # alpha * identity
-    return EvenOddCloverOperator(geom, U, A, prec_sloppy=F64), alpha
+    A = clover_from_links(geom, U, alpha, csw)  # alpha * identity for csw = 0
+    return EvenOddCloverOperator(geom, U, A, mu=mu, prec_sloppy=F64), alpha
 
 
-def _pf_solution(geom, U, phi, kappa, rsd):
-    op, alpha = _wilson_eo_operator(geom, U, kappa)
+def _pf_solution(geom, U, phi, kappa, rsd, csw=0.0, mu=0.0):
+    op, alpha = _eo_operator(geom, U, kappa, csw, mu)
     b = geom.pack_spinor(phi, F64, 1)
     x = geom.new_spinor(F64, cb=1)
     InvCG(op, max_iters=100000)(x, b, rsd)
     return op, alpha, b, x
 
 
-def _pf_force(geom, U, phi, kappa, rsd, scale, project):
-    op, alpha, b, X = _pf_solution(geom, U, phi, kappa, rsd)
-    F = op.force(torch.zeros_like(U), X, U, scale=scale, project=project)
+def _pf_force(geom, U, phi, kappa, rsd, scale, project, csw=0.0, mu=0.0):
+    op, alpha, b, X = _pf_solution(geom, U, phi, kappa, rsd, csw, mu)
+    F = op.force(torch.zeros_like(U), X, U, scale=scale, project=project, csw=csw if csw else None)
     op.close()
     return F
 
 
 class PseudofermionAction(torch.autograd.Function):
-    """S = phi^dag (M^dag M)^-1 phi (even-odd Wilson, FP64 CG to rsd)."""
+    """S = phi^dag (M^dag M)^-1 phi, M the even-odd clover / twisted-mass
+    clover operator with A = clover_from_links(U, 1/(2 kappa), csw) (FP64 CG
+    to rsd); csw = 0, mu = 0 is plain Wilson."""
 
     @staticmethod
-    def forward(ctx, U, phi, geom, kappa, rsd=1e-13):
+    def forward(ctx, U, phi, geom, kappa, rsd=1e-13, csw=0.0, mu=0.0):
         U = U.detach().contiguous()
-        op, alpha, b, x = _pf_solution(geom, U, phi.detach(), kappa, rsd)
+        op, alpha, b, x = _pf_solution(geom, U, phi.detach(), kappa, rsd, csw, mu)
         S = torch.dot(b.data, x.data)
         op.close()
         ctx.save_for_backward(U, phi)
-        ctx.cfg = (geom, kappa, rsd)
+        ctx.cfg = (geom, kappa, rsd, csw, mu)
         return S
 
     @staticmethod
     def backward(ctx, gS):
         U, phi = ctx.saved_tensors
-        geom, kappa, rsd = ctx.cfg
-        F = _pf_force(geom, U, phi.detach(), kappa, rsd, float(gS), project=False)
-        return F, None, None, None, None
+        geom, kappa, rsd, csw, mu = ctx.cfg
+        F = _pf_force(geom, U, phi.detach(), kappa, rsd, float(gS), False, csw, mu)
+        return F, None, None, None, None, None, None
 
 
 def su3_force_dslash(geom, chi_lex, psi_lex, U, isign, cb, prec=F64):
@@ -122,6 +127,6 @@ def su3_force_dslash(geom, chi_lex, psi_lex, U, isign, cb, prec=F64):
     return dslash_force(geom, torch.zeros_like(U), chi, psi, U.contiguous(), isign, cb, project=True)
 
 
-def su3_force_pseudofermion(geom, U, phi, kappa, rsd=1e-13):
+def su3_force_pseudofermion(geom, U, phi, kappa, rsd=1e-13, csw=0.0, mu=0.0):
     """Traceless anti-Hermitian force of S = phi^dag (M^dag M)^-1 phi."""
-    return _pf_force(geom, U.contiguous(), phi, kappa, rsd, 1.0, project=True)
+    return _pf_force(geom, U.contiguous(), phi, kappa, rsd, 1.0, True, csw, mu)

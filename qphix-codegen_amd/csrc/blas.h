@@ -34,5 +34,10 @@ hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const d
                                  double alpha, double csw, hipStream_t st);
 hipError_t run_clover_blocks(const qphix_amd_ctx *ctx, double *out, const double *clov, int mode,
                              double mu, int sign, hipStream_t st);
+// Accumulates scale * link gradient of Re sum_x w(x) chi^dag A psi (lex FP64
+// fields; w = w_even / w_odd by parity); K: workspace of 6*18*V doubles.
+hipError_t run_clover_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *chi,
+                            const double *psi, const double *U, double csw, double w_even, double w_odd,
+                            double scale, int project, hipStream_t st);
 
 }  // namespace qamd

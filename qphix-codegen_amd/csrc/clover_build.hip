@@ -331,6 +331,217 @@ SigmaTable make_sigma()
   return t;
 }
 
+// ---- link derivative of the clover term ------------------------------------
+// For L = Re sum_x w(x) chi(x)^dag A(x) psi(x), w = w_even / w_odd by site
+// parity, dA = (csw/2) sum_{mu<nu} sigma_{mu nu} (x) dF_{mu nu}, and
+//   Re tr[dF L_mn] = Re tr[dQ_mn K_mn],  K_mn = (csw/2)(L' + L'^dag)/(8i),
+// with L_mn[j][i] = w sum_{a,b} sigma_mn[a][b] psi[b][j] conj(chi[a][i]) the
+// spin-traced colour matrix and L' its traceless part (F is the traceless
+// part of (Q - Q^dag)/(8i)).  Each link then collects, from the 24 leaves it
+// lies on (2 squares in each of its 3 planes, every square a leaf of its 4
+// corners), G with dL = Re tr(dU G).
+__global__ __launch_bounds__(128) void clover_weights_k(double *K, const double *chi, const double *psi,
+                                                        Geom g, double csw, double w_even, double w_odd,
+                                                        SigmaTable sig)
+{
+  const long V = (long)g.vcb * 2;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  long r = s;
+  const int x = (int)(r % g.Lx);
+  r /= g.Lx;
+  const int y = (int)(r % g.Ly);
+  r /= g.Ly;
+  const int z = (int)(r % g.Lz);
+  const int t = (int)(r / g.Lz);
+  const double w = ((x + y + z + t) & 1) ? w_odd : w_even;
+  cx<double> a[4][3], b[4][3];  // chi, psi
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      a[sp][c] = {chi[s * 24 + (sp * 3 + c) * 2], chi[s * 24 + (sp * 3 + c) * 2 + 1]};
+      b[sp][c] = {psi[s * 24 + (sp * 3 + c) * 2], psi[s * 24 + (sp * 3 + c) * 2 + 1]};
+    }
+  for (int p = 0; p < 6; p++) {
+    cx<double> L[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) L[j][i] = {0.0, 0.0};
+#pragma unroll
+    for (int sa = 0; sa < 4; sa++) {
+      cx<double> u[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};  // (sigma psi)[sa][j]
+#pragma unroll
+      for (int sb = 0; sb < 4; sb++) {
+        const cx<double> sg = {sig.re[p][sa][sb], sig.im[p][sa][sb]};
+#pragma unroll
+        for (int j = 0; j < 3; j++) u[j] = cmac(u[j], sg, b[sb][j]);
+      }
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) L[j][i] = cmac_conj(L[j][i], a[sa][i], u[j]);
+    }
+    const cx<double> tr = {(L[0][0].re + L[1][1].re + L[2][2].re) / 3.0,
+                           (L[0][0].im + L[1][1].im + L[2][2].im) / 3.0};
+    double *o = K + (s * 6 + p) * 18;
+    const double f = 0.5 * csw * w / 8.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        // H = L' + L'^dag, K = f H / i = -i f H
+        cx<double> h = L[j][i] + conj(L[i][j]);
+        if (i == j) h = {h.re - 2.0 * tr.re, 0.0};
+        o[(j * 3 + i) * 2] = f * h.im;
+        o[(j * 3 + i) * 2 + 1] = -f * h.re;
+      }
+  }
+}
+
+__device__ __forceinline__ M3 ld_k(const double *K, long s, int p)
+{
+  M3 r;
+  const double *q = K + (s * 6 + p) * 18;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) r.m[i][j] = {q[(i * 3 + j) * 2], q[(i * 3 + j) * 2 + 1]};
+  return r;
+}
+
+__device__ __forceinline__ M3 dagm(const M3 &a)
+{
+  M3 r;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) r.m[i][j] = conj(a.m[j][i]);
+  return r;
+}
+
+__device__ __forceinline__ void acc(M3 &G, const M3 &a)
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) G.m[i][j] = G.m[i][j] + a.m[i][j];
+}
+
+__device__ __forceinline__ void acc_dag(M3 &G, const M3 &a)
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) G.m[i][j] = G.m[i][j] + conj(a.m[j][i]);
+}
+
+__device__ __forceinline__ int plane_of(int a, int b)  // a < b
+{
+  return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
+}
+
+__global__ __launch_bounds__(128) void clover_links_k(double *force, const double *K, const double *U, Geom g,
+                                                      double scale, int project)
+{
+  const long V = (long)g.vcb * 2;
+  const long y = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= V) return;
+  int c[4];
+  {
+    long r = y;
+    c[0] = (int)(r % g.Lx);
+    r /= g.Lx;
+    c[1] = (int)(r % g.Ly);
+    r /= g.Ly;
+    c[2] = (int)(r % g.Lz);
+    c[3] = (int)(r / g.Lz);
+  }
+#pragma unroll 1
+  for (int mu = 0; mu < 4; mu++) {
+    M3 G;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) G.m[i][j] = {0.0, 0.0};
+#pragma unroll 1
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == mu) continue;
+      const int a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
+      const int p = plane_of(a, b);
+      // loops run a then b: square(z) = U_a(z) U_b(z+a) U_a(z+b)^dag U_b(z)^dag,
+      // a leaf of each of its corners z, z+a, z+a+b, z+b (cyclic rotations)
+      if (mu == a) {
+        {  // square based at y: the link is its first factor
+          const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
+          const long yb = shifted(g, c, a, 0, b, 1);
+          const M3 U2 = ld_link(U, ya, b), U3 = dagm(ld_link(U, yb, a)), U4 = dagm(ld_link(U, y, b));
+          const M3 U34 = mul(U3, U4), U23 = mul(U2, U3), U234 = mul(U2, U34);
+          acc(G, mul(U234, ld_k(K, y, p)));
+          acc(G, mul(ld_k(K, ya, p), U234));
+          acc(G, mul(mul(U2, ld_k(K, yab, p)), U34));
+          acc(G, mul(mul(U23, ld_k(K, yb, p)), U4));
+        }
+        {  // square based at y - b: the link is its third factor, daggered
+          const long ymb = shifted(g, c, a, 0, b, -1), ymba = shifted(g, c, a, 1, b, -1);
+          const long ya = shifted(g, c, a, 1, b, 0);
+          const M3 V1 = ld_link(U, ymb, a), V2 = ld_link(U, ymba, b), V4 = dagm(ld_link(U, ymb, b));
+          const M3 V41 = mul(V4, V1), V12 = mul(V1, V2), V412 = mul(V41, V2);
+          M3 M = mul(mul(V4, ld_k(K, ymb, p)), V12);
+          acc(M, mul(mul(V41, ld_k(K, ymba, p)), V2));
+          acc(M, mul(V412, ld_k(K, ya, p)));
+          acc(M, mul(ld_k(K, y, p), V412));
+          acc_dag(G, M);
+        }
+      } else {
+        {  // square based at y - a: the link is its second factor
+          const long yma = shifted(g, c, a, -1, b, 0), ymab = shifted(g, c, a, -1, b, 1);
+          const long yb = shifted(g, c, a, 0, b, 1);
+          const M3 W1 = ld_link(U, yma, a), W3 = dagm(ld_link(U, ymab, a)), W4 = dagm(ld_link(U, yma, b));
+          const M3 W34 = mul(W3, W4), W41 = mul(W4, W1), W341 = mul(W34, W1);
+          acc(G, mul(mul(W34, ld_k(K, yma, p)), W1));
+          acc(G, mul(W341, ld_k(K, y, p)));
+          acc(G, mul(ld_k(K, yb, p), W341));
+          acc(G, mul(mul(W3, ld_k(K, ymab, p)), W41));
+        }
+        {  // square based at y: the link is its fourth factor, daggered
+          const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
+          const long yb = shifted(g, c, a, 0, b, 1);
+          const M3 U1 = ld_link(U, y, a), U2 = ld_link(U, ya, b), U3 = dagm(ld_link(U, yb, a));
+          const M3 U12 = mul(U1, U2), U23 = mul(U2, U3), U123 = mul(U12, U3);
+          M3 M = mul(ld_k(K, y, p), U123);
+          acc(M, mul(mul(U1, ld_k(K, ya, p)), U23));
+          acc(M, mul(mul(U12, ld_k(K, yab, p)), U3));
+          acc(M, mul(U123, ld_k(K, yb, p)));
+          acc_dag(G, M);
+        }
+      }
+    }
+    double *out = force + (y * 4 + mu) * 18;
+    if (!project) {  // G^dagger
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          out[(i * 3 + j) * 2] += scale * G.m[j][i].re;
+          out[(i * 3 + j) * 2 + 1] -= scale * G.m[j][i].im;
+        }
+      continue;
+    }
+    // F = TA(U G)
+    const M3 W = mul(ld_link(U, y, mu), G);
+    const double trim = (W.m[0][0].im + W.m[1][1].im + W.m[2][2].im) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        out[(i * 3 + j) * 2] += scale * 0.5 * (W.m[i][j].re - W.m[j][i].re);
+        out[(i * 3 + j) * 2 + 1] += scale * (0.5 * (W.m[i][j].im + W.m[j][i].im) - (i == j ? trim : 0.0));
+      }
+  }
+}
+
 }  // namespace
 
 hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const double *U,
@@ -340,6 +551,21 @@ hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const d
   const int bs = 128;
   hipLaunchKernelGGL(clover_from_links_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, clov, U, ctx->g,
                      alpha, csw, make_sigma());
+  return hipGetLastError();
+}
+
+hipError_t run_clover_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *chi,
+                            const double *psi, const double *U, double csw, double w_even, double w_odd,
+                            double scale, int project, hipStream_t st)
+{
+  const long V = (long)ctx->g.vcb * 2;
+  const int bs = 128;
+  hipLaunchKernelGGL(clover_weights_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, K, chi, psi, ctx->g, csw,
+                     w_even, w_odd, make_sigma());
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(clover_links_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, force, K, U, ctx->g, scale,
+                     project);
   return hipGetLastError();
 }
 

@@ -279,6 +279,59 @@ int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
   return rc;
 }
 
+int qphix_amd_clover_force(qphix_amd_ctx *ctx, double *force_lex, const double *chi_lex,
+                           const double *psi_lex, const double *u_lex, double csw, double scale_even,
+                           double scale_odd, int project, void *stream)
+{
+  if (!ctx || !force_lex || !chi_lex || !psi_lex || !u_lex) return QPHIX_AMD_EINVAL;
+  if (ctx->comm_mask) return QPHIX_AMD_EINVAL;  // needs the whole lattice
+  hipStream_t st = (hipStream_t)stream;
+  const long V = 2L * ctx->g.vcb;
+  DevBuf K;
+  if (K.alloc(V * 108 * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
+  hipError_t e = run_clover_force(ctx, force_lex, (double *)K.p, chi_lex, psi_lex, u_lex, csw, scale_even,
+                                  scale_odd, 1.0, project, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // K is released on return
+  return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP;
+}
+
+int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const void *x,
+                                 const double *u_lex, double csw, double scale, int project,
+                                 void *stream)
+{
+  if (!op || !force_lex || !x || !u_lex) return QPHIX_AMD_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const long n = 24L * op->ctx->g.vcb;
+  DevBuf y, z, w, chi, psi, K;
+  if (y.alloc(n * 8) != hipSuccess || z.alloc(n * 8) != hipSuccess || w.alloc(n * 8) != hipSuccess ||
+      chi.alloc(2 * n * 8) != hipSuccess || psi.alloc(2 * n * 8) != hipSuccess ||
+      K.alloc(2L * op->ctx->g.vcb * 108 * 8) != hipSuccess)
+    return QPHIX_AMD_ENOMEM;
+  // Y = M X, Z = F_ee^-1 D_eo X, W = (F_ee^-1)^dag D_oe^dag Y
+  hipError_t e = eo_apply(op, 0, 0, y.p, x, st);
+  if (e == hipSuccess) e = eo_stage1(op, 0, 0, z.p, x, st);
+  if (e == hipSuccess) e = eo_stage1(op, 0, 1, w.p, y.p, st);
+  if (e != hipSuccess) return QPHIX_AMD_EHIP;
+  // hopping term: dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>]
+  int rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 1, force_lex, y.p, z.p, u_lex, 1.0, 1.0,
+                                  1.0, 0.5 * scale, project, stream);
+  if (rc == QPHIX_AMD_OK)
+    rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 0, force_lex, w.p, x, u_lex, 1.0, 1.0,
+                                1.0, 0.5 * scale, project, stream);
+  if (rc != QPHIX_AMD_OK) return rc;
+  // clover term: dS = -2 Re <Y, dA_oo X> - 1/2 Re <W, dA_ee Z>
+  // (dF = dA: the twisted mass does not depend on the links)
+  e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)psi.p, x, 1, st);
+  if (e == hipSuccess) e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)psi.p, z.p, 0, st);
+  if (e == hipSuccess) e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)chi.p, y.p, 1, st);
+  if (e == hipSuccess) e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)chi.p, w.p, 0, st);
+  if (e == hipSuccess)
+    e = run_clover_force(op->ctx, force_lex, (double *)K.p, (const double *)chi.p, (const double *)psi.p,
+                         u_lex, csw, -0.5 * scale, -2.0 * scale, 1.0, project, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP;
+}
+
 int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_amd_cg_params *prm,
                        qphix_amd_cg_result *res, void *stream)
 {

@@ -45,6 +45,16 @@ def tmclover_blocks(geom, clov_lex, mu, sign, invert):
     return out
 
 
+def clover_force(geom, force, chi_lex, psi_lex, u_lex, csw, scale_even=1.0, scale_odd=1.0, project=True):
+    """Accumulate into force the link gradient of Re sum_x w(x) chi^dag A psi
+    (A = clover_from_links(.., csw); w = scale_even / scale_odd by parity)."""
+    check(lib().qphix_amd_clover_force(geom.ctx, _ptr(force), _ptr(chi_lex.contiguous()),
+                                       _ptr(psi_lex.contiguous()), _ptr(u_lex.contiguous()), float(csw),
+                                       float(scale_even), float(scale_odd), int(bool(project)), _stream()),
+          "qphix_amd_clover_force")
+    return force
+
+
 class EvenOddCloverOperator:
     """M = F_oo - 1/4 D_oe F_ee^-1 D_eo on cb 1 with an FP64 copy (12-real
     links) and a sloppy copy (prec_sloppy storage, recon_sloppy links)."""
@@ -80,12 +90,19 @@ class EvenOddCloverOperator:
         out.cb = 1
 
 
-    def force(self, force, x, u_lex, scale=1.0, project=True):
-        """Accumulate the hopping-term gradient of phi^dag (M^dag M)^-1 phi at
-        x = (M^dag M)^-1 phi (FP64, cb 1) into force (lex FP64 (V,4,3,3,2))."""
-        check(lib().qphix_amd_eo_op_force(self._op, _ptr(force), x.ptr(), _ptr(u_lex.contiguous()),
-                                          float(scale), int(bool(project)), _stream()),
-              "qphix_amd_eo_op_force")
+    def force(self, force, x, u_lex, scale=1.0, project=True, csw=None):
+        """Accumulate the gradient of phi^dag (M^dag M)^-1 phi at
+        x = (M^dag M)^-1 phi (FP64, cb 1) into force (lex FP64 (V,4,3,3,2)):
+        the hopping term, plus the clover term when csw is given (the
+        operator's clover field then being clover_from_links(.., csw))."""
+        u = _ptr(u_lex.contiguous())
+        if csw is None:
+            st = lib().qphix_amd_eo_op_force(self._op, _ptr(force), x.ptr(), u, float(scale),
+                                             int(bool(project)), _stream())
+        else:
+            st = lib().qphix_amd_eo_op_force_clover(self._op, _ptr(force), x.ptr(), u, float(csw),
+                                                    float(scale), int(bool(project)), _stream())
+        check(st, "qphix_amd_eo_op_force")
         return force
 
 

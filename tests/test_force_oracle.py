@@ -74,3 +74,19 @@ def test_oracle_force_euclidean_gradient(qamd, orc):
               - loss(orc, U - 1e-3 * E, chi, psi, isign, cb, kw)) / 2e-3
         an = float(np.sum(G * E))
         assert abs(fd - an) <= 1e-10 * max(abs(an), 1.0)
+
+
+def test_torch_clover_restatement_equals_oracle_builder(qamd, orc):
+    """tests/clover_torch.py (whose autograd is the reference gradient of the
+    clover-term force) builds the same A(U) as the oracle, and A is
+    Hermitian and block diagonal in spin (blocks {0,1} and {2,3})."""
+    import clover_torch as ct
+    dims = (4, 4, 6, 4)
+    L = qamd.lattice
+    U = L.random_gauge(dims, 0.5, 61)
+    Uc = torch.complex(U[..., 0], U[..., 1])
+    A = ct.clover_matrix(Uc, dims, 3.7, 1.3)
+    assert (A - A.conj().transpose(-1, -2)).abs().max().item() < 1e-14
+    assert A[:, :6, 6:].abs().max().item() < 1e-15
+    ref = orc.clover_from_links(dims, U.numpy(), 3.7, 1.3)
+    assert np.abs(ct.pack_blocks(A).numpy() - ref).max() < 1e-14

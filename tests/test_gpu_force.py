@@ -175,3 +175,88 @@ def test_force_finite_differences_pseudofermion_16x32(qamd):
         worst = max(worst, abs(fd - an) / abs(an))
     print("pseudofermion force FD worst rel err", worst)
     assert worst < 1e-6
+
+
+def test_clover_force_vs_autograd(qamd):
+    """qphix_amd_clover_force against torch autograd of the torch restatement
+    of A(U) (tests/clover_torch.py, itself equal to the oracle's builder):
+    L = Re sum_x w(x) chi^dag A psi with parity weights, Euclidean gradient
+    and su(3) projection, 1e-12 relative."""
+    import clover_torch as ct
+    dims = (4, 4, 4, 8)
+    L = qamd.lattice
+    V = 512
+    U = L.random_gauge(dims, 0.5, 91)
+    chi, psi = L.random_spinor(dims, 92), L.random_spinor(dims, 93)
+    csw, we, wo = 1.3, 0.7, -1.9
+    Ur = U.clone().requires_grad_(True)
+    A = ct.clover_matrix(torch.complex(Ur[..., 0], Ur[..., 1]), dims, 3.7, csw)
+    c = torch.complex(chi[..., 0], chi[..., 1]).reshape(V, 12)
+    p = torch.complex(psi[..., 0], psi[..., 1]).reshape(V, 12)
+    w = torch.where(L.parity(dims) == 1, torch.tensor(wo, dtype=torch.float64),
+                    torch.tensor(we, dtype=torch.float64))
+    loss = (w * torch.einsum("vi,vij,vj->v", c.conj(), A, p).real).sum()
+    loss.backward()
+    ref = Ur.grad  # real-form gradient = G^dagger
+    geom = qamd.Geometry(dims)
+    got = qamd.clover_force(geom, torch.zeros_like(U).to(DEV), chi.to(DEV), psi.to(DEV), U.to(DEV), csw, we, wo,
+                            project=False).cpu()
+    assert ((got - ref).abs().max() / ref.abs().max()).item() < 1e-12
+    # projected: F = TA(U G), G = (G^dagger)^dagger
+    Gd = torch.complex(ref[..., 0], ref[..., 1])
+    W = torch.complex(U[..., 0], U[..., 1]) @ Gd.conj().transpose(-1, -2)
+    TA = 0.5 * (W - W.conj().transpose(-1, -2))
+    TA = TA - (torch.diagonal(TA, dim1=-2, dim2=-1).sum(-1) / 3)[..., None, None] * torch.eye(3, dtype=TA.dtype)
+    gotP = qamd.clover_force(geom, torch.zeros_like(U).to(DEV), chi.to(DEV), psi.to(DEV), U.to(DEV), csw, we, wo,
+                             project=True).cpu()
+    gp = torch.complex(gotP[..., 0], gotP[..., 1])
+    assert ((gp - TA).abs().max() / TA.abs().max()).item() < 1e-12
+
+
+@pytest.mark.parametrize("csw,mu", [(1.0, 0.0), (1.5, 0.1)], ids=["clover", "tmclover"])
+def test_force_finite_differences_pseudofermion_clover(qamd, csw, mu):
+    """The whole pseudofermion force (hopping + clover term) of the even-odd
+    clover / twisted-mass clover operator with A = clover_from_links(U, ..,
+    csw) against central differences of S along 16 su(3) directions
+    (eps = 1e-5, FP64): |fd - an| <= 1e-6 of the largest directional
+    derivative (a per-direction ratio is meaningless where a direction's
+    derivative is itself near zero); the hopping term alone misses by far
+    more, so the clover term is what is being checked; and the Euclidean
+    autograd gradient is consistent with the projected force."""
+    dims = (8, 8, 8, 8)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    U = L.random_gauge(dims, 0.3, 101, DEV)
+    phi = L.random_spinor(dims, 102, DEV)
+    kappa, rsd = 0.12, 1e-14
+
+    def action(Uv):
+        return qamd.PseudofermionAction.apply(Uv, phi, geom, kappa, rsd, csw, mu).item()
+
+    F = qamd.su3_force_pseudofermion(geom, U, phi, kappa, rsd, csw, mu)
+    Ur = U.clone().requires_grad_(True)
+    qamd.PseudofermionAction.apply(Ur, phi, geom, kappa, rsd, csw, mu).backward()
+    Gd = torch.view_as_complex(Ur.grad.contiguous())
+    W = torch.view_as_complex(U) @ Gd.conj().transpose(-1, -2)
+    TA = 0.5 * (W - W.conj().transpose(-1, -2))
+    tr = torch.diagonal(TA, dim1=-2, dim2=-1).sum(-1) / 3
+    TA = TA - tr[..., None, None] * torch.eye(3, dtype=TA.dtype, device=DEV)
+    Fc = torch.view_as_complex(F.contiguous())
+    assert ((TA - Fc).abs().max() / Fc.abs().max()).item() < 1e-12
+    # the hopping term alone (the operator's force without the clover term)
+    op, _, _, X = qamd.autograd._pf_solution(geom, U, phi, kappa, rsd, csw, mu)
+    Fh = op.force(torch.zeros_like(U), X, U, project=True)
+    op.close()
+    eps = 1e-5
+    errs, errs_hop, scale = [], [], 0.0
+    for T in _su3_dirs(qamd, U.shape[0], 16, 103):
+        fd = (action(_rotate(U, T, eps)) - action(_rotate(U, T, -eps))) / (2 * eps)
+        an = _directional(F, T)
+        scale = max(scale, abs(an))
+        errs.append(abs(fd - an))
+        errs_hop.append(abs(fd - _directional(Fh, T)))
+    worst, worst_hop = max(errs) / scale, max(errs_hop) / scale
+    print(f"clover pseudofermion force csw={csw} mu={mu}: FD worst err {worst:.3g} of the largest "
+          f"derivative (hopping term alone: {worst_hop:.3g})", flush=True)
+    assert worst < 1e-6
+    assert worst_hop > 1e-3
