@@ -22,6 +22,7 @@
 // All reductions are FP64 with a fixed summation order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <new>
@@ -332,6 +333,16 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
   return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP;
 }
 
+// CG iterations enqueued between two host reads of the scalars: BF16
+// storage (7-bit mantissa) wants its reliable updates on time, the others
+// tolerate a few iterations of delay
+#ifndef QAMD_CG_CHECK
+#define QAMD_CG_CHECK 4
+#endif
+#ifndef QAMD_CG_CHECK_BF16
+#define QAMD_CG_CHECK_BF16 1
+#endif
+
 int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_amd_cg_params *prm,
                        qphix_amd_cg_result *res, void *stream)
 {
@@ -416,12 +427,23 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   bool converged = b2 == 0.0;
   int rc = QPHIX_AMD_OK;
   while (e == hipSuccess && !converged && iters < prm->max_iters) {
-    e = eo_normal(op, lvl, ap.p, mid.p, p.p, st);
-    mv += 2;
+    // a batch of iterations is enqueued back to back (alpha, beta and the
+    // norms never leave the device); the host reads the scalars once per
+    // batch and only then decides between the plain p update and a reliable
+    // update, which therefore comes at most batch - 1 iterations later than a
+    // per-iteration check would place it.
+    const int batch = std::min(ps == QPHIX_AMD_BF16 ? QAMD_CG_CHECK_BF16 : QAMD_CG_CHECK,
+                               prm->max_iters - iters);
+    for (int k = 0; k < batch && e == hipSuccess; k++) {
+      if (k > 0) blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+      e = eo_normal(op, lvl, ap.p, mid.p, p.p, st);
+      mv += 2;
+      if (e != hipSuccess) break;
+      blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
+      blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
+      iters++;
+    }
     if (e != hipSuccess) break;
-    blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
-    blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
-    iters++;
     e = fetch();
     if (e != hipSuccess) break;
     const double rr = hsc->rr;              // |r_s|^2
