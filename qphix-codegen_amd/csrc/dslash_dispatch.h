@@ -31,6 +31,10 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
     } else if (QAMD_BLOCK_ORDER == 2 && ctx->g.Lt % kNumXcd == 0) {
       p.order = 2;
       p.ts = ctx->g.Lt / kNumXcd;
+    } else if (QAMD_BLOCK_ORDER == 3 && ctx->g.Lt % kNumXcd == 0 && ctx->g.Lz % QAMD_ZPARTS == 0) {
+      p.order = 3;
+      p.ts = ctx->g.Lt / kNumXcd;
+      p.zs = ctx->g.Lz / QAMD_ZPARTS;
     }
   }
   p.coeff_s = (R)c.coeff_s;

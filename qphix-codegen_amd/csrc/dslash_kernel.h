@@ -29,7 +29,8 @@ template <typename R> struct DslashParams {
   int cb;
   int nblocks;  // real (unpadded) work-groups
   int order;    // work-group traversal: 0 t-slabs per XCD (natural order),
-                // 1 z-slabs per XCD visited t-major, 2 t-slabs visited z-major
+                // 1 z-slabs per XCD visited t-major, 2 t-slabs visited z-major,
+                // 3 t-slabs visited one z-part at a time
   int zs, ts, bpp;  // z-planes / t-slices per XCD slab, work-groups per (z,t) plane
   R coeff_s, coeff_tf, coeff_tb, tbound;
   int comm;  // bit mu: dimension mu is split; hops across its wrap-around are dropped
@@ -44,6 +45,10 @@ template <typename R> struct DslashParams {
 // default work-group traversal (see DslashParams::order)
 #ifndef QAMD_BLOCK_ORDER
 #define QAMD_BLOCK_ORDER 0
+#endif
+// z-parts per t-slab for order 3
+#ifndef QAMD_ZPARTS
+#define QAMD_ZPARTS 2
 #endif
 #ifndef QAMD_DSLASH_BLOCK
 #define QAMD_DSLASH_BLOCK 128
@@ -296,6 +301,17 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     const int k = blk / per, r = blk - k * per;
     const int z = r / (p.ts * p.bpp), r2 = r - z * (p.ts * p.bpp);
     const int t = k * p.ts + r2 / p.bpp, h = r2 % p.bpp;
+    idx = (t * g.Lz + z) * g.sz + h * kDslashBlock + (int)threadIdx.x;
+  } else if (p.order == 3) {
+    // XCD k owns t-slices [k ts, (k+1) ts) and sweeps them one z-part (zs
+    // planes) at a time: the spinor rows reused across three t-slices then
+    // span a part of the slice, not the whole slice
+    const int per = p.ts * g.Lz * p.bpp;
+    const int k = blk / per, r = blk - k * per;
+    const int part = p.ts * p.zs * p.bpp;
+    const int zp = r / part, r2 = r - zp * part;
+    const int t = k * p.ts + r2 / (p.zs * p.bpp), r3 = r2 % (p.zs * p.bpp);
+    const int z = zp * p.zs + r3 / p.bpp, h = r3 % p.bpp;
     idx = (t * g.Lz + z) * g.sz + h * kDslashBlock + (int)threadIdx.x;
   } else {
     idx = blk * kDslashBlock + (int)threadIdx.x;
