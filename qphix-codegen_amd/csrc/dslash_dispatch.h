@@ -23,6 +23,10 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   // z-slab traversal needs whole work-groups per (z,t) plane and Lz % 8 == 0
   p.order = 0;
   p.zs = p.ts = p.bpp = 1;
+  // LDS rows: whole rows per work-group, whole work-groups per (z,t) plane
+  const int rows = ctx->g.Lxh > 0 ? kDslashBlock / ctx->g.Lxh : 0;
+  p.lds = QAMD_LDS_XY && ctx->g.Lxh <= kLdsRowsMax && kDslashBlock % ctx->g.Lxh == 0 && rows > 0 &&
+          ctx->g.Ly % rows == 0 && QAMD_BLOCK_ORDER == 0;
   if (ctx->g.sz % kDslashBlock == 0) {
     p.bpp = ctx->g.sz / kDslashBlock;
     if (QAMD_BLOCK_ORDER == 1 && ctx->g.Lz % kNumXcd == 0) {

@@ -33,6 +33,10 @@ SPLITS = {  # name: (global dims, parts per dimension)
     "x2": ((16, 4, 4, 4), (2, 1, 1, 1)),
     "yz2x2": ((8, 8, 8, 4), (1, 2, 2, 1)),
     "xyzt16": ((16, 8, 8, 8), (2, 2, 2, 2)),
+    # parts whose rows fill whole work-groups: the x and y hops come from
+    # the rows staged in LDS
+    "x2_rows": ((64, 8, 4, 4), (2, 1, 1, 1)),
+    "y2_rows": ((32, 16, 4, 4), (1, 2, 1, 1)),
 }
 
 
