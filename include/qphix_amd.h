@@ -73,6 +73,21 @@ int qphix_amd_ctx_destroy(qphix_amd_ctx *ctx);
    reference's template flags tbc_x..tbc_t and tbc_phases[4][2]).         */
 int qphix_amd_set_tbc(qphix_amd_ctx *ctx, const int tbc[4],
                       const double tbc_phases[8]);
+/* Work-group traversal of the operator kernels.  No counterpart in the
+   reference (its SIMD loop has no work-group schedule); this is an MI355X
+   tuning knob for callers that know their workload better than the default.
+     -1  automatic (the default): 1 when one t-slice holds more than 32768
+         sites per checkerboard, else 0;
+      0  each of the 8 XCDs sweeps its own t-slab;
+      1  the XCDs sweep t together, each owning a z-range (Lz % 8 == 0);
+      2  t-slabs visited z-major (Lt % 8 == 0);
+      3  t-slabs swept two z-halves at a time (Lt % 8 == 0).
+   Orders 1-3 need whole 128-site work-groups per (z,t) plane
+   ((Lx/2)*Ly % 128 == 0); an order the geometry does not admit runs as 0.
+   Results are bit-identical for every order.  qphix_amd_traversal returns
+   the order the calls on this context use (-1 for a null context).       */
+int qphix_amd_set_traversal(qphix_amd_ctx *ctx, int order);
+int qphix_amd_traversal(const qphix_amd_ctx *ctx);
 /* Bytes of one checkerboard of a field in native layout.                 */
 size_t qphix_amd_field_bytes(const qphix_amd_ctx *ctx, int field, int prec,
                              int recon);

@@ -81,6 +81,8 @@ SIGNATURES = {
                                                        _vp, _vp, _c_double, _c_double, _c_double, _c_double,
                                                        _vp]),
     "qphix_amd_set_comm": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "qphix_amd_set_traversal": (_c_int, [_vp, _c_int]),
+    "qphix_amd_traversal": (_c_int, [_vp]),
     "qphix_amd_face_sites": (ctypes.c_long, [_vp, _c_int]),
     "qphix_amd_face_pack": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "qphix_amd_face_finish": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,

@@ -2,6 +2,7 @@
 // Every entry validates its arguments, never exits, and returns a status.
 #include <new>
 
+#include "dslash_kernel.h"
 #include "internal.h"
 
 using qamd::DslashCall;
@@ -66,6 +67,10 @@ hipError_t run_dslash_any(const qphix_amd_ctx *ctx, const DslashCall &c, hipStre
     default: return run_dslash_bf16(ctx, c, st);
   }
 }
+int dslash_traversal(const qphix_amd_ctx *ctx)
+{
+  return dslash_traversal_for(ctx->g, ctx->traversal >= 0 ? ctx->traversal : QAMD_BLOCK_ORDER);
+}
 }  // namespace qamd
 
 extern "C" {
@@ -92,6 +97,7 @@ int qphix_amd_ctx_create(qphix_amd_ctx **ctx, int Lx, int Ly, int Lz, int Lt, do
   c->g.st = Lx / 2 * Ly * Lz;
   c->tbound = t_boundary;
   c->comm_mask = 0;
+  c->traversal = -1;
   c->tbc_mask = 0;
   for (int m = 0; m < 4; m++) {
     c->tbc_re[m] = 1.0;
@@ -118,6 +124,15 @@ int qphix_amd_set_tbc(qphix_amd_ctx *ctx, const int tbc[4], const double tbc_pha
   }
   return QPHIX_AMD_OK;
 }
+
+int qphix_amd_set_traversal(qphix_amd_ctx *ctx, int order)
+{
+  if (!ctx || order < -1 || order > 3) return QPHIX_AMD_EINVAL;
+  ctx->traversal = order;
+  return QPHIX_AMD_OK;
+}
+
+int qphix_amd_traversal(const qphix_amd_ctx *ctx) { return ctx ? qamd::dslash_traversal(ctx) : -1; }
 
 long qphix_amd_sites_cb(const qphix_amd_ctx *ctx) { return ctx ? (long)ctx->g.vcb : 0; }
 

@@ -20,26 +20,29 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.g = ctx->g;
   p.cb = c.cb;
   p.nblocks = blocks_for(ctx->g.vcb, kDslashBlock);
-  // z-slab traversal needs whole work-groups per (z,t) plane and Lz % 8 == 0
-  p.order = 0;
   p.zs = p.ts = p.bpp = 1;
-  // LDS rows: whole rows per work-group, whole work-groups per (z,t) plane
-  const int rows = ctx->g.Lxh > 0 ? kDslashBlock / ctx->g.Lxh : 0;
-  p.lds = QAMD_LDS_XY && ctx->g.Lxh <= kLdsRowsMax && kDslashBlock % ctx->g.Lxh == 0 && rows > 0 &&
-          ctx->g.Ly % rows == 0 && QAMD_BLOCK_ORDER == 0;
-  if (ctx->g.sz % kDslashBlock == 0) {
-    p.bpp = ctx->g.sz / kDslashBlock;
-    if (QAMD_BLOCK_ORDER == 1 && ctx->g.Lz % kNumXcd == 0) {
-      p.order = 1;
-      p.zs = ctx->g.Lz / kNumXcd;
-    } else if (QAMD_BLOCK_ORDER == 2 && ctx->g.Lt % kNumXcd == 0) {
-      p.order = 2;
-      p.ts = ctx->g.Lt / kNumXcd;
-    } else if (QAMD_BLOCK_ORDER == 3 && ctx->g.Lt % kNumXcd == 0 && ctx->g.Lz % QAMD_ZPARTS == 0) {
-      p.order = 3;
-      p.ts = ctx->g.Lt / kNumXcd;
-      p.zs = ctx->g.Lz / QAMD_ZPARTS;
+  // LDS rows: whole work-groups per (z,t) plane, and the rows a work-group
+  // touches plus one on either side fit the staging array
+  p.lds = 0;
+  // (every traversal order maps a work-group to 128 consecutive sites of one
+  // plane when the plane holds whole work-groups)
+  if (QAMD_LDS_XY && ctx->g.Lxh > 0 && ctx->g.sz % kDslashBlock == 0) {
+    int worst = 0;
+    for (int pb = 0; pb < ctx->g.sz; pb += kDslashBlock) {
+      const int rows = (pb + kDslashBlock - 1) / ctx->g.Lxh - pb / ctx->g.Lxh + 3;
+      worst = rows * ctx->g.Lxh > worst ? rows * ctx->g.Lxh : worst;
     }
+    p.lds = worst <= kLdsSites;
+  }
+  p.order = dslash_traversal(ctx);
+  if (p.order != 0) p.bpp = ctx->g.sz / kDslashBlock;
+  if (p.order == 1) {
+    p.zs = ctx->g.Lz / kNumXcd;
+  } else if (p.order == 2) {
+    p.ts = ctx->g.Lt / kNumXcd;
+  } else if (p.order == 3) {
+    p.ts = ctx->g.Lt / kNumXcd;
+    p.zs = ctx->g.Lz / QAMD_ZPARTS;
   }
   p.coeff_s = (R)c.coeff_s;
   p.coeff_tf = (R)c.coeff_tf;

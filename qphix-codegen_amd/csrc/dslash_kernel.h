@@ -43,9 +43,10 @@ template <typename R> struct DslashParams {
 #ifndef QAMD_XCD_REMAP
 #define QAMD_XCD_REMAP 1
 #endif
-// default work-group traversal (see DslashParams::order)
+// default work-group traversal (see DslashParams::order); -1 picks it from
+// the geometry
 #ifndef QAMD_BLOCK_ORDER
-#define QAMD_BLOCK_ORDER 0
+#define QAMD_BLOCK_ORDER -1
 #endif
 // z-parts per t-slab for order 3
 #ifndef QAMD_ZPARTS
@@ -55,6 +56,29 @@ template <typename R> struct DslashParams {
 #define QAMD_DSLASH_BLOCK 128
 #endif
 constexpr int kDslashBlock = QAMD_DSLASH_BLOCK;
+
+// Work-group traversal order (DslashParams::order) for this geometry: the
+// requested one (qphix_amd_set_traversal, else QAMD_BLOCK_ORDER) when the
+// geometry admits it, else 0.  Automatic choice: 1 when one t-slice holds
+// more than 32768 sites per checkerboard.  With order 0 every XCD sweeps its
+// own t-slab, so between the first and the last read of an input t-slice
+// the eight XCDs stream two slices each of every field; at 48^3 slices that
+// is ~760 MB (FP32 clover), beyond the 256 MB Infinity Cache, and the input
+// re-reads go to HBM.  Order 1 has the XCDs sweep t together, each owning a
+// z-range.  Measured (FP32/FP16 clov_dslash): 48^3x96 908 -> 855 us and
+// 428 -> 410 us; 48^3x48 444 -> 425 us; 32x32x64 and 64x32x32 slices equal;
+// 32^3 slices 152 -> 166 us (order 0 kept).
+inline int dslash_traversal_for(const Geom &g, int requested)
+{
+  if (g.sz % kDslashBlock != 0) return 0;
+  int ord = requested;
+  if (ord < 0) ord = (long)g.sz * g.Lz > 32768 ? 1 : 0;
+  if (ord == 1 && g.Lz % kNumXcd == 0) return 1;
+  if (ord == 2 && g.Lt % kNumXcd == 0) return 2;
+  if (ord == 3 && g.Lt % kNumXcd == 0 && g.Lz % QAMD_ZPARTS == 0) return 3;
+  return 0;
+}
+
 // Minimum waves per SIMD requested from the register allocator (0 = none).
 #ifndef QAMD_DSLASH_MIN_WAVES
 #define QAMD_DSLASH_MIN_WAVES 0
@@ -154,17 +178,18 @@ __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site
 }
 
 // Input spinors staged in LDS (QAMD_LDS_XY): [16-byte chunk][staged site].
-// A work-group of whole rows stages its rows and the two neighbouring rows
-// of the input once; the x and y hops (4 of the 8) then read LDS instead of
-// issuing global loads that mostly hit L2.  The barrier also splits the
+// A work-group (128 consecutive sites of one (z, t) plane) stages the x rows
+// it touches and the row on either side of them once; the x and y hops (4 of
+// the 8) then read LDS instead of issuing global loads that mostly hit L2.  The barrier also splits the
 // kernel into phases, which lowers register pressure (FP32 166 -> 128 VGPRs,
 // occupancy 3 -> 4 waves/SIMD): clov_dslash FP32 155 -> 152 us, FP16
 // 81 -> 77 us at 32^3x64.
 #ifndef QAMD_LDS_XY
 #define QAMD_LDS_XY 1
 #endif
-constexpr int kLdsRowsMax = 32;  // Lx/2 up to this many sites per row
-constexpr int kLdsSites = kDslashBlock + 2 * kLdsRowsMax;
+// staged sites per work-group: 8 rows of 16 plus 2, 4 of 32 plus 2, or up to
+// 6 partial rows of 24 plus 2 (the host checks the geometry's worst case)
+constexpr int kLdsSites = kDslashBlock + 64;
 using Vec16 = typename RawVec<16>::T;
 
 template <typename S, typename R>
@@ -408,14 +433,15 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
 #if QAMD_LDS_XY
   if (p.lds) {
-    // the work-group's rows y0 .. y0+R-1 of one (z, t) plane and the rows
-    // above and below (y periodic), staged once for the x and y hops
+    // the rows y0 .. y1 of one (z, t) plane that the work-group's sites lie
+    // in and the rows above and below (y periodic), staged once for the x
+    // and y hops; the work-group never straddles two planes (the host checks
+    // that a plane holds whole work-groups)
     __shared__ Vec16 lds[(24 * sizeof(S) / 16) * kLdsSites];
-    const int R_rows = kDslashBlock / g.Lxh;
-    const int base = idx - (int)threadIdx.x;
-    const int y0 = c.y - (int)threadIdx.x / g.Lxh;
-    const int plane = base - y0 * g.Lxh;
-    const int nst = (R_rows + 2) * g.Lxh;
+    const int plane = row - c.y * g.Lxh;
+    const int pb = idx - (int)threadIdx.x - plane;  // first site's offset in the plane
+    const int y0 = pb / g.Lxh;
+    const int nst = ((pb + kDslashBlock - 1) / g.Lxh - y0 + 3) * g.Lxh;
     const Vec16 *vin = reinterpret_cast<const Vec16 *>(in);
     constexpr int NC = 24 * (int)sizeof(S) / 16;
     for (int j = (int)threadIdx.x; j < nst; j += kDslashBlock) {
@@ -425,7 +451,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       for (int ch = 0; ch < NC; ch++) lds[ch * kLdsSites + j] = vin[(unsigned)(ch * vcb + src)];
     }
     __syncthreads();
-    const int rl = (int)threadIdx.x / g.Lxh + 1;  // staged row of this site
+    const int rl = c.y - y0 + 1;  // staged row of this site
     hop_mu_lds<S, RECON, 0, ISIGN>(acc, lds, gauge, vcb, idx, rl * g.Lxh + (xb >> 1),
                                    rl * g.Lxh + (xf >> 1), bxb, bxf, tm & 1, ph0);
     hop_mu_lds<S, RECON, 1, ISIGN>(acc, lds, gauge, vcb, idx, (rl - 1) * g.Lxh + c.xh,

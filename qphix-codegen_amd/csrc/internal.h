@@ -9,6 +9,7 @@ struct qphix_amd_ctx {
   int comm_mask;  // split dimensions (qphix_amd_set_comm)
   int tbc_mask;
   double tbc_re[4], tbc_im[4];
+  int traversal;  // work-group order requested (qphix_amd_set_traversal), -1 auto
 };
 
 namespace qamd {
@@ -22,6 +23,9 @@ struct DslashCall {
   const void *in, *chi, *gauge, *clov;
   double coeff_s, coeff_tf, coeff_tb, alpha, mu, muinv, rho;
 };
+
+// The work-group traversal order the operator calls use on this geometry.
+int dslash_traversal(const qphix_amd_ctx *ctx);
 
 hipError_t run_dslash_f64(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);
 hipError_t run_dslash_f32(const qphix_amd_ctx *ctx, const DslashCall &c, hipStream_t st);

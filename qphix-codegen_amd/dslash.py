@@ -97,6 +97,17 @@ class Geometry:
     def face_sites(self, dim):
         return int(lib().qphix_amd_face_sites(self._ctx, dim))
 
+    def set_traversal(self, order):
+        """Work-group traversal of the operator kernels: -1 automatic, 0 a
+        t-slab per XCD, 1 XCDs sweep t together owning z-ranges, 2 t-slabs
+        z-major, 3 t-slabs in two z-halves (bit-identical results)."""
+        check(lib().qphix_amd_set_traversal(self._ctx, int(order)), "qphix_amd_set_traversal")
+
+    @property
+    def traversal(self):
+        """The traversal order the operator calls on this geometry use."""
+        return int(lib().qphix_amd_traversal(self._ctx))
+
     def new_face(self, prec, dim, device="cuda"):
         return torch.empty(self.face_sites(dim) * 12, dtype=TORCH_DTYPE[prec], device=device)
 

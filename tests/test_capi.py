@@ -95,3 +95,34 @@ def test_face_entry_points_validate(qamd):
         assert lib.qphix_amd_dslash_force(ctx, 0, 1, 0, p, q, q, p, 1.0, 1.0, 1.0, 1.0, 1, None) == -1
     finally:
         lib.qphix_amd_ctx_destroy(ctx)
+
+
+def test_traversal_choice(qamd):
+    """Automatic work-group traversal: order 1 (XCDs sweep t together) when a
+    t-slice holds more than 32768 sites per checkerboard, else 0; requested
+    orders the geometry does not admit run as 0; bad arguments rejected."""
+    lib = qamd.lib()
+    cases = {  # dims: (automatic, effective order for requests 0..3)
+        (32, 32, 32, 64): (0, [0, 1, 2, 3]),
+        (48, 48, 48, 96): (1, [0, 1, 2, 3]),
+        (64, 32, 32, 64): (0, [0, 1, 2, 3]),
+        (48, 48, 44, 96): (0, [0, 0, 2, 3]),   # Lz % 8 != 0
+        (40, 40, 40, 80): (0, [0, 0, 0, 0]),   # (Lx/2) Ly % 128 != 0
+        (8, 8, 8, 8): (0, [0, 0, 0, 0]),
+    }
+    for dims, (auto, eff) in cases.items():
+        ctx = ctypes.c_void_p()
+        assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), *dims, -1.0) == 0
+        try:
+            assert lib.qphix_amd_traversal(ctx) == auto, dims
+            for req, want in enumerate(eff):
+                assert lib.qphix_amd_set_traversal(ctx, req) == 0
+                assert lib.qphix_amd_traversal(ctx) == want, (dims, req)
+            assert lib.qphix_amd_set_traversal(ctx, -1) == 0
+            assert lib.qphix_amd_traversal(ctx) == auto
+            assert lib.qphix_amd_set_traversal(ctx, 4) == -1
+            assert lib.qphix_amd_set_traversal(ctx, -2) == -1
+        finally:
+            lib.qphix_amd_ctx_destroy(ctx)
+    assert lib.qphix_amd_set_traversal(None, 0) == -1
+    assert lib.qphix_amd_traversal(None) == -1

@@ -160,6 +160,58 @@ def test_tbc_anisotropy_periodic(qamd, orc, lat8):
         assert np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max() < 1e-13
 
 
+@pytest.mark.parametrize("prec", [0, 1, 2, 3], ids=["f64", "f32", "f16", "bf16"])
+@pytest.mark.parametrize("dims", [(32, 16, 4, 4), (48, 16, 4, 4), (24, 32, 4, 4)],
+                         ids=["rows16", "rows24", "rows12"])
+def test_lds_rows_with_phases(qamd, orc, dims, prec):
+    """Lattices whose (z,t) planes hold whole 128-site work-groups, so the x
+    and y hops read the rows staged in LDS.  32x16x4x4: 8 whole x rows per
+    work-group, two work-groups per plane, the y wrap crossing between them;
+    48x16x4x4 and 24x32x4x4: work-groups that start and end inside a row
+    (5.3 and 10.7 rows).  Twisted phases on x, y and t, antiperiodic time,
+    every family, both isigns and checkerboards."""
+    lat = make_lattice(qamd, dims, 1300)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    tbc = [1, 1, 0, 1]
+    phases = [np.exp(1j * t) for t in (0.4, -0.9, 0.6, 1.7)]
+    geom.set_tbc(tbc, phases)
+    for kind in range(4):
+        for achi in (False, True):
+            for isign, cb in ((1, 0), (-1, 1)):
+                for recon in ((12, 18) if prec < 2 else (12,)):
+                    out = run_gpu(qamd, geom, kind, achi, prec, recon, isign, cb, lat)
+                    kw = oracle_args(kind, achi, isign, lat, dims)
+                    kw.update(tbc=tbc, tbc_phases=phases)
+                    ref = orc.dslash(dims, lat["psi"].numpy(), lat["U"].numpy(), isign, cb, **kw)
+                    m = cb_mask(dims, cb)
+                    err = np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max()
+                    assert err < TOL[prec], (kind, achi, isign, cb, recon, err)
+
+
+@pytest.mark.parametrize("dims", [(32, 16, 16, 16), (48, 16, 8, 8)], ids=["rows16", "rows24"])
+def test_traversal_orders_bit_identical(qamd, dims):
+    """Every work-group traversal (a t-slab per XCD; XCDs sweeping t
+    together over z-ranges; z-major t-slabs; t-slabs in z-halves) gives the
+    same bits: each site's arithmetic does not depend on the schedule.  Both
+    lattices take the LDS path; 48x16x8x8 has work-groups that start inside
+    an x row."""
+    lat = make_lattice(qamd, dims, 1400)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    geom.set_tbc([1, 1, 1, 1], [np.exp(1j * t) for t in (0.2, -0.5, 0.9, 1.3)])
+    for prec in (0, 1, 2):
+        for kind in range(4):
+            for achi in (False, True):
+                isign, cb = (1, 0) if achi else (-1, 1)
+                outs = []
+                for order in range(4):
+                    geom.set_traversal(order)
+                    assert geom.traversal == order
+                    outs.append(run_gpu(qamd, geom, kind, achi, prec, 12, isign, cb, lat))
+                for order in range(1, 4):
+                    assert np.array_equal(outs[order], outs[0]), (prec, kind, achi, order)
+    geom.set_traversal(-1)
+
+
 def test_small_lattice_4x4(qamd, orc):
     """Smallest extents (2 sites in x per checkerboard row), FP64 clover."""
     dims = (4, 2, 4, 2)

@@ -171,8 +171,9 @@ def test_split_lattice_equals_whole_lattice(qamd, orc, split):
                 assert err < tol, (split, kind, achi, isign, cb, prec, err)
 
 
-def test_split_with_twisted_boundary_phases(qamd, orc):
-    G, grid = SPLITS["xyzt16"]
+@pytest.mark.parametrize("split", ["xyzt16", "x2_rows", "y2_rows"])
+def test_split_with_twisted_boundary_phases(qamd, orc, split):
+    G, grid = SPLITS[split]
     lat = _lattice(qamd, G, 710)
     tbc = ([1, 0, 1, 1], [np.exp(0.3j), 1.0, np.exp(-0.7j), np.exp(1.1j)])
     for kind, achi in ((0, 0), (1, 0), (2, 0), (3, 1)):
