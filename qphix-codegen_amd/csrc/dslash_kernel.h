@@ -60,19 +60,23 @@ constexpr int kDslashBlock = QAMD_DSLASH_BLOCK;
 // Work-group traversal order (DslashParams::order) for this geometry: the
 // requested one (qphix_amd_set_traversal, else QAMD_BLOCK_ORDER) when the
 // geometry admits it, else 0.  Automatic choice: 1 when one t-slice holds
-// more than 32768 sites per checkerboard.  With order 0 every XCD sweeps its
-// own t-slab, so between the first and the last read of an input t-slice
-// the eight XCDs stream two slices each of every field; at 48^3 slices that
-// is ~760 MB (FP32 clover), beyond the 256 MB Infinity Cache, and the input
-// re-reads go to HBM.  Order 1 has the XCDs sweep t together, each owning a
-// z-range.  Measured (FP32/FP16 clov_dslash): 48^3x96 908 -> 855 us and
-// 428 -> 410 us; 48^3x48 444 -> 425 us; 32x32x64 and 64x32x32 slices equal;
-// 32^3 slices 152 -> 166 us (order 0 kept).
+// more than 32768 sites per checkerboard or Lt <= Lz, else 0.
+// * With order 0 every XCD sweeps its own t-slab.  Between the first and the
+//   last read of an input t-slice the eight XCDs then stream two slices each
+//   of every field: ~230 MB at 32^3 slices (FP32 clover), within the 256 MB
+//   Infinity Cache, but ~760 MB at 48^3 slices, where the input re-reads go
+//   to HBM.  Thin slabs (Lt/8 <= Lz/8 slices) also put most slices next to
+//   another XCD's slab.
+// * Order 1 has the XCDs sweep t together, each owning a z-range.
+// Measured FP32 clov_dslash, order 0 -> 1: 48^3x96 915 -> 855 us, 32^3x32
+// 82.7 -> 82.0, 32^3x16 46.6 -> 42.7, 32^3x8 24.3 -> 22.6, 32x32x64x64
+// 340 -> 334; order 0 kept: 32^3x64 152 -> 166, 32^3x128 331 -> 334,
+// 16^3x32 and 64x32x32x64 equal.
 inline int dslash_traversal_for(const Geom &g, int requested)
 {
   if (g.sz % kDslashBlock != 0) return 0;
   int ord = requested;
-  if (ord < 0) ord = (long)g.sz * g.Lz > 32768 ? 1 : 0;
+  if (ord < 0) ord = ((long)g.sz * g.Lz > 32768 || g.Lt <= g.Lz) ? 1 : 0;
   if (ord == 1 && g.Lz % kNumXcd == 0) return 1;
   if (ord == 2 && g.Lt % kNumXcd == 0) return 2;
   if (ord == 3 && g.Lt % kNumXcd == 0 && g.Lz % QAMD_ZPARTS == 0) return 3;
