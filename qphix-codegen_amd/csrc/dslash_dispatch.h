@@ -21,15 +21,22 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.cb = c.cb;
   p.nblocks = blocks_for(ctx->g.vcb, kDslashBlock);
   p.zs = p.ts = p.bpp = 1;
-  // LDS rows: whole work-groups per (z,t) plane, and the rows a work-group
-  // touches plus one on either side fit the staging array
+  // LDS rows: every work-group full (no lane leaves before the barrier),
+  // and the rows a work-group touches plus one on either side fit the
+  // staging array.  A work-group starts at a multiple of 128, so its offset
+  // in a row is a multiple of gcd(128, Lx/2); every traversal order keeps
+  // the 128 sites of a work-group consecutive.
   p.lds = 0;
-  // (every traversal order maps a work-group to 128 consecutive sites of one
-  // plane when the plane holds whole work-groups)
-  if (QAMD_LDS_XY && ctx->g.Lxh > 0 && ctx->g.sz % kDslashBlock == 0) {
+  if (QAMD_LDS_XY && ctx->g.Lxh > 0 && ctx->g.vcb % kDslashBlock == 0) {
+    int step = kDslashBlock, b = ctx->g.Lxh;
+    while (b) {
+      const int t = step % b;
+      step = b;
+      b = t;
+    }
     int worst = 0;
-    for (int pb = 0; pb < ctx->g.sz; pb += kDslashBlock) {
-      const int rows = (pb + kDslashBlock - 1) / ctx->g.Lxh - pb / ctx->g.Lxh + 3;
+    for (int o = 0; o < ctx->g.Lxh; o += step) {
+      const int rows = (o + kDslashBlock - 1) / ctx->g.Lxh + 3;
       worst = rows * ctx->g.Lxh > worst ? rows * ctx->g.Lxh : worst;
     }
     p.lds = worst <= kLdsSites;

@@ -182,18 +182,20 @@ __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site
 }
 
 // Input spinors staged in LDS (QAMD_LDS_XY): [16-byte chunk][staged site].
-// A work-group (128 consecutive sites of one (z, t) plane) stages the x rows
-// it touches and the row on either side of them once; the x and y hops (4 of
-// the 8) then read LDS instead of issuing global loads that mostly hit L2.  The barrier also splits the
+// A work-group (128 consecutive sites) stages the x rows it touches and the
+// row on either side of them once; the x and y hops (4 of the 8) then read
+// LDS instead of issuing global loads that mostly hit L2.  The barrier also splits the
 // kernel into phases, which lowers register pressure (FP32 166 -> 128 VGPRs,
 // occupancy 3 -> 4 waves/SIMD): clov_dslash FP32 155 -> 152 us, FP16
 // 81 -> 77 us at 32^3x64.
 #ifndef QAMD_LDS_XY
 #define QAMD_LDS_XY 1
 #endif
-// staged sites per work-group: 8 rows of 16 plus 2, 4 of 32 plus 2, or up to
-// 6 partial rows of 24 plus 2 (the host checks the geometry's worst case)
-constexpr int kLdsSites = kDslashBlock + 64;
+// staged sites per work-group: 8 rows of 16 plus 2, 4 of 32 plus 2, up to 6
+// partial rows of 24 plus 2, or up to 8 of 20 plus 2 (the host checks the
+// geometry's worst case).  FP32: 19,968 B, so 8 work-groups (4 waves per
+// SIMD) still fit a CU's 160 KB.
+constexpr int kLdsSites = kDslashBlock + 80;
 using Vec16 = typename RawVec<16>::T;
 
 template <typename S, typename R>
@@ -436,21 +438,29 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
   const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
 #if QAMD_LDS_XY
+  // A work-group inside one (z, t) plane stages its x rows y0 .. y1 and the
+  // rows y0 - 1 and y1 + 1 (y periodic) once; its x and y hops then read
+  // LDS.  One that straddles two planes (when a plane does not hold whole
+  // work-groups) reads its neighbours from global memory.  The host checks
+  // that every work-group is full (no lane leaves before the barrier, which
+  // the branch below reaches uniformly per work-group) and that the rows fit
+  // the array.
+  int y0 = 0, r1 = 0;
   if (p.lds) {
-    // the rows y0 .. y1 of one (z, t) plane that the work-group's sites lie
-    // in and the rows above and below (y periodic), staged once for the x
-    // and y hops; the work-group never straddles two planes (the host checks
-    // that a plane holds whole work-groups)
+    const int base = idx - (int)threadIdx.x;
+    const int r0 = base / g.Lxh;  // rows numbered over the checkerboard
+    r1 = (base + kDslashBlock - 1) / g.Lxh - r0;
+    y0 = r0 % g.Ly;
+  }
+  if (p.lds && y0 + r1 < g.Ly) {
     __shared__ Vec16 lds[(24 * sizeof(S) / 16) * kLdsSites];
     const int plane = row - c.y * g.Lxh;
-    const int pb = idx - (int)threadIdx.x - plane;  // first site's offset in the plane
-    const int y0 = pb / g.Lxh;
-    const int nst = ((pb + kDslashBlock - 1) / g.Lxh - y0 + 3) * g.Lxh;
+    const int nst = (r1 + 3) * g.Lxh;
     const Vec16 *vin = reinterpret_cast<const Vec16 *>(in);
     constexpr int NC = 24 * (int)sizeof(S) / 16;
     for (int j = (int)threadIdx.x; j < nst; j += kDslashBlock) {
-      const int yy = (y0 + j / g.Lxh - 1 + g.Ly) % g.Ly;
-      const int src = plane + yy * g.Lxh + j % g.Lxh;
+      const int k = j / g.Lxh;
+      const int src = plane + ((y0 + k - 1 + g.Ly) % g.Ly) * g.Lxh + (j - k * g.Lxh);
 #pragma unroll
       for (int ch = 0; ch < NC; ch++) lds[ch * kLdsSites + j] = vin[(unsigned)(ch * vcb + src)];
     }

@@ -161,15 +161,18 @@ def test_tbc_anisotropy_periodic(qamd, orc, lat8):
 
 
 @pytest.mark.parametrize("prec", [0, 1, 2, 3], ids=["f64", "f32", "f16", "bf16"])
-@pytest.mark.parametrize("dims", [(32, 16, 4, 4), (48, 16, 4, 4), (24, 32, 4, 4)],
-                         ids=["rows16", "rows24", "rows12"])
+@pytest.mark.parametrize("dims", [(32, 16, 4, 4), (48, 16, 4, 4), (24, 32, 4, 4), (24, 24, 4, 4),
+                                  (40, 8, 4, 4)],
+                         ids=["rows16", "rows24", "rows12", "planes288", "planes160"])
 def test_lds_rows_with_phases(qamd, orc, dims, prec):
-    """Lattices whose (z,t) planes hold whole 128-site work-groups, so the x
-    and y hops read the rows staged in LDS.  32x16x4x4: 8 whole x rows per
-    work-group, two work-groups per plane, the y wrap crossing between them;
-    48x16x4x4 and 24x32x4x4: work-groups that start and end inside a row
-    (5.3 and 10.7 rows).  Twisted phases on x, y and t, antiperiodic time,
-    every family, both isigns and checkerboards."""
+    """Lattices whose work-groups read their x and y neighbours from rows
+    staged in LDS.  32x16x4x4: 8 whole x rows per work-group, two work-groups
+    per plane, the y wrap crossing between them; 48x16x4x4 and 24x32x4x4:
+    work-groups that start and end inside a row (5.3 and 10.7 rows);
+    24x24x4x4 and 40x8x4x4: planes of 288 and 160 sites, where the
+    work-groups inside one plane use LDS and those straddling two planes
+    read globally, in the same launch.  Twisted phases on x, y and t,
+    antiperiodic time, every family, both isigns and checkerboards."""
     lat = make_lattice(qamd, dims, 1300)
     geom = qamd.Geometry(dims, t_boundary=-1.0)
     tbc = [1, 1, 0, 1]
