@@ -1,0 +1,214 @@
+// C++ host program over include/qphix_amd.hpp (test infrastructure).
+//
+//   host_api_check cpu
+//     context-only checks, no device work: geometry queries, the traversal
+//     choice, and the exceptions the classes throw on bad arguments.
+//   host_api_check gpu <in.bin> <out.bin>
+//     reads a lattice written by tests/test_gpu_cpp_host.py, runs every
+//     operator family through the C++ classes on the GPU (FP64 and FP32,
+//     12-real links, both isigns), writes the unpacked results to out.bin,
+//     then solves M^dag M x = b with the even-odd operator (FP64 CG and
+//     FP16 mixed CG) and prints one line "cg ..." with both results.
+//
+// in.bin: int32 dims[4], then float64 arrays, lexicographic: U [V][4][3][3][2],
+// psi, chi [V][4][3][2], clov, inv [V][2][36], and full blocks
+// [V][2][6][6][2] for (isign +1: B = inverse, F = direct), (isign -1: B, F).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "qphix_amd.hpp"
+
+namespace {
+
+int fail(const char *msg)
+{
+  std::fprintf(stderr, "host_api_check: %s\n", msg);
+  return 1;
+}
+
+#define HIPCHECK(x)                                              \
+  do {                                                           \
+    hipError_t e_ = (x);                                         \
+    if (e_ != hipSuccess) {                                      \
+      std::fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e_)); \
+      std::exit(1);                                              \
+    }                                                            \
+  } while (0)
+
+struct Dev {
+  void *p = nullptr;
+  explicit Dev(size_t bytes) { HIPCHECK(hipMalloc(&p, bytes)); }
+  ~Dev() { hipFree(p); }
+  Dev(const Dev &) = delete;
+  Dev &operator=(const Dev &) = delete;
+  double *d() const { return static_cast<double *>(p); }
+};
+
+int cpu_mode()
+{
+  using namespace qphix_amd;
+  Geometry g(16, 16, 8, 8, -1.0);
+  if (g.sites_cb() != 16 * 16 * 8 * 8 / 2) return fail("sites_cb");
+  if (g.field_bytes(QPHIX_AMD_FIELD_SPINOR, QPHIX_AMD_F32) != (size_t)g.sites_cb() * 24 * 4)
+    return fail("field_bytes");
+  if (g.traversal() != 1) return fail("automatic traversal (Lt <= Lz)");
+  g.set_traversal(0);
+  if (g.traversal() != 0) return fail("set_traversal");
+  bool threw = false;
+  try {
+    g.set_traversal(7);
+  } catch (const Error &e) {
+    threw = e.code() == QPHIX_AMD_EINVAL;
+  }
+  if (!threw) return fail("set_traversal(7) did not throw EINVAL");
+  threw = false;
+  try {
+    Geometry bad(15, 16, 8, 8);
+  } catch (const Error &e) {
+    threw = e.code() == QPHIX_AMD_EGEOM;
+  }
+  if (!threw) return fail("odd extent did not throw EGEOM");
+  // argument checks happen before any device work
+  Dslash D(g, QPHIX_AMD_F32, 12);
+  threw = false;
+  try {
+    D.dslash(nullptr, nullptr, nullptr, 2, 0);
+  } catch (const Error &e) {
+    threw = e.code() == QPHIX_AMD_EINVAL;
+  }
+  if (!threw) return fail("isign 2 did not throw EINVAL");
+  ClovDslash C(g, QPHIX_AMD_F16, 11);
+  threw = false;
+  try {
+    C.dslash(nullptr, nullptr, nullptr, nullptr, 1, 0);
+  } catch (const Error &e) {
+    threw = e.code() == QPHIX_AMD_EINVAL;
+  }
+  if (!threw) return fail("recon 11 did not throw EINVAL");
+  std::printf("cpu ok\n");
+  return 0;
+}
+
+int gpu_mode(const char *in_path, const char *out_path)
+{
+  using namespace qphix_amd;
+  FILE *f = std::fopen(in_path, "rb");
+  if (!f) return fail("cannot open input");
+  int dims[4];
+  if (std::fread(dims, sizeof(int), 4, f) != 4) return fail("short input");
+  const long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+  const long nU = V * 72, nS = V * 24, nC = V * 72, nF = V * 144;
+  std::vector<double> h(nU + 2 * nS + 2 * nC + 4 * nF);
+  if (std::fread(h.data(), sizeof(double), h.size(), f) != h.size()) return fail("short input");
+  std::fclose(f);
+  const double *hU = h.data(), *hpsi = hU + nU, *hchi = hpsi + nS, *hclov = hchi + nS, *hinv = hclov + nC;
+  const double *hBp = hinv + nC, *hFp = hBp + nF, *hBm = hFp + nF, *hFm = hBm + nF;
+
+  Dev U(nU * 8), psi(nS * 8), chi(nS * 8), clov(nC * 8), inv(nC * 8), Bp(nF * 8), Fp(nF * 8), Bm(nF * 8),
+      Fm(nF * 8), outlex(nS * 8);
+  HIPCHECK(hipMemcpy(U.p, hU, nU * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(psi.p, hpsi, nS * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(chi.p, hchi, nS * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(clov.p, hclov, nC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(inv.p, hinv, nC * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(Bp.p, hBp, nF * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(Fp.p, hFp, nF * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(Bm.p, hBm, nF * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(Fm.p, hFm, nF * 8, hipMemcpyHostToDevice));
+
+  Geometry geom(dims[0], dims[1], dims[2], dims[3], -1.0);
+  const double aniso_s = 0.9, aniso_t = 1.1, alpha = 4.2, beta = 0.25, rho = 0.05, mass = 0.2, tmu = 0.1;
+  FILE *o = std::fopen(out_path, "wb");
+  if (!o) return fail("cannot open output");
+  std::vector<double> host(nS);
+  const int precs[2] = {QPHIX_AMD_F64, QPHIX_AMD_F32};
+  for (int pi = 0; pi < 2; pi++) {
+    const int prec = precs[pi];
+    Dslash W(geom, prec, 12, aniso_s, aniso_t);
+    ClovDslash C(geom, prec, 12, aniso_s, aniso_t);
+    TMDslash T(geom, prec, 12, mass, tmu, aniso_s, aniso_t);
+    TMClovDslash TC(geom, prec, 12, aniso_s, aniso_t);
+    for (int isign = 1; isign >= -1; isign -= 2) {
+      const int cb = isign > 0 ? 0 : 1;
+      Dev g(geom.field_bytes(QPHIX_AMD_FIELD_GAUGE, prec, 12)), p(geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, prec)),
+          c(geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, prec)), r(geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, prec)),
+          A(geom.field_bytes(QPHIX_AMD_FIELD_CLOVER, prec)), Ai(geom.field_bytes(QPHIX_AMD_FIELD_CLOVER, prec)),
+          B(geom.field_bytes(QPHIX_AMD_FIELD_FULLCLOVER, prec)), F(geom.field_bytes(QPHIX_AMD_FIELD_FULLCLOVER, prec));
+      geom.pack_gauge(g.p, prec, 12, U.d(), cb);
+      geom.pack_spinor(p.p, prec, psi.d(), 1 - cb);
+      geom.pack_spinor(c.p, prec, chi.d(), cb);
+      geom.pack_clover(A.p, prec, clov.d(), cb);
+      geom.pack_clover(Ai.p, prec, inv.d(), cb);
+      geom.pack_full_clover(B.p, prec, (isign > 0 ? Bp : Bm).d(), cb);
+      geom.pack_full_clover(F.p, prec, (isign > 0 ? Fp : Fm).d(), cb);
+      for (int kind = 0; kind < 4; kind++) {
+        for (int achi = 0; achi < 2; achi++) {
+          switch (kind) {
+            case 0:
+              if (achi) W.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, alpha, beta, isign, cb);
+              else W.dslash(r.p, p.p, g.p, isign, cb);
+              break;
+            case 1:
+              if (achi) C.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, A.p, beta, isign, cb, rho);
+              else C.dslash(r.p, p.p, g.p, Ai.p, isign, cb);
+              break;
+            case 2:
+              if (achi) T.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, alpha, beta, isign, cb);
+              else T.dslash(r.p, p.p, g.p, isign, cb);
+              break;
+            default:
+              if (achi) TC.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, F.p, beta, isign, cb, rho);
+              else TC.dslash(r.p, p.p, g.p, B.p, isign, cb);
+          }
+          HIPCHECK(hipMemset(outlex.p, 0, nS * 8));
+          geom.unpack_spinor(outlex.d(), r.p, prec, cb);
+          HIPCHECK(hipMemcpy(host.data(), outlex.p, nS * 8, hipMemcpyDeviceToHost));
+          std::fwrite(host.data(), sizeof(double), nS, o);
+        }
+      }
+    }
+  }
+  std::fclose(o);
+
+  // even-odd Wilson-clover operator from the links, FP64 CG and FP16 mixed CG
+  Dev A(nC * 8);
+  geom.clover_from_links(A.d(), U.d(), 1.0 / (2 * 0.135), 1.0);
+  EvenOddOperator op(geom, U.d(), A.d(), 0.0, QPHIX_AMD_F16, 12);
+  const size_t sb = geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, QPHIX_AMD_F64);
+  Dev b(sb), x64(sb), x16(sb);
+  geom.pack_spinor(b.p, QPHIX_AMD_F64, psi.d(), 1);
+  HIPCHECK(hipMemset(x64.p, 0, sb));
+  HIPCHECK(hipMemset(x16.p, 0, sb));
+  qphix_amd_cg_result r64 = op.cg(x64.p, b.p, 1e-10, 2000, false);
+  qphix_amd_cg_result r16 = op.cg(x16.p, b.p, 1e-10, 2000, true);
+  std::vector<double> a64(sb / 8), a16(sb / 8);
+  HIPCHECK(hipMemcpy(a64.data(), x64.p, sb, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(a16.data(), x16.p, sb, hipMemcpyDeviceToHost));
+  double num = 0, den = 0;
+  for (size_t i = 0; i < a64.size(); i++) {
+    num += (a16[i] - a64[i]) * (a16[i] - a64[i]);
+    den += a64[i] * a64[i];
+  }
+  std::printf("cg f64_iters %d f64_rsd %.3e f16_iters %d f16_rsd %.3e f16_reliable %d rel_diff %.3e\n", r64.iters,
+              r64.rsd_final, r16.iters, r16.rsd_final, r16.reliable_updates, std::sqrt(num / den));
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+  try {
+    if (argc >= 2 && std::strcmp(argv[1], "cpu") == 0) return cpu_mode();
+    if (argc >= 4 && std::strcmp(argv[1], "gpu") == 0) return gpu_mode(argv[2], argv[3]);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "host_api_check: %s\n", e.what());
+    return 1;
+  }
+  return fail("usage: host_api_check cpu | gpu <in.bin> <out.bin>");
+}

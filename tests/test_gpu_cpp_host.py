@@ -1,0 +1,79 @@
+"""GPU test of the C++ host classes (include/qphix_amd.hpp) through the C++
+host program tests/cpp/host_api_check, built with the library.
+
+16x16x8x8: every family (Dslash, ClovDslash, TMDslash, TMClovDslash) x
+{dslash, dslashAChiMinusBDPsi} x isign +1 (cb 0) / -1 (cb 1), FP64 and FP32
+storage with 12-real links, anisotropic coefficients and antiperiodic time,
+against the FP64 oracle (1e-13 / 1e-6 relative); then the even-odd clover
+operator's FP64 CG and FP16 mixed CG to a true residual of 1e-10 and their
+agreement."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from _util import cb_mask
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "host_api_check")
+DIMS = (16, 16, 8, 8)
+# the scalars the host program uses
+ANISO_S, ANISO_T, ALPHA, BETA, RHO, MASS, TMU = 0.9, 1.1, 4.2, 0.25, 0.05, 0.2, 0.1
+
+
+def test_cpp_classes_against_oracle(qamd, orc, tmp_path):
+    assert os.path.exists(EXE), "tests/cpp/host_api_check is not built (build() builds it)"
+    L = qamd.lattice
+    U = L.random_gauge(DIMS, 0.3, 3100).numpy()
+    psi = L.random_spinor(DIMS, 3101).numpy()
+    chi = L.random_spinor(DIMS, 3102).numpy()
+    clov_t = L.random_clover(DIMS, 3103)
+    clov, inv = clov_t.numpy(), L.invert_clover(clov_t).numpy()
+    full = {(s, invert): L.tmclover_blocks(clov_t, TMU, s, invert).numpy() for s in (1, -1) for invert in (True, False)}
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        np.asarray(DIMS, dtype=np.int32).tofile(f)
+        for a in (U, psi, chi, clov, inv, full[1, True], full[1, False], full[-1, True], full[-1, False]):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    out = tmp_path / "out.bin"
+    print("running host_api_check gpu", DIMS, flush=True)
+    r = subprocess.run([EXE, "gpu", str(inp), str(out)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    V = int(np.prod(DIMS))
+    res = np.fromfile(out, dtype=np.float64).reshape(-1, V, 4, 3, 2)
+    assert res.shape[0] == 2 * 2 * 4 * 2
+    k = 0
+    for prec, tol in (("f64", 1e-13), ("f32", 1e-6)):
+        for isign in (1, -1):
+            cb = 0 if isign > 0 else 1
+            m = cb_mask(DIMS, cb)
+            for kind in range(4):
+                for achi in (0, 1):
+                    c = BETA if achi else 1.0
+                    kw = dict(kind=kind, achi=bool(achi), coeff_s=c * ANISO_S, coeff_t_f=c * ANISO_T,
+                              coeff_t_b=c * ANISO_T, t_boundary=-1.0, alpha=ALPHA, rho=RHO, chi=chi)
+                    if kind == 1:
+                        kw["clov"] = clov if achi else inv
+                    elif kind == 2:
+                        a_op = 4.0 + MASS
+                        kw["mu"] = TMU / ALPHA if achi else TMU / a_op
+                        kw["muinv"] = a_op / (a_op ** 2 + TMU ** 2)
+                    elif kind == 3:
+                        kw["clov"] = full[isign, not achi]
+                    want = orc.dslash(DIMS, psi, U, isign, cb, **kw)
+                    got = res[k]
+                    k += 1
+                    err = np.abs(got[m] - want[m]).max() / np.abs(want[m]).max()
+                    assert err < tol, (prec, isign, kind, achi, err)
+                    assert np.all(got[~m] == 0)
+    line = re.search(r"cg f64_iters (\d+) f64_rsd (\S+) f16_iters (\d+) f16_rsd (\S+) f16_reliable (\d+) "
+                     r"rel_diff (\S+)", r.stdout)
+    assert line, r.stdout
+    f64_rsd, f16_rsd, diff = float(line.group(2)), float(line.group(4)), float(line.group(6))
+    print(r.stdout.strip(), flush=True)
+    assert f64_rsd < 1e-10 and f16_rsd < 1e-10
+    assert diff < 1e-8
