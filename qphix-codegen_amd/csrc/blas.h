@@ -14,7 +14,11 @@ struct CGScalars {
   double tmp;      // result of a plain reduction
 };
 
-constexpr int kMaxPartials = 1024;
+// work-groups of a vector kernel (and partial sums of a reduction)
+#ifndef QAMD_MAX_PARTIALS
+#define QAMD_MAX_PARTIALS 1024
+#endif
+constexpr int kMaxPartials = QAMD_MAX_PARTIALS;
 
 int blas_grid(long nsites);
 // Re<a, b> over n = 24 * vcb reals (whole sites); kind: 0 -> alpha = rr / result,

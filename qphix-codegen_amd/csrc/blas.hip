@@ -8,6 +8,8 @@
 // partial sum, and a single work-group adds the partials in a fixed order.
 // Solver scalars (alpha, beta, residual norms) stay in device memory, so the
 // iteration is enqueued without host round trips.
+#include <type_traits>
+
 #include "blas.h"
 
 namespace qamd {
@@ -16,17 +18,39 @@ namespace {
 
 constexpr int kBlasBlock = 256;
 
-// all 24 reals of one site, each operand in its own native layout
-template <typename S, typename R>
-__device__ __forceinline__ void lds(const S *p, int vcb, int site, R v[24])
+// A site's 24 reals are handled in three chunks of 8 (a whole number of
+// native vectors for every storage type), so that even an FP64 update of
+// four operands keeps ~100 VGPRs and 4+ waves per SIMD.
+constexpr int kChunk = 8;
+
+template <int C, typename S, typename R>
+__device__ __forceinline__ void ldc(const S *__restrict__ p, int vcb, int site, R v[kChunk])
 {
-  load_reals<S, spinor_w<S>(), 0, 24>(p, vcb, site, v);
+  load_reals<S, spinor_w<S>(), C * kChunk, kChunk>(p, vcb, site, v);
 }
 
-template <typename S, typename R>
-__device__ __forceinline__ void sts(S *p, int vcb, int site, const R v[24])
+template <int C, typename S, typename R>
+__device__ __forceinline__ void stc(S *__restrict__ p, int vcb, int site, const R v[kChunk])
 {
-  store_reals<S, spinor_w<S>(), 0, 24>(p, vcb, site, v);
+  store_reals<S, spinor_w<S>(), C * kChunk, kChunk>(p, vcb, site, v);
+}
+
+// f(integral_constant<int, c>) for the chunks c = 0, 1, 2
+template <typename F>
+__device__ __forceinline__ void for_chunks(F &&f)
+{
+  f(std::integral_constant<int, 0>{});
+  f(std::integral_constant<int, 1>{});
+  f(std::integral_constant<int, 2>{});
+}
+
+// v rounded to storage S and back, in registers: exactly the values a load
+// of what stc() stores would return
+template <typename S, typename R>
+__device__ __forceinline__ void round_to(R v[kChunk])
+{
+#pragma unroll
+  for (int k = 0; k < kChunk; k++) v[k] = (R)to_real(to_storage<S>(v[k]));
 }
 
 #define QAMD_SITE_LOOP(site, vcb) \
@@ -47,19 +71,22 @@ __device__ __forceinline__ double block_sum(double v)
 }
 
 template <typename S>
-__global__ __launch_bounds__(kBlasBlock) void dot_k(const S *a, const S *b, int vcb,
+__global__ __launch_bounds__(kBlasBlock) void dot_k(const S *__restrict__ a, const S *__restrict__ b, int vcb,
                                                     double *partial)
 {
   using R = typename Traits<S>::R;
   double acc = 0.0;
   QAMD_SITE_LOOP(site, vcb)
   {
-    R x[24], y[24];
-    lds(a, vcb, site, x);
-    lds(b, vcb, site, y);
     R s = 0;
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      R x[kChunk], y[kChunk];
+      ldc<C>(a, vcb, site, x);
+      ldc<C>(b, vcb, site, y);
 #pragma unroll
-    for (int k = 0; k < 24; k++) s += x[k] * y[k];
+      for (int k = 0; k < kChunk; k++) s += x[k] * y[k];
+    });
     acc += (double)s;
   }
   const double t = block_sum(acc);
@@ -68,28 +95,31 @@ __global__ __launch_bounds__(kBlasBlock) void dot_k(const S *a, const S *b, int 
 
 // y = a*x + b*y (a, b from the host), optional |y|^2
 template <typename SX, typename SY>
-__global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *x, SY *y, double a, double b,
-                                                      int vcb, double *partial)
+__global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *__restrict__ x, SY *__restrict__ y, double a,
+                                                      double b, int vcb, double *partial)
 {
   using R = typename Traits<SY>::R;
   double acc = 0.0;
   QAMD_SITE_LOOP(site, vcb)
   {
-    double xv[24], yv[24];
-    lds(x, vcb, site, xv);
-    if (b != 0.0)
-      lds(y, vcb, site, yv);
-    else
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      double xv[kChunk], yv[kChunk];
+      ldc<C>(x, vcb, site, xv);
+      if (b != 0.0)
+        ldc<C>(y, vcb, site, yv);
+      else
 #pragma unroll
-      for (int k = 0; k < 24; k++) yv[k] = 0.0;
-    R o[24];
+        for (int k = 0; k < kChunk; k++) yv[k] = 0.0;
+      R o[kChunk];
 #pragma unroll
-    for (int k = 0; k < 24; k++) o[k] = (R)(a * xv[k] + b * yv[k]);
-    sts(y, vcb, site, o);
-    // norm of the stored (rounded) values
-    lds(y, vcb, site, xv);
+      for (int k = 0; k < kChunk; k++) o[k] = (R)(a * xv[k] + b * yv[k]);
+      stc<C>(y, vcb, site, o);
+      // norm of the stored (rounded) values
+      round_to<SY>(o);
 #pragma unroll
-    for (int k = 0; k < 24; k++) acc += xv[k] * xv[k];
+      for (int k = 0; k < kChunk; k++) acc += (double)o[k] * (double)o[k];
+    });
   }
   if (partial) {
     const double t = block_sum(acc);
@@ -99,9 +129,9 @@ __global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *x, SY *y, double
 
 // CG step: r -= alpha Ap; x += alpha p; partial |r|^2.  alpha = sc->alpha.
 template <typename S, typename SXS>
-__global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *r, SXS *x, const S *p, const S *Ap,
-                                                          const CGScalars *sc, int vcb,
-                                                          double *partial)
+__global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *__restrict__ r, SXS *__restrict__ x,
+                                                          const S *__restrict__ p, const S *__restrict__ Ap,
+                                                          const CGScalars *sc, int vcb, double *partial)
 {
   using R = typename Traits<S>::R;
   using RX = typename Traits<SXS>::R;
@@ -109,24 +139,27 @@ __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *r, SXS *x, const S 
   double acc = 0.0;
   QAMD_SITE_LOOP(site, vcb)
   {
-    R rv[24], pv[24], av[24];
-    RX xv[24];
-    lds(r, vcb, site, rv);
-    lds(p, vcb, site, pv);
-    lds(Ap, vcb, site, av);
-    lds(x, vcb, site, xv);
-#pragma unroll
-    for (int k = 0; k < 24; k++) {
-      rv[k] = rv[k] - alpha * av[k];
-      xv[k] = xv[k] + (RX)(alpha * pv[k]);
-    }
-    sts(r, vcb, site, rv);
-    sts(x, vcb, site, xv);
-    // norm of the stored (rounded) residual
-    lds(r, vcb, site, rv);
     R s = 0;
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      R rv[kChunk], pv[kChunk], av[kChunk];
+      RX xv[kChunk];
+      ldc<C>(r, vcb, site, rv);
+      ldc<C>(p, vcb, site, pv);
+      ldc<C>(Ap, vcb, site, av);
+      ldc<C>(x, vcb, site, xv);
 #pragma unroll
-    for (int k = 0; k < 24; k++) s += rv[k] * rv[k];
+      for (int k = 0; k < kChunk; k++) {
+        rv[k] = rv[k] - alpha * av[k];
+        xv[k] = xv[k] + (RX)(alpha * pv[k]);
+      }
+      stc<C>(r, vcb, site, rv);
+      stc<C>(x, vcb, site, xv);
+      // norm of the stored (rounded) residual
+      round_to<S>(rv);
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) s += rv[k] * rv[k];
+    });
     acc += (double)s;
   }
   const double t = block_sum(acc);
@@ -135,18 +168,22 @@ __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *r, SXS *x, const S 
 
 // p = r + beta p, beta = sc->beta
 template <typename S>
-__global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *p, const S *r, const CGScalars *sc, int vcb)
+__global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *__restrict__ p, const S *__restrict__ r,
+                                                        const CGScalars *sc, int vcb)
 {
   using R = typename Traits<S>::R;
   const R beta = (R)sc->beta;
   QAMD_SITE_LOOP(site, vcb)
   {
-    R rv[24], pv[24];
-    lds(r, vcb, site, rv);
-    lds(p, vcb, site, pv);
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      R rv[kChunk], pv[kChunk];
+      ldc<C>(r, vcb, site, rv);
+      ldc<C>(p, vcb, site, pv);
 #pragma unroll
-    for (int k = 0; k < 24; k++) pv[k] = rv[k] + beta * pv[k];
-    sts(p, vcb, site, pv);
+      for (int k = 0; k < kChunk; k++) pv[k] = rv[k] + beta * pv[k];
+      stc<C>(p, vcb, site, pv);
+    });
   }
 }
 
