@@ -242,6 +242,11 @@ void blas_dot(int prec, const void *a, const void *b, long n, double *partial, C
   hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, kind);
 }
 
+void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, hipStream_t st)
+{
+  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, (int)n, sc, kind);
+}
+
 void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double b, long n,
                 double *partial, CGScalars *sc, hipStream_t st)
 {

@@ -38,6 +38,9 @@ template <typename R> struct DslashParams {
   int tbc_mask;
   R tbc_re[4], tbc_im[4];
   R alpha, mu, muinv, rho;
+  // when set: per work-group sums of |out|^2 over the stored values, at
+  // norm[first site / kDslashBlock] (every work-group full)
+  double *norm;
 };
 
 #ifndef QAMD_XCD_REMAP
@@ -355,7 +358,10 @@ __device__ __forceinline__ void twist(const cx<R> in[4][3], cx<R> out[4][3], R a
 #define QAMD_WAVES_ATTR
 #endif
 
-template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
+// NORM: also sum |out|^2 per work-group into p.norm (instantiated for the
+// clover and twisted-mass-clover achimbdpsi kernels, the second half of the
+// even-odd operator, whose |M p|^2 is the CG's <p, M^dag M p>)
+template <typename S, int RECON, int OP, bool ACHI, int ISIGN, bool NORM = false>
 __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {
   using R = typename Traits<S>::R;
@@ -537,6 +543,28 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       o[(s * 3 + k) * 2 + 1] = res[s][k].im;
     }
   store_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
+  if (NORM) {
+    // |out|^2 of the stored (rounded) values: lanes in FP32/FP64, the wave
+    // and the work-group in FP64 in a fixed order
+    R s2 = 0;
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+      const R v = (R)to_real(to_storage<S>(o[k]));
+      s2 += v * v;
+    }
+    double w = (double)s2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+    __shared__ double wsum[kDslashBlock / 64];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < kDslashBlock / 64; i++) t += wsum[i];
+      p.norm[(idx - (int)threadIdx.x) / kDslashBlock] = t;
+    }
+  }
 }
 
 // Host-side launch for one instantiation.
@@ -544,6 +572,14 @@ template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
 hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream_t stream)
 {
   const int grid = xcd_grid(p.nblocks);
+  if constexpr (ACHI && (OP == OP_CLOVER || OP == OP_TMCLOVER)) {
+    if (p.norm) {
+      hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, true>), dim3(grid), dim3(kDslashBlock), 0,
+                         stream, p);
+      return hipGetLastError();
+    }
+  }
+  if (p.norm) return hipErrorInvalidValue;
   hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN>), dim3(grid), dim3(kDslashBlock), 0,
                      stream, p);
   return hipGetLastError();

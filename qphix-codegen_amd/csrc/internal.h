@@ -22,6 +22,9 @@ struct DslashCall {
   void *out;
   const void *in, *chi, *gauge, *clov;
   double coeff_s, coeff_tf, coeff_tb, alpha, mu, muinv, rho;
+  // optional: per work-group |out|^2 partials, vcb / 128 of them (only when
+  // vcb is a multiple of 128)
+  double *norm;
 };
 
 // The work-group traversal order the operator calls use on this geometry.

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "blas.h"
+#include "dslash_kernel.h"
 
 using namespace qamd;
 
@@ -70,8 +71,9 @@ namespace {
 int level_prec(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? QPHIX_AMD_F64 : op->prec_s; }
 int level_recon(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? 12 : op->recon_s; }
 
+// norm: when set, per work-group partials of |out|^2 (vcb / 128 of them)
 hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
-                    hipStream_t st)
+                    hipStream_t st, double *norm = nullptr)
 {
   const int isign = dagger ? -1 : 1;
   DslashCall c{};
@@ -99,6 +101,7 @@ hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const v
   c.clov = op->achi[lvl][dagger].p;
   c.coeff_s = c.coeff_tf = c.coeff_tb = 0.25;
   c.rho = 0.0;
+  c.norm = norm;
   return run_dslash_any(op->ctx, c, st);
 }
 
@@ -121,11 +124,12 @@ hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
   return run_dslash_any(op->ctx, c, st);
 }
 
-// out = M^dag M in, via the cb-1 temporary mid
+// out = M^dag M in, via the cb-1 temporary mid = M in; with norm set, also
+// the partials of |M in|^2 = <in, M^dag M in>
 hipError_t eo_normal(qphix_amd_eo_op *op, int lvl, void *out, void *mid, const void *in,
-                     hipStream_t st)
+                     hipStream_t st, double *norm = nullptr)
 {
-  hipError_t e = eo_apply(op, lvl, 0, mid, in, st);
+  hipError_t e = eo_apply(op, lvl, 0, mid, in, st, norm);
   if (e != hipSuccess) return e;
   return eo_apply(op, lvl, 1, out, mid, st);
 }
@@ -342,6 +346,15 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
 #ifndef QAMD_CG_CHECK_BF16
 #define QAMD_CG_CHECK_BF16 1
 #endif
+// <p, M^dag M p> = |M p|^2 summed by the kernel that stores M p, instead of
+// a dot product of p and M^dag M p (which reads both again).  Not for BF16
+// storage: its summing kernel needs 175 VGPRs (2 waves per SIMD) against 133.
+#ifndef QAMD_CG_FUSED_NORM
+#define QAMD_CG_FUSED_NORM 1
+#endif
+#ifndef QAMD_CG_FUSED_NORM_BF16
+#define QAMD_CG_FUSED_NORM_BF16 0
+#endif
 
 int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_amd_cg_params *prm,
                        qphix_amd_cg_result *res, void *stream)
@@ -364,7 +377,12 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
                        : ps == QPHIX_AMD_F16  ? 0.05
                                               : 0.01;
 
-  DevBuf r64, ax64, rs, p, ap, xs, mid, scb, part, host;
+  // fused |M p|^2 needs every work-group of the operator kernels full
+  const bool fuse = QAMD_CG_FUSED_NORM && vcb % kDslashBlock == 0 &&
+                    (ps != QPHIX_AMD_BF16 || QAMD_CG_FUSED_NORM_BF16);
+  const long nwg = vcb / kDslashBlock;
+  DevBuf r64, ax64, rs, p, ap, xs, mid, scb, part, pnorm, host;
+  if (fuse && pnorm.alloc(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
   if (r64.alloc(n * 8) != hipSuccess || ax64.alloc(n * 8) != hipSuccess ||
       rs.alloc(n * sbytes(ps)) != hipSuccess || p.alloc(n * sbytes(ps)) != hipSuccess ||
       ap.alloc(n * sbytes(ps)) != hipSuccess || xs.alloc(n * sbytes(px)) != hipSuccess ||
@@ -436,10 +454,13 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
                                prm->max_iters - iters);
     for (int k = 0; k < batch && e == hipSuccess; k++) {
       if (k > 0) blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
-      e = eo_normal(op, lvl, ap.p, mid.p, p.p, st);
+      e = eo_normal(op, lvl, ap.p, mid.p, p.p, st, fuse ? (double *)pnorm.p : nullptr);
       mv += 2;
       if (e != hipSuccess) break;
-      blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
+      if (fuse)
+        blas_finalize((double *)pnorm.p, nwg, sc, 0, st);
+      else
+        blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
       blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
       iters++;
     }
