@@ -347,13 +347,9 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
 #define QAMD_CG_CHECK_BF16 1
 #endif
 // <p, M^dag M p> = |M p|^2 summed by the kernel that stores M p, instead of
-// a dot product of p and M^dag M p (which reads both again).  Not for BF16
-// storage: its summing kernel needs 175 VGPRs (2 waves per SIMD) against 133.
+// a dot product of p and M^dag M p (which reads both again)
 #ifndef QAMD_CG_FUSED_NORM
 #define QAMD_CG_FUSED_NORM 1
-#endif
-#ifndef QAMD_CG_FUSED_NORM_BF16
-#define QAMD_CG_FUSED_NORM_BF16 0
 #endif
 
 int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_amd_cg_params *prm,
@@ -378,8 +374,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
                                               : 0.01;
 
   // fused |M p|^2 needs every work-group of the operator kernels full
-  const bool fuse = QAMD_CG_FUSED_NORM && vcb % kDslashBlock == 0 &&
-                    (ps != QPHIX_AMD_BF16 || QAMD_CG_FUSED_NORM_BF16);
+  const bool fuse = QAMD_CG_FUSED_NORM && vcb % kDslashBlock == 0;
   const long nwg = vcb / kDslashBlock;
   DevBuf r64, ax64, rs, p, ap, xs, mid, scb, part, pnorm, host;
   if (fuse && pnorm.alloc(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
