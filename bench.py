@@ -336,9 +336,17 @@ def main():
                     help="clover32: the 32^3x64 clover operator (the headline line); "
                          "tmclover48: the 48^3x96 twisted-mass clover case")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="launch each operator kernel 10 times plus a 1 GiB device copy (for "
-                         "rocprofv3 --pmc runs and their copy-kernel calibration), print nothing")
+                    help="launch clov_dslash, clov_dslash_achimbdpsi and Wilson dslash 10 times "
+                         "each plus a 1 GiB device copy (for rocprofv3 --pmc runs and their "
+                         "copy-kernel calibration), print nothing")
+    ap.add_argument("--lattice", default=None,
+                    help="with --kernels-only: Lx,Ly,Lz,Lt (default 32,32,32,64)")
+    ap.add_argument("--traversal", type=int, default=-1,
+                    help="with --kernels-only: work-group traversal order (-1 automatic)")
     args = ap.parse_args()
+    if (args.lattice or args.traversal != -1) and not args.kernels_only:
+        print("--lattice and --traversal apply to --kernels-only runs", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -354,15 +362,17 @@ def main():
         print(json.dumps(tmclover48(args, q, dev)), flush=True)
         return
     prec, recon = q.F32, 12
-    geom = q.Geometry(DIMS, t_boundary=-1.0)
+    dims = tuple(int(v) for v in args.lattice.split(",")) if args.lattice else DIMS
+    geom = q.Geometry(dims, t_boundary=-1.0)
+    geom.set_traversal(args.traversal)
     vcb = geom.vcb
 
     sets = []
     for k in range(2):
-        U = L.random_gauge(DIMS, 0.3, 1000 + 10 * k, device=dev)
-        clov = L.random_clover(DIMS, 1003 + 10 * k, device=dev)
+        U = L.random_gauge(dims, 0.3, 1000 + 10 * k, device=dev)
+        clov = L.random_clover(dims, 1003 + 10 * k, device=dev)
         inv = L.invert_clover(clov)
-        psi = L.random_spinor(DIMS, 1001 + 10 * k, device=dev)
+        psi = L.random_spinor(dims, 1001 + 10 * k, device=dev)
         s = dict(g0=geom.pack_gauge(U, prec, recon, 0), g1=geom.pack_gauge(U, prec, recon, 1),
                  inv0=geom.pack_clover(inv, prec, 0), clov1=geom.pack_clover(clov, prec, 1),
                  psi1=geom.pack_spinor(psi, prec, 1), t0=geom.new_spinor(prec, cb=0),
@@ -386,6 +396,10 @@ def main():
         for i in range(10):
             s = sets[i & 1]
             op.dslashAChiMinusBDPsi(s["out1"], s["t0"], s["psi1"], s["g1"], s["clov1"], 0.25, 1, 1)
+        wil = q.Dslash(geom)
+        for i in range(10):
+            s = sets[i & 1]
+            wil.dslash(s["t0"], s["psi1"], s["g0"], 1, 0)
         del sets
         a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
