@@ -28,8 +28,10 @@ void blas_dot(int prec, const void *a, const void *b, long n, double *partial, C
 // y = a x + b y (types may differ); with sc != null, sc->tmp = |y|^2.
 void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double b, long n,
                 double *partial, CGScalars *sc, hipStream_t st);
-// Sum n partials in a fixed order into sc (kind as for blas_dot).
-void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, hipStream_t st);
+// Sum n partials in a fixed order into sc (kind as for blas_dot); above
+// 2048 partials in two stages through scratch (n / 2048 doubles, at most
+// kMaxPartials).
+void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, double *scratch, hipStream_t st);
 // r -= alpha Ap, x += alpha p, then beta = |r|^2 / rr, rr = |r|^2.
 void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const void *Ap, long n,
                     double *partial, CGScalars *sc, hipStream_t st);

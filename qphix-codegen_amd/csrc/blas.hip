@@ -187,6 +187,25 @@ __global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *__restrict__ p, const
   }
 }
 
+// First stage of a long sum: work-group w adds partial[w*2048 .. +2048)
+// (strided over its threads, then the fixed work-group tree) into out[w].
+constexpr int kReduceChunk = 8 * kBlasBlock;
+
+__global__ __launch_bounds__(kBlasBlock) void reduce_partials_k(const double *__restrict__ partial, int n,
+                                                                double *__restrict__ out)
+{
+  const int base = blockIdx.x * kReduceChunk;
+  const int end = base + kReduceChunk < n ? base + kReduceChunk : n;
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < kReduceChunk / kBlasBlock; k++) {
+    const int i = base + k * kBlasBlock + (int)threadIdx.x;
+    if (i < end) v += partial[i];
+  }
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 // Sum the partials in a fixed order and update the scalars:
 //   kind 0: pAp  -> alpha = rr / pAp
 //   kind 1: rr_new -> beta = rr_new / rr, rr = rr_new
@@ -242,8 +261,14 @@ void blas_dot(int prec, const void *a, const void *b, long n, double *partial, C
   hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, kind);
 }
 
-void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, hipStream_t st)
+void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, double *scratch, hipStream_t st)
 {
+  if (n > kReduceChunk) {  // one work-group per 2048 partials first
+    const int g = (int)((n + kReduceChunk - 1) / kReduceChunk);
+    hipLaunchKernelGGL(reduce_partials_k, dim3(g), dim3(kBlasBlock), 0, st, partial, (int)n, scratch);
+    hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, scratch, g, sc, kind);
+    return;
+  }
   hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, (int)n, sc, kind);
 }
 

@@ -375,7 +375,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
 
   // fused |M p|^2 needs every work-group of the operator kernels full
   const bool fuse = QAMD_CG_FUSED_NORM && vcb % kDslashBlock == 0;
-  const long nwg = vcb / kDslashBlock;
+  const long nwg = vcb / kDslashBlock;  // (vcb < 2^31 / 72: nwg / 2048 < kMaxPartials)
   DevBuf r64, ax64, rs, p, ap, xs, mid, scb, part, pnorm, host;
   if (fuse && pnorm.alloc(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
   if (r64.alloc(n * 8) != hipSuccess || ax64.alloc(n * 8) != hipSuccess ||
@@ -453,7 +453,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
       mv += 2;
       if (e != hipSuccess) break;
       if (fuse)
-        blas_finalize((double *)pnorm.p, nwg, sc, 0, st);
+        blas_finalize((double *)pnorm.p, nwg, sc, 0, partial, st);
       else
         blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
       blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
