@@ -6,7 +6,10 @@
   F_oo - 1/4 D_oe F_ee^-1 D_eo of its own dslash;
 * CG on M^dag M, FP64 and mixed precision (FP32/FP16/BF16 sloppy), to a
   true relative residual of 1e-10 recomputed by the FP64 oracle on the CPU,
-  and mixed vs FP64 solution agreement.
+  and mixed vs FP64 solution agreement;
+* repeated solves bit-identical; the CG on a lattice whose checkerboard is
+  not a whole number of 128-site work-groups (separate dot product instead
+  of the norm fused into the operator kernel).
 """
 import numpy as np
 import pytest
@@ -116,6 +119,51 @@ def test_cg_fp64_and_mixed(qamd, orc, lat, mu):
     for prec_s, xs in sols.items():
         d = np.linalg.norm(xs - ref) / np.linalg.norm(ref)
         assert d < 1e-8, (prec_s, d)
+
+
+def test_cg_repeatable(qamd, lat):
+    """Deterministic reductions: two solves of the same system give the same
+    iterations and bit-identical solutions, for FP64 and the mixed solves."""
+    dims, U, b = lat
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+    for prec_s in (qamd.F64, qamd.F16, qamd.BF16):
+        op = qamd.EvenOddCloverOperator(geom, Ud, A, prec_sloppy=prec_s, recon_sloppy=12)
+        runs = []
+        for _ in range(2):
+            x = geom.new_spinor(qamd.F64, cb=1)
+            res = qamd.InvCG(op, max_iters=5000)(x, bf, 1e-10, mixed=prec_s != qamd.F64)
+            runs.append((res["iters"], res["reliable_updates"], x.data.clone()))
+        assert runs[0][:2] == runs[1][:2], prec_s
+        assert torch.equal(runs[0][2], runs[1][2]), prec_s
+        op.close()
+
+
+def test_cg_without_fused_norm(qamd, orc):
+    """4x4x4x6: 192 sites per checkerboard, not a whole number of 128-site
+    work-groups, so the CG takes <p, M^dag M p> from a separate dot product
+    instead of the norm fused into the operator kernel."""
+    dims = (4, 4, 4, 6)
+    L = qamd.lattice
+    U = L.random_gauge(dims, 0.2, 2100)
+    b = L.random_spinor(dims, 2101)
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    An = A.cpu().numpy()
+    bn = on_cb(b.numpy(), dims, 1)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+    for prec_s in (qamd.F64, qamd.F16):
+        op = qamd.EvenOddCloverOperator(geom, Ud, A, prec_sloppy=prec_s, recon_sloppy=12)
+        x = geom.new_spinor(qamd.F64, cb=1)
+        res = qamd.InvCG(op, max_iters=5000)(x, bf, 1e-10, mixed=prec_s != qamd.F64)
+        assert res["rsd_final"] < 1e-10
+        xn = geom.unpack_spinor(x).cpu().numpy()
+        r = _true_residual(orc, dims, U.numpy(), An, xn, bn, 0.0)
+        assert r < 1.5e-10, (prec_s, r)
+        op.close()
 
 
 def test_cg_reports_nonconvergence(qamd, lat):
