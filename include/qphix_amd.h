@@ -144,7 +144,10 @@ int qphix_amd_pack_full_clover(qphix_amd_ctx *ctx, void *dst, int prec,
    the reference's kernels read it (packed gauges, codegen/data_types.cc:
    218-241).  Requirements: soalen divides Lx/2 and veclen, ngy divides Ly,
    pad_xy and pad_xyz are non-negative multiples of ngy; ft is F64, F32 or
-   F16 (IEEE binary16, the reference's `half`, decl_common.h:7).  Padding
+   F16 (IEEE binary16, the reference's `half`, decl_common.h:7).  A host
+   built without USE_PACKED_GAUGES / USE_PACKED_CLOVER (gauge and clover
+   blocks of soalen lanes, one per spinor block, codegen/data_types.cc:4-30)
+   passes veclen = soalen.  Padding
    blocks are neither read nor written.  The gauge conversion rebuilds row 2
    of compressed blocks in FP64 before storing the applied matrices.    */
 typedef struct {
