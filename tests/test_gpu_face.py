@@ -9,8 +9,9 @@ with a gauge halo provides them -- runs the operator (the interior part)
 and packs its faces; the faces are exchanged between the parts on the host
 side; the finish calls complete the boundary sites.  The assembled result
 must equal the FP64 oracle's operator on the whole lattice (1e-13 in FP64,
-1e-6 in FP32), for every family, dslash and achimbdpsi, both isigns and
-checkerboards, antiperiodic time, and with twisted boundary phases.
+1e-6 in FP32, the storage rounding in FP16 and BF16), for every family,
+dslash and achimbdpsi, both isigns and checkerboards, antiperiodic time, and
+with twisted boundary phases.
 """
 import itertools
 
@@ -169,6 +170,24 @@ def test_split_lattice_equals_whole_lattice(qamd, orc, split):
                 got = split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon)
                 err = np.abs(got[m] - ref[m]).max() / np.abs(ref[m]).max()
                 assert err < tol, (split, kind, achi, isign, cb, prec, err)
+
+
+@pytest.mark.parametrize("split", ["t2", "xyzt16", "x2_rows"])
+def test_split_low_precision_storage(qamd, orc, split):
+    """FP16 and BF16 storage (FP32 arithmetic; the face buffers in the same
+    storage): the assembled result equals the whole-lattice oracle within
+    the storage rounding, 5e-3 (FP16) and 4e-2 (BF16) relative as for the
+    unsplit kernels."""
+    G, grid = SPLITS[split]
+    lat = _lattice(qamd, G, 730)
+    for kind, achi in ((0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1)):
+        isign, cb = (1, 0) if achi == 0 else (-1, 1)
+        ref = oracle_apply(orc, lat, G, kind, achi, isign, cb)
+        m = cb_mask(G, cb)
+        for prec, tol in ((qamd.F16, 5e-3), (qamd.BF16, 4e-2)):
+            got = split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, 12)
+            err = np.abs(got[m] - ref[m]).max() / np.abs(ref[m]).max()
+            assert err < tol, (split, kind, achi, prec, err)
 
 
 @pytest.mark.parametrize("split", ["xyzt16", "x2_rows", "y2_rows"])
