@@ -12,7 +12,8 @@ hold a step's working set.
 
 Prints one JSON line: metric, value = GFLOP/s of the step, the roofline of
 the dominant kernel from HIP-event timing, "cg" = FP64 vs mixed-precision CG
-time to a true residual of 1e-10 (config 2), and the CPU baseline (the
+time to a true residual of 1e-10 (the 32^3x64 Wilson-clover solve), and
+the CPU baseline (the
 reference's own AVX-512/AVX2 kernel on the host cores).
 """
 import argparse
@@ -122,7 +123,7 @@ def profile_traffic():
 
 
 def tm_kernel(q, geom, s, dev, n):
-    """Config 1's twisted-mass variant: tm_dslash, FP32, 12-real links,
+    """The twisted-mass variant of the 32^3x64 operator: tm_dslash, FP32, 12-real links,
     mu = 0.1, kappa = 0.1350 (alpha = 1/(2 kappa); the kernels take mu/alpha
     and alpha/(alpha^2 + mu^2))."""
     alpha, mu = 1.0 / (2 * 0.1350), 0.1
@@ -219,7 +220,7 @@ def sloppy_kernels(q, dev):
 
 
 def cg_compare(q, dev):
-    """Config 2: even-odd Wilson-clover M^dag M at 32^3x64, kappa = 0.1350,
+    """The 32^3x64 solve: even-odd Wilson-clover M^dag M, kappa = 0.1350,
     csw = 1.0, links sigma = 0.2; FP64 CG vs mixed-precision reliable-update
     CG (FP16 and BF16 storage, FP32 arithmetic, FP64 true residuals) to a
     true relative residual of 1e-10 on the same Gaussian right-hand side."""
@@ -249,7 +250,7 @@ def cg_compare(q, dev):
 
 
 def tmclover48(args, q, dev):
-    """Config 4: 48^3x96 (5,308,416 sites/cb) twisted-mass clover, mu = 0.05,
+    """The 48^3x96 (5,308,416 sites/cb) twisted-mass clover workload, mu = 0.05,
     kappa = 0.1355, csw = 1.5, links sigma = 0.2.
       * dslash throughput in half storage (FP16 spinors, 12-real FP16 links,
         FP16 full clover blocks, FP32 arithmetic) and in FP32;
