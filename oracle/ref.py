@@ -29,7 +29,7 @@ def lib(config):
         l = ctypes.CDLL(path(config))
         i, d = ctypes.c_int, ctypes.c_double
         l.ref_apply.argtypes = [i, i, i, i, i, i, i, i, i, _dp, _dp, _dp, _dp, _dp, d, d, d, d, d,
-                                d, d, d, i, i, _dp]
+                                d, d, d, i, i, _dp, _dp]
         l.ref_apply.restype = i
         l.ref_veclen.restype = i
         l.ref_precision.restype = i
@@ -53,18 +53,23 @@ def _p(a):
 
 def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=None, recon=18,
           coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.0, t_boundary=-1.0, alpha=0.0, mu=0.0,
-          muinv=0.0, rho=0.0, nthreads=1, reps=0):
+          muinv=0.0, rho=0.0, nthreads=1, reps=0, tbc_phases=None):
     """Run one reference kernel over checkerboard cb.  Returns (out, seconds
-    per repetition or None); out is (V,4,3,2) with the cb sites set."""
+    per repetition or None); out is (V,4,3,2) with the cb sites set.
+    tbc_phases: four complex forward-hop phases (x, y, z, t) -> the bodies
+    generated with twisted boundary conditions in every dimension."""
     Lx, Ly, Lz, Lt = dims
     V = Lx * Ly * Lz * Lt
     c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
     psi, U, chi, clov = c(psi), c(U), c(chi), c(clov)
     out = np.zeros((V, 4, 3, 2))
     sec = ctypes.c_double(0.0)
+    ph = None
+    if tbc_phases is not None:
+        ph = np.array([v for z in tbc_phases for v in (complex(z).real, complex(z).imag)], dtype=np.float64)
     rc = lib(config).ref_apply(kind, int(achi), isign, recon, cb, Lx, Ly, Lz, Lt, _p(psi), _p(chi),
                                _p(U), _p(clov), _p(out), coeff_s, coeff_t_f, coeff_t_b, t_boundary,
-                               alpha, mu, muinv, rho, nthreads, reps, ctypes.byref(sec))
+                               alpha, mu, muinv, rho, nthreads, reps, ctypes.byref(sec), _p(ph))
     if rc != 0:
         raise ValueError(f"reference kernel rejected the call (config {config}, dims {dims})")
     return out, (sec.value if reps > 0 else None)

@@ -279,11 +279,14 @@ int ref_lowprec(void)
 // FP64 arrays; psi is read on checkerboard 1-cb, chi on cb, and out is
 // written on cb.  Returns 0, or -1 for an unsupported geometry/argument.
 // reps > 0 times the block loop (not the packing) and stores seconds/rep.
+// tbc_phases (8 doubles, (re, im) per dimension x y z t) selects the bodies
+// generated with twisted boundary conditions in every dimension and passes
+// the phases (codegen/dslash.cc:206-230); NULL selects the simple bodies.
 int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, int Lz, int Lt,
               const double *psi, const double *chi, const double *U, const double *clov,
               double *out, double coeff_s, double coeff_t_f, double coeff_t_b,
               double t_boundary, double alpha, double mu, double muinv, double rho,
-              int nthreads, int reps, double *seconds)
+              int nthreads, int reps, double *seconds, const double *tbc_phases)
 {
   if (kind < 0 || kind > 3 || (recon != 12 && recon != 18) || (cb != 0 && cb != 1)) return -1;
   if ((Lx | Ly | Lz | Lt) & 1) return -1;
@@ -310,7 +313,8 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
   if (kind == 1) pack_clover(cl, clov, g, cb);
   if (kind == 3) pack_full_clover(cl, clov, g, cb);
 
-  ref_body_fn fn = ref_table[kind][achi ? 1 : 0][isign > 0 ? 0 : 1][recon == 12 ? 0 : 1];
+  ref_body_fn fn = (tbc_phases ? ref_table_tbc : ref_table)[kind][achi ? 1 : 0][isign > 0 ? 0 : 1]
+                                                          [recon == 12 ? 0 : 1];
   const unsigned int all = VECLEN >= 32 ? 0xffffffffu : ((1u << VECLEN) - 1u);
 
   auto run = [&]() {
@@ -359,6 +363,11 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
       A.mu = to_ft(mu);
       A.muinv = to_ft(muinv);
       A.prec_mass_rho = to_ft(rho);
+      if (tbc_phases)
+        for (int m = 0; m < 4; m++) {
+          A.tbc_phases[m][0] = to_ft(tbc_phases[2 * m]);
+          A.tbc_phases[m][1] = to_ft(tbc_phases[2 * m + 1]);
+        }
       fn(A);
     }
   };

@@ -114,6 +114,48 @@ def test_oracle_matches_reference_kernels_live(qamd, orc, config):
                         assert err < tol, (kind, achi, isign, cb, recon, err)
 
 
+@pytest.mark.parametrize("config", ["avx512_f64", "avx512_f32", "avx2_f64"])
+def test_oracle_matches_reference_twisted_bc_kernels_live(qamd, orc, config):
+    """The reference's bodies generated with twisted boundary conditions in
+    every dimension ("tttt", codegen/dslash.cc:595-630): a distinct U(1)
+    phase per dimension on every forward hop, its conjugate on backward hops
+    (codegen/dslash_common.cc:1491-1524), for every family, both isigns and
+    checkerboards, 12/18-real links, against the oracle's tbc."""
+    from oracle import ref
+    if not ref.available(config):
+        pytest.skip("oracle/_ref not built (needs the reference sources at build time)")
+    if "avx512" in config and not _cpu_has("avx512f"):
+        pytest.skip("host CPU has no AVX-512")
+    if "avx2" in config and not _cpu_has("avx2"):
+        pytest.skip("host CPU has no AVX2")
+    L = qamd.lattice
+    dims = REF_DIMS[config]
+    U = L.random_gauge(dims, 0.3, 41).numpy()
+    psi = L.random_spinor(dims, 42).numpy()
+    chi = L.random_spinor(dims, 43).numpy()
+    clt = L.random_clover(dims, 44)
+    phases = [np.exp(1j * t) for t in (0.3, -0.7, 1.1, 2.0)]
+    tol = 1e-13 if config.endswith("f64") else 1e-6
+    kw0 = dict(alpha=3.7, mu=0.027, muinv=0.26, rho=0.05, coeff_s=0.9, coeff_t_f=1.1, coeff_t_b=1.2,
+               t_boundary=-1.0)
+    for kind in range(4):
+        for achi in (0, 1):
+            for isign, cb in ((1, 0), (-1, 1), (1, 1)):
+                kw = dict(kw0, kind=kind, achi=bool(achi), chi=chi)
+                if kind == 1:
+                    kw["clov"] = clt.numpy() if achi else L.invert_clover(clt).numpy()
+                if kind == 3:
+                    kw["clov"] = L.tmclover_blocks(clt, 0.1, isign, not achi).numpy()
+                a = orc.dslash(dims, psi, U, isign, cb, tbc=[1, 1, 1, 1], tbc_phases=phases, **kw)
+                plain = orc.dslash(dims, psi, U, isign, cb, **kw)
+                m = cb_mask(dims, cb)
+                assert np.abs(a[m] - plain[m]).max() > 0.1 * np.abs(a[m]).max()  # the phases matter
+                for recon in (12, 18):
+                    b, _ = ref.apply(config, dims, psi, U, isign, cb, recon=recon, tbc_phases=phases, **kw)
+                    err = np.abs(a[m] - b[m]).max() / np.abs(a[m]).max()
+                    assert err < tol, (kind, achi, isign, cb, recon, err)
+
+
 def _f16_fixture_case(orc, f, case):
     """The FP64 oracle's result for one case of the FP16 fixture, rounded to
     FP16, with the links the 12-real kernels apply."""

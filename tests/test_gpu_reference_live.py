@@ -14,6 +14,9 @@ both checkerboards, with anisotropic coefficients and antiperiodic time.
   up to 9 ulps from the exact result on elements near 1e-4 of a field whose
   largest element is 19), and the same bits in >= 99% of the elements.
 
+The same with the reference's twisted-boundary bodies ("tttt") and a complex
+phase per dimension.
+
 Skipped where the reference build or AVX-512 is absent.
 """
 import numpy as np
@@ -136,3 +139,41 @@ def test_gpu_equals_reference_kernels(qamd, lat, config):
                         worst = max(worst, err)
     print(f"{config}: worst {'fraction of differing FP16 elements' if prec == qamd.F16 else 'relative error'}"
           f" {worst:.3g}", flush=True)
+
+
+# forward-hop phases per dimension, exactly representable in FP16 (a plain
+# complex factor; the kernels do not assume unit modulus)
+TBC_PHASES = [0.75 + 0.5j, 0.5 - 0.875j, -0.625 + 0.75j, 0.875 + 0.25j]
+
+
+@pytest.mark.parametrize("config", ["avx512_f64", "avx512_f32", "avx512_f16"])
+def test_gpu_equals_reference_twisted_bc_kernels(qamd, lat, config):
+    """The reference's bodies generated with twisted boundary conditions in
+    every dimension ("tttt", codegen/dslash.cc:595-630) against the GPU
+    kernels with qphix_amd_set_tbc on all four dimensions: every family x
+    {dslash, achimbdpsi}, both isigns and checkerboards, 12/18-real links, to
+    the same bars as above."""
+    ref = _ref(config)
+    prec = {"avx512_f64": qamd.F64, "avx512_f32": qamd.F32, "avx512_f16": qamd.F16}[config]
+    if prec == qamd.F16:
+        lat = _quantized(lat)
+    geom = qamd.Geometry(DIMS, t_boundary=P["t_boundary"])
+    geom.set_tbc([1, 1, 1, 1], TBC_PHASES)
+    for kind in range(4):
+        for achi in (0, 1):
+            for isign, cb in ((1, 0), (-1, 1)):
+                for recon in (12, 18):
+                    ep = _epi(lat, kind, achi, isign)
+                    want, _ = ref.apply(config, DIMS, lat["psi"], lat["U"], isign, cb, kind=kind, achi=achi,
+                                        chi=lat["chi"], clov=ep, recon=recon, nthreads=16,
+                                        tbc_phases=TBC_PHASES, **P)
+                    got = _gpu(qamd, geom, prec, kind, achi, isign, cb, recon, lat)
+                    m = cb_mask(DIMS, cb)
+                    case = (config, kind, achi, isign, cb, recon)
+                    if prec == qamd.F16:
+                        ulps = f16_ulps(got[m].astype(np.float16), want[m].astype(np.float16))
+                        same = float(np.mean(ulps == 0))
+                        assert f16_within(got[m], want[m]).all() and same >= 0.99, (case, ulps.max(), same)
+                    else:
+                        err = np.abs(got[m] - want[m]).max() / np.abs(want[m]).max()
+                        assert err < (1e-13 if prec == qamd.F64 else 1e-6), (case, err)
