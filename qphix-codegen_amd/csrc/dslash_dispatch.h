@@ -65,6 +65,7 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.mu = (R)c.mu;
   p.muinv = (R)c.muinv;
   p.rho = (R)c.rho;
+  p.one = (R)1;
   p.norm = c.norm;
   return dispatch_dslash<S>(c.recon, c.op, c.achi, c.isign, p, st);
 }

@@ -15,6 +15,8 @@
 //   (codegen/dslash_common.cc:784-890).
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace qamd {
@@ -38,6 +40,7 @@ template <typename R> struct DslashParams {
   int tbc_mask;
   R tbc_re[4], tbc_im[4];
   R alpha, mu, muinv, rho;
+  R one;  // 1, opaque to the compiler: the FP16 projection's FMA multiplier
   // when set: per work-group sums of |out|^2 over the stored values, at
   // norm[first site / kDslashBlock] (every work-group full)
   double *norm;
@@ -201,27 +204,91 @@ __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site
 constexpr int kLdsSites = kDslashBlock + 80;
 using Vec16 = typename RawVec<16>::T;
 
+// A neighbour spinor in registers.  FP16 storage keeps the raw halves: its
+// projection reads them directly with mixed-precision FMAs (one v_fma_mix
+// per projected real instead of two conversions and an add; the FP16
+// operator kernels are close to the VALU issue limit).  Every other storage
+// type is converted to the arithmetic type on load.
+template <typename S, typename R> struct NbrSpinor {
+  R v[24];
+};
+template <> struct NbrSpinor<__half, float> {
+  __half v[24];
+};
+
 template <typename S, typename R>
-__device__ __forceinline__ void load_spinor_lds(const Vec16 *lds, int j, R *psi)
+__device__ __forceinline__ void load_nbr(const S *__restrict__ in, int vcb, int site, NbrSpinor<S, R> &n)
+{
+  if constexpr (std::is_same<S, __half>::value) {
+    constexpr int W = spinor_w<S>();
+    using V = typename RawVec<W * sizeof(S)>::T;
+    const V *vb = reinterpret_cast<const V *>(in);
+#pragma unroll
+    for (int c = 0; c < 24 / W; c++) {
+      V v = vb[(unsigned)(c * vcb + site)];
+      __builtin_memcpy(n.v + c * W, &v, sizeof(V));
+    }
+  } else {
+    load_reals<S, spinor_w<S>(), 0, 24>(in, vcb, site, n.v);
+  }
+}
+
+template <typename S, typename R>
+__device__ __forceinline__ void load_nbr_lds(const Vec16 *lds, int j, NbrSpinor<S, R> &n)
 {
   constexpr int W = 16 / (int)sizeof(S);
 #pragma unroll
   for (int c = 0; c < 24 / W; c++) {
     Vec16 v = lds[c * kLdsSites + j];
-    S tmp[W];
-    __builtin_memcpy(tmp, &v, 16);
+    if constexpr (std::is_same<S, __half>::value) {
+      __builtin_memcpy(n.v + c * W, &v, 16);
+    } else {
+      S tmp[W];
+      __builtin_memcpy(tmp, &v, 16);
 #pragma unroll
-    for (int e = 0; e < W; e++) psi[c * W + e] = (R)to_real(tmp[e]);
+      for (int e = 0; e < W; e++) n.v[c * W + e] = (R)to_real(tmp[e]);
+    }
+  }
+}
+
+// (1 + SGN*gamma_MU) psi from raw FP16 components: every projected real is
+// a + s b = fma(b, s, a) with s = +-one, exact products and one rounding,
+// the same value as the FP32 add of the converted components.
+template <int MU, int SGN>
+__device__ __forceinline__ void project_mixed(const __half *p, float one, cx<float> h[2][3])
+{
+  const float s = SGN > 0 ? one : -one;
+  auto f = [](__half v) { return __half2float(v); };
+  auto re = [&](int sp, int c) { return f(p[(sp * 3 + c) * 2]); };
+  auto im = [&](int sp, int c) { return f(p[(sp * 3 + c) * 2 + 1]); };
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (MU == 0) {  // h0 = p0 + s i p3, h1 = p1 + s i p2
+      h[0][c] = {fmaf(im(3, c), -s, re(0, c)), fmaf(re(3, c), s, im(0, c))};
+      h[1][c] = {fmaf(im(2, c), -s, re(1, c)), fmaf(re(2, c), s, im(1, c))};
+    } else if (MU == 1) {  // h0 = p0 - s p3, h1 = p1 + s p2
+      h[0][c] = {fmaf(re(3, c), -s, re(0, c)), fmaf(im(3, c), -s, im(0, c))};
+      h[1][c] = {fmaf(re(2, c), s, re(1, c)), fmaf(im(2, c), s, im(1, c))};
+    } else if (MU == 2) {  // h0 = p0 + s i p2, h1 = p1 - s i p3
+      h[0][c] = {fmaf(im(2, c), -s, re(0, c)), fmaf(re(2, c), s, im(0, c))};
+      h[1][c] = {fmaf(im(3, c), s, re(1, c)), fmaf(re(3, c), -s, im(1, c))};
+    } else {  // h0 = p0 + s p2, h1 = p1 + s p3
+      h[0][c] = {fmaf(re(2, c), s, re(0, c)), fmaf(im(2, c), s, im(0, c))};
+      h[1][c] = {fmaf(re(3, c), s, re(1, c)), fmaf(im(3, c), s, im(1, c))};
+    }
   }
 }
 
 template <typename S, int MU, int FB, int ISIGN, typename R>
-__device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const R *psi, const cx<R> u[3][3], R beta,
-                                          bool tbc, cx<R> phase)
+__device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const NbrSpinor<S, R> &n, const cx<R> u[3][3],
+                                          R beta, bool tbc, cx<R> phase, R one)
 {
   constexpr int SGN = FB == 0 ? ISIGN : -ISIGN;
   cx<R> h[2][3];
-  project<MU, SGN>(psi, h);
+  if constexpr (std::is_same<S, __half>::value)
+    project_mixed<MU, SGN>(n.v, one, h);
+  else
+    project<MU, SGN>(n.v, h);
   cx<R> w[2][3];
 #pragma unroll
   for (int k = 0; k < 2; k++)
@@ -236,40 +303,33 @@ __device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const R *psi, const c
   reconstruct<MU, SGN>(acc, w, beta);
 }
 
-template <typename S, int MU, int FB, int ISIGN, typename R>
-__device__ __forceinline__ void hop_dir(cx<R> acc[4][3], const S *__restrict__ in,
-                                        const cx<R> u[3][3], int vcb, int nbr, R beta, bool tbc,
-                                        cx<R> phase)
-{
-  R psi[24];
-  load_reals<S, spinor_w<S>(), 0, 24>(in, vcb, nbr, psi);
-  hop_apply<S, MU, FB, ISIGN>(acc, psi, u, beta, tbc, phase);
-}
-
 template <typename S, int RECON, int MU, int ISIGN, typename R>
 __device__ __forceinline__ void hop_mu(cx<R> acc[4][3], const S *__restrict__ in,
                                        const S *__restrict__ gauge, int vcb, int site, int nb,
-                                       int nf, R bb, R bf, bool tbc, cx<R> phase)
+                                       int nf, R bb, R bf, bool tbc, cx<R> phase, R one)
 {
   cx<R> ub[3][3], uf[3][3];
   load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf);
-  hop_dir<S, MU, 0, ISIGN>(acc, in, ub, vcb, nb, bb, tbc, conj(phase));
-  hop_dir<S, MU, 1, ISIGN>(acc, in, uf, vcb, nf, bf, tbc, phase);
+  NbrSpinor<S, R> n;
+  load_nbr(in, vcb, nb, n);
+  hop_apply<S, MU, 0, ISIGN>(acc, n, ub, bb, tbc, conj(phase), one);
+  load_nbr(in, vcb, nf, n);
+  hop_apply<S, MU, 1, ISIGN>(acc, n, uf, bf, tbc, phase, one);
 }
 
 // the same with both neighbours read from the staged LDS rows
 template <typename S, int RECON, int MU, int ISIGN, typename R>
 __device__ __forceinline__ void hop_mu_lds(cx<R> acc[4][3], const Vec16 *lds, const S *__restrict__ gauge,
                                            int vcb, int site, int jb, int jf, R bb, R bf, bool tbc,
-                                           cx<R> phase)
+                                           cx<R> phase, R one)
 {
   cx<R> ub[3][3], uf[3][3];
   load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf);
-  R psi[24];
-  load_spinor_lds<S>(lds, jb, psi);
-  hop_apply<S, MU, 0, ISIGN>(acc, psi, ub, bb, tbc, conj(phase));
-  load_spinor_lds<S>(lds, jf, psi);
-  hop_apply<S, MU, 1, ISIGN>(acc, psi, uf, bf, tbc, phase);
+  NbrSpinor<S, R> n;
+  load_nbr_lds<S>(lds, jb, n);
+  hop_apply<S, MU, 0, ISIGN>(acc, n, ub, bb, tbc, conj(phase), one);
+  load_nbr_lds<S>(lds, jf, n);
+  hop_apply<S, MU, 1, ISIGN>(acc, n, uf, bf, tbc, phase, one);
 }
 
 // out = A in, A = two packed Hermitian 6x6 blocks
@@ -473,17 +533,17 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     __syncthreads();
     const int rl = c.y - y0 + 1;  // staged row of this site
     hop_mu_lds<S, RECON, 0, ISIGN>(acc, lds, gauge, vcb, idx, rl * g.Lxh + (xb >> 1),
-                                   rl * g.Lxh + (xf >> 1), bxb, bxf, tm & 1, ph0);
+                                   rl * g.Lxh + (xf >> 1), bxb, bxf, tm & 1, ph0, p.one);
     hop_mu_lds<S, RECON, 1, ISIGN>(acc, lds, gauge, vcb, idx, (rl - 1) * g.Lxh + c.xh,
-                                   (rl + 1) * g.Lxh + c.xh, byb, byf, tm & 2, ph1);
+                                   (rl + 1) * g.Lxh + c.xh, byb, byf, tm & 2, ph1, p.one);
   } else
 #endif
   {
-    hop_mu<S, RECON, 0, ISIGN>(acc, in, gauge, vcb, idx, n_xb, n_xf, bxb, bxf, tm & 1, ph0);
-    hop_mu<S, RECON, 1, ISIGN>(acc, in, gauge, vcb, idx, n_yb, n_yf, byb, byf, tm & 2, ph1);
+    hop_mu<S, RECON, 0, ISIGN>(acc, in, gauge, vcb, idx, n_xb, n_xf, bxb, bxf, tm & 1, ph0, p.one);
+    hop_mu<S, RECON, 1, ISIGN>(acc, in, gauge, vcb, idx, n_yb, n_yf, byb, byf, tm & 2, ph1, p.one);
   }
-  hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bzb, bzf, tm & 4, ph2);
-  hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3);
+  hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bzb, bzf, tm & 4, ph2, p.one);
+  hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3, p.one);
 
   cx<R> res[4][3];
   if (!ACHI) {
