@@ -37,6 +37,10 @@ void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const
                     double *partial, CGScalars *sc, hipStream_t st);
 // p = r + beta p
 void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st);
+// x += alpha p, then (update_p) p = r + beta p: the vector half of a CG step
+// whose r -= alpha Ap and |r|^2 the operator's last kernel did
+void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long n, CGScalars *sc,
+                    bool update_p, hipStream_t st);
 
 hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const double *U,
                                  double alpha, double csw, hipStream_t st);

@@ -187,6 +187,38 @@ __global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *__restrict__ p, const
   }
 }
 
+// CG step after an operator launch that already updated r (and summed
+// |r|^2): x += alpha p, then (P) p = r + beta p; alpha, beta = sc->alpha,
+// sc->beta.  The arithmetic is that of cg_update_k (x) and cg_xpay_k (p).
+template <typename S, typename SXS, bool P>
+__global__ __launch_bounds__(kBlasBlock) void cg_xpay_x_k(S *__restrict__ p, SXS *__restrict__ x,
+                                                          const S *__restrict__ r, const CGScalars *sc,
+                                                          int vcb)
+{
+  using R = typename Traits<S>::R;
+  using RX = typename Traits<SXS>::R;
+  const R alpha = (R)sc->alpha, beta = (R)sc->beta;
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      R pv[kChunk], rv[kChunk];
+      RX xv[kChunk];
+      ldc<C>(p, vcb, site, pv);
+      ldc<C>(x, vcb, site, xv);
+      if constexpr (P) ldc<C>(r, vcb, site, rv);
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) xv[k] = xv[k] + (RX)(alpha * pv[k]);
+      stc<C>(x, vcb, site, xv);
+      if constexpr (P) {
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) pv[k] = rv[k] + beta * pv[k];
+        stc<C>(p, vcb, site, pv);
+      }
+    });
+  }
+}
+
 // First stage of a long sum: work-group w adds partial[w*2048 .. +2048)
 // (strided over its threads, then the fixed work-group tree) into out[w].
 constexpr int kReduceChunk = 8 * kBlasBlock;
@@ -317,6 +349,31 @@ void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const
   QAMD_PREC_SWITCH(prec, QAMD_CGU)
 #undef QAMD_CGU
   hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, 1);
+}
+
+void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long n, CGScalars *sc,
+                    bool update_p, hipStream_t st)
+{
+  const int ng = (int)(n / 24);  // sites
+  const int grid = blas_grid(ng);
+#define QAMD_XPX_P(T, TX, P)                                                                    \
+  hipLaunchKernelGGL((cg_xpay_x_k<T, TX, P>), dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(p), \
+                     mp<TX>(x), cp<T>(r), sc, ng)
+#define QAMD_XPX_X(T, TX)        \
+  if (update_p)                  \
+    QAMD_XPX_P(T, TX, true);     \
+  else                           \
+    QAMD_XPX_P(T, TX, false);
+#define QAMD_XPX(T)                   \
+  if (prec_x == QPHIX_AMD_F64) {      \
+    QAMD_XPX_X(T, double)             \
+  } else {                            \
+    QAMD_XPX_X(T, float)              \
+  }
+  QAMD_PREC_SWITCH(prec, QAMD_XPX)
+#undef QAMD_XPX
+#undef QAMD_XPX_X
+#undef QAMD_XPX_P
 }
 
 void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st)

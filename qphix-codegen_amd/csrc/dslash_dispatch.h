@@ -67,6 +67,7 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.rho = (R)c.rho;
   p.one = (R)1;
   p.norm = c.norm;
+  p.step = c.step;
   return dispatch_dslash<S>(c.recon, c.op, c.achi, c.isign, p, st);
 }
 

@@ -44,7 +44,15 @@ template <typename R> struct DslashParams {
   // when set: per work-group sums of |out|^2 over the stored values, at
   // norm[first site / kDslashBlock] (every work-group full)
   double *norm;
+  // when set (with norm): out holds a CG residual r, and the launch stores
+  // r - (*step) * result in its place; norm then sums |r|^2
+  const double *step;
 };
+
+// What a launch does with its result at a site: store it; store it and sum
+// |out|^2 per work-group; or subtract it, times the device-resident CG step
+// length, from the residual held in out, and sum |r|^2 per work-group.
+enum { EPI_STORE = 0, EPI_NORM = 1, EPI_RESIDUAL = 2 };
 
 #ifndef QAMD_XCD_REMAP
 #define QAMD_XCD_REMAP 1
@@ -418,10 +426,11 @@ __device__ __forceinline__ void twist(const cx<R> in[4][3], cx<R> out[4][3], R a
 #define QAMD_WAVES_ATTR
 #endif
 
-// NORM: also sum |out|^2 per work-group into p.norm (instantiated for the
-// clover and twisted-mass-clover achimbdpsi kernels, the second half of the
-// even-odd operator, whose |M p|^2 is the CG's <p, M^dag M p>)
-template <typename S, int RECON, int OP, bool ACHI, int ISIGN, bool NORM = false>
+// EPI (EPI_STORE, EPI_NORM, EPI_RESIDUAL) is instantiated beyond EPI_STORE
+// for the clover and twisted-mass-clover achimbdpsi kernels, the second half
+// of the even-odd operator M: EPI_NORM on M p gives the CG's
+// <p, M^dag M p> = |M p|^2, EPI_RESIDUAL on M^dag (M p) its residual update.
+template <typename S, int RECON, int OP, bool ACHI, int ISIGN, int EPI = EPI_STORE>
 __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {
   using R = typename Traits<S>::R;
@@ -602,8 +611,16 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       o[(s * 3 + k) * 2] = res[s][k].re;
       o[(s * 3 + k) * 2 + 1] = res[s][k].im;
     }
+  if constexpr (EPI == EPI_RESIDUAL) {
+    // r - step * result; each site reads and writes only its own r
+    R rv[24];
+    load_reals<S, spinor_w<S>(), 0, 24, R>(static_cast<const S *>(p.out), vcb, idx, rv);
+    const R a = (R)*p.step;
+#pragma unroll
+    for (int k = 0; k < 24; k++) o[k] = rv[k] - a * o[k];
+  }
   store_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
-  if (NORM) {
+  if constexpr (EPI != EPI_STORE) {
     // |out|^2 of the stored (rounded) values: lanes in FP32/FP64, the wave
     // and the work-group in FP64 in a fixed order
     R s2 = 0;
@@ -633,13 +650,21 @@ hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream
 {
   const int grid = xcd_grid(p.nblocks);
   if constexpr (ACHI && (OP == OP_CLOVER || OP == OP_TMCLOVER)) {
-    if (p.norm) {
-      hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, true>), dim3(grid), dim3(kDslashBlock), 0,
-                         stream, p);
+    if (p.norm && !p.step) {
+      hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_NORM>), dim3(grid), dim3(kDslashBlock),
+                         0, stream, p);
       return hipGetLastError();
     }
+    // the residual update follows M^dag (isign -1), the second half of M^dag M
+    if constexpr (ISIGN == -1) {
+      if (p.norm && p.step) {
+        hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_RESIDUAL>), dim3(grid),
+                           dim3(kDslashBlock), 0, stream, p);
+        return hipGetLastError();
+      }
+    }
   }
-  if (p.norm) return hipErrorInvalidValue;
+  if (p.norm || p.step) return hipErrorInvalidValue;
   hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN>), dim3(grid), dim3(kDslashBlock), 0,
                      stream, p);
   return hipGetLastError();

@@ -25,6 +25,9 @@ struct DslashCall {
   // optional: per work-group |out|^2 partials, vcb / 128 of them (only when
   // vcb is a multiple of 128)
   double *norm;
+  // optional, with norm, for the clover achimbdpsi kernels with isign -1:
+  // out holds a residual r and receives r - (*step) * result
+  const double *step;
 };
 
 // The work-group traversal order the operator calls use on this geometry.

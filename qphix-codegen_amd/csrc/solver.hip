@@ -71,9 +71,11 @@ namespace {
 int level_prec(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? QPHIX_AMD_F64 : op->prec_s; }
 int level_recon(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? 12 : op->recon_s; }
 
-// norm: when set, per work-group partials of |out|^2 (vcb / 128 of them)
+// norm: when set, per work-group partials of |out|^2 (vcb / 128 of them);
+// step: when set too (dagger only), out holds a residual r, which receives
+// r - (*step) M^dag in, and norm gets the partials of its |r|^2
 hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
-                    hipStream_t st, double *norm = nullptr)
+                    hipStream_t st, double *norm = nullptr, const double *step = nullptr)
 {
   const int isign = dagger ? -1 : 1;
   DslashCall c{};
@@ -102,6 +104,7 @@ hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const v
   c.coeff_s = c.coeff_tf = c.coeff_tb = 0.25;
   c.rho = 0.0;
   c.norm = norm;
+  c.step = step;
   return run_dslash_any(op->ctx, c, st);
 }
 
@@ -124,12 +127,11 @@ hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
   return run_dslash_any(op->ctx, c, st);
 }
 
-// out = M^dag M in, via the cb-1 temporary mid = M in; with norm set, also
-// the partials of |M in|^2 = <in, M^dag M in>
+// out = M^dag M in, via the cb-1 temporary mid = M in
 hipError_t eo_normal(qphix_amd_eo_op *op, int lvl, void *out, void *mid, const void *in,
-                     hipStream_t st, double *norm = nullptr)
+                     hipStream_t st)
 {
-  hipError_t e = eo_apply(op, lvl, 0, mid, in, st, norm);
+  hipError_t e = eo_apply(op, lvl, 0, mid, in, st);
   if (e != hipSuccess) return e;
   return eo_apply(op, lvl, 1, out, mid, st);
 }
@@ -347,7 +349,10 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
 #define QAMD_CG_CHECK_BF16 1
 #endif
 // <p, M^dag M p> = |M p|^2 summed by the kernel that stores M p, instead of
-// a dot product of p and M^dag M p (which reads both again)
+// a dot product of p and M^dag M p (which reads both again); and the
+// residual update r -= alpha M^dag M p with its |r|^2 done by the kernel that
+// computes M^dag M p, which then is never stored (x += alpha p moves into the
+// p update, which reads p anyway)
 #ifndef QAMD_CG_FUSED_NORM
 #define QAMD_CG_FUSED_NORM 1
 #endif
@@ -380,7 +385,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   if (fuse && pnorm.alloc(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
   if (r64.alloc(n * 8) != hipSuccess || ax64.alloc(n * 8) != hipSuccess ||
       rs.alloc(n * sbytes(ps)) != hipSuccess || p.alloc(n * sbytes(ps)) != hipSuccess ||
-      ap.alloc(n * sbytes(ps)) != hipSuccess || xs.alloc(n * sbytes(px)) != hipSuccess ||
+      (!fuse && ap.alloc(n * sbytes(ps)) != hipSuccess) || xs.alloc(n * sbytes(px)) != hipSuccess ||
       mid.alloc(n * (size_t)sbytes(QPHIX_AMD_F64)) != hipSuccess ||
       scb.alloc(sizeof(CGScalars)) != hipSuccess || part.alloc(kMaxPartials * 8) != hipSuccess)
     return QPHIX_AMD_ENOMEM;
@@ -448,15 +453,25 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
     const int batch = std::min(ps == QPHIX_AMD_BF16 ? QAMD_CG_CHECK_BF16 : QAMD_CG_CHECK,
                                prm->max_iters - iters);
     for (int k = 0; k < batch && e == hipSuccess; k++) {
-      if (k > 0) blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
-      e = eo_normal(op, lvl, ap.p, mid.p, p.p, st, fuse ? (double *)pnorm.p : nullptr);
-      mv += 2;
-      if (e != hipSuccess) break;
-      if (fuse)
-        blas_finalize((double *)pnorm.p, nwg, sc, 0, partial, st);
-      else
+      if (fuse) {
+        // (x += alpha p, p = r + beta p); mid = M p with the partials of
+        // |M p|^2 -> alpha; r -= alpha M^dag mid with those of |r|^2 -> beta
+        if (k > 0) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, true, st);
+        double *pn = (double *)pnorm.p;
+        e = eo_apply(op, lvl, 0, mid.p, p.p, st, pn);
+        if (e != hipSuccess) break;
+        blas_finalize(pn, nwg, sc, 0, partial, st);
+        e = eo_apply(op, lvl, 1, rs.p, mid.p, st, pn, &sc->alpha);
+        if (e != hipSuccess) break;
+        blas_finalize(pn, nwg, sc, 1, partial, st);
+      } else {
+        if (k > 0) blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+        e = eo_normal(op, lvl, ap.p, mid.p, p.p, st);
+        if (e != hipSuccess) break;
         blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
-      blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
+        blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
+      }
+      mv += 2;
       iters++;
     }
     if (e != hipSuccess) break;
@@ -472,6 +487,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
                                 : (rest * rest < target2);
     if (reliable) {
       // fold the increment into x, recompute the true residual in FP64
+      if (fuse) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, false, st);
       blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
       e = hipMemsetAsync(xs.p, 0, n * sbytes(px), st);
       if (e != hipSuccess) break;
@@ -493,7 +509,10 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
       sigma = sig_new;
       rmax = std::sqrt(r2);
     } else {
-      blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+      if (fuse)
+        blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, true, st);
+      else
+        blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
       if (rest > rmax) rmax = rest;
     }
   }
