@@ -12,8 +12,9 @@
 // that QPhiX's even-odd clover operator builds from the same two generated
 // kernels (codegen/jinja/clov_dslash_decl.h.j2).
 //
-// Mixed-precision CG: the iterated residual r, search direction p and A p
-// live in the sloppy storage type (FP16/BF16/FP32 storage, FP32 arithmetic),
+// Mixed-precision CG: the iterated residual r and search direction p (and
+// A p, where the operator's last kernel does not update r itself) live in
+// the sloppy storage type (FP16/BF16/FP32 storage, FP32 arithmetic),
 // in units of a scale sigma = |r_true| / sqrt(n) fixed at each reliable
 // update so that FP16's range holds them.  The iterate increment is
 // accumulated in FP32 and folded into the FP64 solution at each reliable
