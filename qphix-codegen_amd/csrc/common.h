@@ -181,7 +181,31 @@ __device__ __forceinline__ cx<R> cdot3(cx<R> u0, cx<R> u1, cx<R> u2, cx<R> h0, c
   r.im = fma(u0.im, h0.re, r.im);
   return cmac(cmac(r, u1, h1), u2, h2);
 }
+// The same with the first product of each chain taken as fma(a, b, nz):
+// with nz = -0.0 that is exactly the product (signed zeros included), and
+// when nz is opaque to the compiler the product stays a fused multiply-add,
+// which can read an FP16 operand without a separate conversion.
+template <typename R>
+__device__ __forceinline__ cx<R> cdot3(cx<R> u0, cx<R> u1, cx<R> u2, cx<R> h0, cx<R> h1, cx<R> h2, R nz)
+{
+  cx<R> r = {fma(u0.re, h0.re, nz), fma(u0.re, h0.im, nz)};
+  r.re = fma(-u0.im, h0.im, r.re);
+  r.im = fma(u0.im, h0.re, r.im);
+  return cmac(cmac(r, u1, h1), u2, h2);
+}
 // conj(p q - r s) (the third row of an SU(3) matrix from its first two)
+template <typename R> __device__ __forceinline__ cx<R> conj_diff(cx<R> p, cx<R> q, cx<R> r, cx<R> s, R nz)
+{
+  R re = fma(p.re, q.re, nz);
+  re = fma(-p.im, q.im, re);
+  re = fma(-r.re, s.re, re);
+  re = fma(r.im, s.im, re);
+  R im = fma(-p.re, q.im, nz);
+  im = fma(-p.im, q.re, im);
+  im = fma(r.re, s.im, im);
+  im = fma(r.im, s.re, im);
+  return {re, im};
+}
 template <typename R> __device__ __forceinline__ cx<R> conj_diff(cx<R> p, cx<R> q, cx<R> r, cx<R> s)
 {
   R re = p.re * q.re;

@@ -160,10 +160,11 @@ __device__ __forceinline__ void reconstruct(cx<R> acc[4][3], const cx<R> w[2][3]
 // rows are the reference's stored rows (it stores the transpose of each link,
 // codegen/dslash_common.cc:1678-1727): rows 0,1 stored, row 2 stored (18) or
 // rebuilt as conj(row0 x row1) (12; codegen/dslash_common.cc:1664-1674).
-// Returns the applied matrices: ub = X_back, uf = X_fwd^T.
+// Returns the applied matrices: ub = X_back, uf = X_fwd^T.  FP16 links: nz
+// (an opaque -0.0) starts the product chains of the rebuilt row.
 template <typename S, int RECON, int MU, typename R>
 __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site, cx<R> ub[3][3],
-                                               cx<R> uf[3][3])
+                                               cx<R> uf[3][3], R nz = R(-0.0))
 {
   constexpr int WL = gauge_w<S, RECON>();
   R lk[2 * RECON];
@@ -180,7 +181,10 @@ __device__ __forceinline__ void load_link_pair(const S *gauge, int vcb, int site
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-        x[2][c] = conj_diff(x[0][c1], x[1][c2], x[0][c2], x[1][c1]);
+        if constexpr (std::is_same<S, __half>::value)
+          x[2][c] = conj_diff(x[0][c1], x[1][c2], x[0][c2], x[1][c1], nz);
+        else
+          x[2][c] = conj_diff(x[0][c1], x[1][c2], x[0][c2], x[1][c1]);
       }
     }
 #pragma unroll
@@ -301,7 +305,12 @@ __device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const NbrSpinor<S, R>
 #pragma unroll
   for (int k = 0; k < 2; k++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) w[k][i] = cdot3(u[i][0], u[i][1], u[i][2], h[k][0], h[k][1], h[k][2]);
+    for (int i = 0; i < 3; i++) {
+      if constexpr (std::is_same<S, __half>::value)  // FP16 links read by the FMAs
+        w[k][i] = cdot3(u[i][0], u[i][1], u[i][2], h[k][0], h[k][1], h[k][2], -R(0) * one);
+      else
+        w[k][i] = cdot3(u[i][0], u[i][1], u[i][2], h[k][0], h[k][1], h[k][2]);
+    }
   if (tbc) {
 #pragma unroll
     for (int k = 0; k < 2; k++)
@@ -317,7 +326,7 @@ __device__ __forceinline__ void hop_mu(cx<R> acc[4][3], const S *__restrict__ in
                                        int nf, R bb, R bf, bool tbc, cx<R> phase, R one)
 {
   cx<R> ub[3][3], uf[3][3];
-  load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf);
+  load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf, -R(0) * one);
   NbrSpinor<S, R> n;
   load_nbr(in, vcb, nb, n);
   hop_apply<S, MU, 0, ISIGN>(acc, n, ub, bb, tbc, conj(phase), one);
@@ -332,7 +341,7 @@ __device__ __forceinline__ void hop_mu_lds(cx<R> acc[4][3], const Vec16 *lds, co
                                            cx<R> phase, R one)
 {
   cx<R> ub[3][3], uf[3][3];
-  load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf);
+  load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf, -R(0) * one);
   NbrSpinor<S, R> n;
   load_nbr_lds<S>(lds, jb, n);
   hop_apply<S, MU, 0, ISIGN>(acc, n, ub, bb, tbc, conj(phase), one);
