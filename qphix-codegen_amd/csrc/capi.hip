@@ -95,6 +95,9 @@ int qphix_amd_ctx_create(qphix_amd_ctx **ctx, int Lx, int Ly, int Lz, int Lt, do
   c->g.sy = Lx / 2;
   c->g.sz = Lx / 2 * Ly;
   c->g.st = Lx / 2 * Ly * Lz;
+  c->g.fxh = qamd::make_fastdiv((unsigned)(Lx / 2));
+  c->g.fy = qamd::make_fastdiv((unsigned)Ly);
+  c->g.fz = qamd::make_fastdiv((unsigned)Lz);
   c->tbound = t_boundary;
   c->comm_mask = 0;
   c->traversal = -1;

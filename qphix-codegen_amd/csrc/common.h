@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "../../include/qphix_amd.h"
+#include "fastdiv.h"
 
 namespace qamd {
 
@@ -225,6 +226,7 @@ struct Geom {
   int Lx, Ly, Lz, Lt, Lxh;
   int vcb;         // sites per checkerboard
   int sy, sz, st;  // cb-index strides of y, z, t
+  FastDiv fxh, fy, fz;  // division by Lxh, Ly, Lz
 };
 
 struct SiteCoord {
@@ -236,13 +238,13 @@ __device__ __forceinline__ SiteCoord decode(const Geom &g, int i, int cb)
 {
   SiteCoord c;
   unsigned r = (unsigned)i;
-  unsigned q = r / (unsigned)g.Lxh;
+  unsigned q = fdiv(r, g.fxh);
   c.xh = (int)(r - q * (unsigned)g.Lxh);
   r = q;
-  q = r / (unsigned)g.Ly;
+  q = fdiv(r, g.fy);
   c.y = (int)(r - q * (unsigned)g.Ly);
   r = q;
-  q = r / (unsigned)g.Lz;
+  q = fdiv(r, g.fz);
   c.z = (int)(r - q * (unsigned)g.Lz);
   c.t = (int)q;
   c.x = 2 * c.xh + ((cb + c.y + c.z + c.t) & 1);

@@ -51,6 +51,9 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
     p.ts = ctx->g.Lt / kNumXcd;
     p.zs = ctx->g.Lz / QAMD_ZPARTS;
   }
+  p.f_per = make_fastdiv((unsigned)(p.zs * ctx->g.Lt * p.bpp));
+  p.f_zb = make_fastdiv((unsigned)(p.zs * p.bpp));
+  p.f_bpp = make_fastdiv((unsigned)p.bpp);
   p.coeff_s = (R)c.coeff_s;
   p.coeff_tf = (R)c.coeff_tf;
   p.coeff_tb = (R)c.coeff_tb;

@@ -34,6 +34,7 @@ template <typename R> struct DslashParams {
                 // 1 z-slabs per XCD visited t-major, 2 t-slabs visited z-major,
                 // 3 t-slabs visited one z-part at a time
   int zs, ts, bpp;  // z-planes / t-slices per XCD slab, work-groups per (z,t) plane
+  FastDiv f_per, f_zb, f_bpp;  // order 1: division by zs*Lt*bpp, zs*bpp and bpp
   int lds;          // stage the x/y neighbour rows in LDS (whole rows per work-group)
   R coeff_s, coeff_tf, coeff_tb, tbound;
   int comm;  // bit mu: dimension mu is split; hops across its wrap-around are dropped
@@ -454,11 +455,11 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   if (p.order == 1) {
     // XCD k owns z-planes [k zs, (k+1) zs) for every t, visited t-major, so
     // t-neighbours are zs*bpp work-groups apart and z-neighbours bpp apart
-    const int per = p.zs * g.Lt * p.bpp;
-    const int k = blk / per, r = blk - k * per;
-    const int t = r / (p.zs * p.bpp), r2 = r - t * (p.zs * p.bpp);
-    const int z = k * p.zs + r2 / p.bpp, h = r2 % p.bpp;
-    idx = (t * g.Lz + z) * g.sz + h * kDslashBlock + (int)threadIdx.x;
+    const unsigned k = fdiv((unsigned)blk, p.f_per), r = (unsigned)blk - k * p.f_per.d;
+    const unsigned t = fdiv(r, p.f_zb), r2 = r - t * p.f_zb.d;
+    const unsigned zl = fdiv(r2, p.f_bpp), h = r2 - zl * p.f_bpp.d;
+    const int z = (int)k * p.zs + (int)zl;
+    idx = ((int)t * g.Lz + z) * g.sz + (int)h * kDslashBlock + (int)threadIdx.x;
   } else if (p.order == 2) {
     // XCD k owns t-slices [k ts, (k+1) ts), visited z-major: t-neighbours
     // are bpp work-groups apart, z-neighbours ts*bpp
@@ -531,10 +532,10 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   // the array.
   int y0 = 0, r1 = 0;
   if (p.lds) {
-    const int base = idx - (int)threadIdx.x;
-    const int r0 = base / g.Lxh;  // rows numbered over the checkerboard
-    r1 = (base + kDslashBlock - 1) / g.Lxh - r0;
-    y0 = r0 % g.Ly;
+    const unsigned base = (unsigned)(idx - (int)threadIdx.x);
+    const unsigned r0 = fdiv(base, g.fxh);  // rows numbered over the checkerboard
+    r1 = (int)(fdiv(base + kDslashBlock - 1, g.fxh) - r0);
+    y0 = (int)frem(r0, g.fy);
   }
   if (p.lds && y0 + r1 < g.Ly) {
     __shared__ Vec16 lds[(24 * sizeof(S) / 16) * kLdsSites];
@@ -543,8 +544,11 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     const Vec16 *vin = reinterpret_cast<const Vec16 *>(in);
     constexpr int NC = 24 * (int)sizeof(S) / 16;
     for (int j = (int)threadIdx.x; j < nst; j += kDslashBlock) {
-      const int k = j / g.Lxh;
-      const int src = plane + ((y0 + k - 1 + g.Ly) % g.Ly) * g.Lxh + (j - k * g.Lxh);
+      const int k = (int)fdiv((unsigned)j, g.fxh);
+      // staged row k is y0 + k - 1, in [-1, Ly] since k <= r1 + 2 and y0 + r1 < Ly
+      int yk = y0 + k - 1;
+      yk = yk < 0 ? yk + g.Ly : (yk >= g.Ly ? yk - g.Ly : yk);
+      const int src = plane + yk * g.Lxh + (j - k * g.Lxh);
 #pragma unroll
       for (int ch = 0; ch < NC; ch++) lds[ch * kLdsSites + j] = vin[(unsigned)(ch * vcb + src)];
     }
