@@ -127,7 +127,9 @@ __global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *__restrict__ x, 
   }
 }
 
-// CG step: r -= alpha Ap; x += alpha p; partial |r|^2.  alpha = sc->alpha.
+// CG step where the operator stored Ap (lattices whose checkerboard is not
+// a whole number of operator work-groups): r -= alpha Ap; x += alpha p;
+// partial |r|^2.  alpha = sc->alpha.
 template <typename S, typename SXS>
 __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *__restrict__ r, SXS *__restrict__ x,
                                                           const S *__restrict__ p, const S *__restrict__ Ap,
