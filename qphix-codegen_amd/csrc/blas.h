@@ -47,7 +47,8 @@ hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const d
 hipError_t run_clover_blocks(const qphix_amd_ctx *ctx, double *out, const double *clov, int mode,
                              double mu, int sign, hipStream_t st);
 // Accumulates scale * link gradient of Re sum_x w(x) chi^dag A psi (lex FP64
-// fields; w = w_even / w_odd by parity); K: workspace of 6*18*V doubles.
+// fields; w = w_even / w_odd by parity); K: workspace of 6*18*V doubles
+// (component-major, K[(plane * 18 + e) * V + site]).
 hipError_t run_clover_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *chi,
                             const double *psi, const double *U, double csw, double w_even, double w_odd,
                             double scale, int project, hipStream_t st);

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(128) void clover_weights_k(double *K, const double 
     }
     const cx<double> tr = {(L[0][0].re + L[1][1].re + L[2][2].re) / 3.0,
                            (L[0][0].im + L[1][1].im + L[2][2].im) / 3.0};
-    double *o = K + (s * 6 + p) * 18;
+    double *o = K + (long)p * 18 * V + s;  // component e at o[e * V]
     const double f = 0.5 * csw * w / 8.0;
 #pragma unroll
     for (int j = 0; j < 3; j++)
@@ -394,20 +394,23 @@ __global__ __launch_bounds__(128) void clover_weights_k(double *K, const double 
         // H = L' + L'^dag, K = f H / i = -i f H
         cx<double> h = L[j][i] + conj(L[i][j]);
         if (i == j) h = {h.re - 2.0 * tr.re, 0.0};
-        o[(j * 3 + i) * 2] = f * h.im;
-        o[(j * 3 + i) * 2 + 1] = -f * h.re;
+        o[((j * 3 + i) * 2) * V] = f * h.im;
+        o[((j * 3 + i) * 2 + 1) * V] = -f * h.re;
       }
   }
 }
 
-__device__ __forceinline__ M3 ld_k(const double *K, long s, int p)
+// K is stored component-major, K[(p * 18 + e) * V + s] (e = 0..17 the
+// reals of the 3x3 matrix of plane p at site s), so a wave's loads of one
+// component are contiguous.
+__device__ __forceinline__ M3 ld_k(const double *K, long V, long s, int p)
 {
   M3 r;
-  const double *q = K + (s * 6 + p) * 18;
+  const double *q = K + (long)p * 18 * V + s;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) r.m[i][j] = {q[(i * 3 + j) * 2], q[(i * 3 + j) * 2 + 1]};
+    for (int j = 0; j < 3; j++) r.m[i][j] = {q[((i * 3 + j) * 2) * V], q[((i * 3 + j) * 2 + 1) * V]};
   return r;
 }
 
@@ -478,20 +481,20 @@ __global__ __launch_bounds__(128) void clover_links_k(double *force, const doubl
           const long yb = shifted(g, c, a, 0, b, 1);
           const M3 U2 = ld_link(U, ya, b), U3 = dagm(ld_link(U, yb, a)), U4 = dagm(ld_link(U, y, b));
           const M3 U34 = mul(U3, U4), U23 = mul(U2, U3), U234 = mul(U2, U34);
-          acc(G, mul(U234, ld_k(K, y, p)));
-          acc(G, mul(ld_k(K, ya, p), U234));
-          acc(G, mul(mul(U2, ld_k(K, yab, p)), U34));
-          acc(G, mul(mul(U23, ld_k(K, yb, p)), U4));
+          acc(G, mul(U234, ld_k(K, V, y, p)));
+          acc(G, mul(ld_k(K, V, ya, p), U234));
+          acc(G, mul(mul(U2, ld_k(K, V, yab, p)), U34));
+          acc(G, mul(mul(U23, ld_k(K, V, yb, p)), U4));
         }
         {  // square based at y - b: the link is its third factor, daggered
           const long ymb = shifted(g, c, a, 0, b, -1), ymba = shifted(g, c, a, 1, b, -1);
           const long ya = shifted(g, c, a, 1, b, 0);
           const M3 V1 = ld_link(U, ymb, a), V2 = ld_link(U, ymba, b), V4 = dagm(ld_link(U, ymb, b));
           const M3 V41 = mul(V4, V1), V12 = mul(V1, V2), V412 = mul(V41, V2);
-          M3 M = mul(mul(V4, ld_k(K, ymb, p)), V12);
-          acc(M, mul(mul(V41, ld_k(K, ymba, p)), V2));
-          acc(M, mul(V412, ld_k(K, ya, p)));
-          acc(M, mul(ld_k(K, y, p), V412));
+          M3 M = mul(mul(V4, ld_k(K, V, ymb, p)), V12);
+          acc(M, mul(mul(V41, ld_k(K, V, ymba, p)), V2));
+          acc(M, mul(V412, ld_k(K, V, ya, p)));
+          acc(M, mul(ld_k(K, V, y, p), V412));
           acc_dag(G, M);
         }
       } else {
@@ -500,20 +503,20 @@ __global__ __launch_bounds__(128) void clover_links_k(double *force, const doubl
           const long yb = shifted(g, c, a, 0, b, 1);
           const M3 W1 = ld_link(U, yma, a), W3 = dagm(ld_link(U, ymab, a)), W4 = dagm(ld_link(U, yma, b));
           const M3 W34 = mul(W3, W4), W41 = mul(W4, W1), W341 = mul(W34, W1);
-          acc(G, mul(mul(W34, ld_k(K, yma, p)), W1));
-          acc(G, mul(W341, ld_k(K, y, p)));
-          acc(G, mul(ld_k(K, yb, p), W341));
-          acc(G, mul(mul(W3, ld_k(K, ymab, p)), W41));
+          acc(G, mul(mul(W34, ld_k(K, V, yma, p)), W1));
+          acc(G, mul(W341, ld_k(K, V, y, p)));
+          acc(G, mul(ld_k(K, V, yb, p), W341));
+          acc(G, mul(mul(W3, ld_k(K, V, ymab, p)), W41));
         }
         {  // square based at y: the link is its fourth factor, daggered
           const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
           const long yb = shifted(g, c, a, 0, b, 1);
           const M3 U1 = ld_link(U, y, a), U2 = ld_link(U, ya, b), U3 = dagm(ld_link(U, yb, a));
           const M3 U12 = mul(U1, U2), U23 = mul(U2, U3), U123 = mul(U12, U3);
-          M3 M = mul(ld_k(K, y, p), U123);
-          acc(M, mul(mul(U1, ld_k(K, ya, p)), U23));
-          acc(M, mul(mul(U12, ld_k(K, yab, p)), U3));
-          acc(M, mul(U123, ld_k(K, yb, p)));
+          M3 M = mul(ld_k(K, V, y, p), U123);
+          acc(M, mul(mul(U1, ld_k(K, V, ya, p)), U23));
+          acc(M, mul(mul(U12, ld_k(K, V, yab, p)), U3));
+          acc(M, mul(U123, ld_k(K, V, yb, p)));
           acc_dag(G, M);
         }
       }
