@@ -45,6 +45,16 @@ def test_ctx_validation(qamd):
     assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 3, 4, 4, 4, -1.0) == -2
     assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 4, 4, 4, 5, -1.0) == -2
     assert lib.qphix_amd_ctx_create(None, 4, 4, 4, 4, -1.0) == -1
+    # every field index must fit 32 bits: 72 reals x sites per checkerboard < 2^31
+    assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 128, 128, 128, 128, -1.0) == -2
+    assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 96, 96, 96, 96, -1.0) == -2
+    big = ctypes.c_void_p()  # 64^4: 72 x 8,388,608 < 2^31
+    assert lib.qphix_amd_ctx_create(ctypes.byref(big), 64, 64, 64, 64, -1.0) == 0
+    lib.qphix_amd_ctx_destroy(big)
+    small = ctypes.c_void_p()
+    assert lib.qphix_amd_ctx_create(ctypes.byref(small), 2, 2, 2, 2, -1.0) == 0  # smallest lattice
+    assert lib.qphix_amd_sites_cb(small) == 8
+    lib.qphix_amd_ctx_destroy(small)
     assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 8, 8, 8, 8, -1.0) == 0
     try:
         assert lib.qphix_amd_sites_cb(ctx) == 2048
