@@ -215,9 +215,10 @@ def test_traversal_orders_bit_identical(qamd, dims):
     geom.set_traversal(-1)
 
 
-def test_small_lattice_4x4(qamd, orc):
-    """Smallest extents (2 sites in x per checkerboard row), FP64 clover."""
-    dims = (4, 2, 4, 2)
+@pytest.mark.parametrize("dims", [(4, 2, 4, 2), (2, 2, 2, 2), (2, 4, 2, 6)])
+def test_small_lattice_4x4(qamd, orc, dims):
+    """Smallest extents, FP64 clover: 2 in y and t; Lx = 2 (one site per
+    checkerboard row, both x neighbours the same site); the 2^4 lattice."""
     lat = make_lattice(qamd, dims, 77)
     geom = qamd.Geometry(dims, t_boundary=-1.0)
     for isign in (1, -1):
@@ -225,7 +226,7 @@ def test_small_lattice_4x4(qamd, orc):
             out = run_gpu(qamd, geom, 1, True, 0, 12, isign, cb, lat)
             ref = run_oracle(orc, dims, 1, True, isign, cb, lat)
             m = cb_mask(dims, cb)
-            assert np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max() < 1e-13
+            assert np.abs(out[m] - ref[m]).max() / np.abs(ref[m]).max() < 1e-13, (dims, isign, cb)
 
 
 def test_bad_arguments_raise(qamd):
