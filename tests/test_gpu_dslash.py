@@ -267,6 +267,31 @@ def test_parity_32x64_sampled_sites(qamd, orc, lat32):
             assert err < 1e-6, (kind, achi, isign, err)
 
 
+def test_parity_48_slices_sampled_sites(qamd, orc):
+    """48x48x48x16: t-slices of 55,296 sites per checkerboard as at 48^3x96,
+    so the automatic traversal has the XCDs sweep t together over z-ranges
+    (order 1), with the x/y rows staged in LDS.  Twisted-mass clover dslash
+    and achimbdpsi and the clover dslash, FP32 and FP16 storage, against the
+    FP64 oracle at 4096 sampled sites of the output checkerboard."""
+    dims = (48, 48, 48, 16)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    assert geom.traversal == 1
+    lat = make_lattice(qamd, dims, 1500, device=DEV)
+    cpu = {k: v.cpu() for k, v in lat.items()}
+    rng = np.random.default_rng(5)
+    par = parity(dims)
+    for prec, tol in ((1, 1e-6), (2, 5e-3)):
+        for kind, achi, isign in ((3, False, 1), (3, True, 1), (3, False, -1), (1, False, 1)):
+            cb = 0 if not achi else 1
+            print(f"case 48^3x16 prec={prec} kind={kind} achi={achi} isign={isign}", flush=True)
+            out = run_gpu(qamd, geom, kind, achi, prec, 12, isign, cb, lat)
+            sites = rng.choice(np.nonzero(par == cb)[0], 4096, replace=False)
+            kw = oracle_args(kind, achi, isign, cpu, dims)
+            ref = orc.dslash_sites(dims, sites, cpu["psi"].numpy(), cpu["U"].numpy(), isign, **kw)
+            err = np.abs(out[sites] - ref).max() / np.abs(ref).max()
+            assert err < tol, (prec, kind, achi, isign, err)
+
+
 def test_recon12_equals_recon18_32x64(qamd, lat32):
     dims = (32, 32, 32, 64)
     geom = qamd.Geometry(dims, t_boundary=-1.0)
