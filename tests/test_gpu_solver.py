@@ -9,7 +9,8 @@
   and mixed vs FP64 solution agreement;
 * repeated solves bit-identical; the CG on a lattice whose checkerboard is
   not a whole number of 128-site work-groups (separate dot product instead
-  of the norm fused into the operator kernel).
+  of the norm fused into the operator kernel); the CG at 16^4, where the
+  operator kernels run traversal order 1.
 """
 import numpy as np
 import pytest
@@ -119,6 +120,34 @@ def test_cg_fp64_and_mixed(qamd, orc, lat, mu):
     for prec_s, xs in sols.items():
         d = np.linalg.norm(xs - ref) / np.linalg.norm(ref)
         assert d < 1e-8, (prec_s, d)
+
+
+def test_cg_traversal_order1_16x4(qamd, orc):
+    """16^4: Lt <= Lz, so the operator kernels run with the XCDs sweeping t
+    together over z-ranges (order 1) and the x/y rows staged in LDS, and the
+    CG takes <p, M^dag M p> and the residual update from the operator's
+    fused epilogues.  Twisted-mass clover, FP64 and FP16/BF16 mixed, to a
+    true residual of 1e-10 recomputed by the oracle."""
+    dims = (16, 16, 16, 16)
+    L = qamd.lattice
+    U = L.random_gauge(dims, 0.2, 2200)
+    b = L.random_spinor(dims, 2201)
+    geom = qamd.Geometry(dims)
+    assert geom.traversal == 1
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    An = A.cpu().numpy()
+    bn = on_cb(b.numpy(), dims, 1)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+    for prec_s in (qamd.F64, qamd.F16, qamd.BF16):
+        op = qamd.EvenOddCloverOperator(geom, Ud, A, mu=0.05, prec_sloppy=prec_s, recon_sloppy=12)
+        x = geom.new_spinor(qamd.F64, cb=1)
+        res = qamd.InvCG(op, max_iters=5000)(x, bf, 1e-10, mixed=prec_s != qamd.F64)
+        print(f"16^4 order 1 sloppy={qamd.PREC_NAME[prec_s]}: {res}", flush=True)
+        xn = geom.unpack_spinor(x).cpu().numpy()
+        r = _true_residual(orc, dims, U.numpy(), An, xn, bn, 0.05)
+        assert res["rsd_final"] < 1e-10 and r < 1.5e-10, (prec_s, res["rsd_final"], r)
+        op.close()
 
 
 def test_cg_repeatable(qamd, lat):
