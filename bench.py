@@ -64,6 +64,25 @@ def _host_isa():
     return None
 
 
+def host_triad_gbs(nthreads, n=1 << 26):
+    """STREAM-style triad c = a + s b on the host (torch's CPU kernels on
+    nthreads threads, 3 x 256 MB FP32 arrays, best of 5): 12 bytes per
+    element counted."""
+    import torch
+    prev = torch.get_num_threads()
+    torch.set_num_threads(nthreads)
+    try:
+        a, b, c = torch.rand(n), torch.rand(n), torch.empty(n)
+        best = float("inf")
+        for _ in range(5):
+            t0 = time.perf_counter()
+            torch.add(a, b, alpha=1.5, out=c)
+            best = min(best, time.perf_counter() - t0)
+        return 12.0 * n / best / 1e9
+    finally:
+        torch.set_num_threads(prev)
+
+
 def cpu_baseline(seconds=10.0):
     """The reference's own generated FP32 clover kernel (oracle/_ref, built
     from the qphix-codegen sources) under an OpenMP block loop on the host
@@ -86,8 +105,11 @@ def cpu_baseline(seconds=10.0):
         _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=3)
         reps = max(3, int(seconds / max(sec, 1e-6)))
         _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=reps)
+        triad = host_triad_gbs(nthr)
+        kernel_gbs = vcb * 864 / sec / 1e9  # algorithmic bytes per site, FP32 12-real links
         return {"value": round(vcb * FLOP_CLOVER / sec / 1e9, 2), "unit": "GFLOP/s", "cores": nthr,
-                "kind": "reference",
+                "kind": "reference", "host_triad_gbs": round(triad, 1),
+                "host_roofline_frac": round(kernel_gbs / triad, 3),
                 "sample": f"reference {cfg} clov_dslash (VECLEN=SOALEN={ref.soalen(cfg)}, 12-real links) "
                           f"on 32^3x8 ({vcb} sites/cb), {reps} calls x {sec * 1e3:.2f} ms, "
                           f"OpenMP {nthr} threads"}
