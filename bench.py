@@ -86,20 +86,27 @@ def host_triad_gbs(nthreads, n=1 << 26):
 def cpu_baseline(seconds=10.0):
     """The reference's own generated FP32 clover kernel (oracle/_ref, built
     from the qphix-codegen sources) under an OpenMP block loop on the host
-    cores, on a bounded slab of the workload: 32^3 x 8 (same per-site work as
-    32^3 x 64).  Falls back to the FP64 C oracle (one thread) if the
-    reference build is absent."""
+    cores, on the whole 32^3 x 64 lattice (about 0.9 GB of fields per call,
+    so the kernel streams from host memory rather than from caches), timed
+    over about `seconds` of calls.  The inputs are generated on the GPU when
+    there is one (generation is not timed).  Falls back to the FP64 C oracle
+    (one thread, a bounded sample of sites) if the reference build is
+    absent."""
     import numpy as np
+    import torch
     from oracle import ref
     import __graft_entry__ as ge
     q = ge.load_package()
     L = q.lattice
     cfg = _host_isa()
     if cfg and ref.available(cfg):
-        dims = (32, 32, 32, 8)
-        U = L.random_gauge(dims, 0.3, 5).numpy()
-        psi = L.random_spinor(dims, 6).numpy()
-        inv = L.invert_clover(L.random_clover(dims, 7)).numpy()
+        dims = (32, 32, 32, 64)
+        gen = "cuda:0" if torch.cuda.is_available() else "cpu"
+        U = L.random_gauge(dims, 0.3, 5, device=gen).cpu().numpy()
+        psi = L.random_spinor(dims, 6, device=gen).cpu().numpy()
+        inv = L.invert_clover(L.random_clover(dims, 7, device=gen)).cpu().numpy()
+        if gen != "cpu":
+            torch.cuda.empty_cache()
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
         vcb = U.shape[0] // 2
         _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=3)
@@ -111,7 +118,7 @@ def cpu_baseline(seconds=10.0):
                 "kind": "reference", "host_triad_gbs": round(triad, 1),
                 "host_roofline_frac": round(kernel_gbs / triad, 3),
                 "sample": f"reference {cfg} clov_dslash (VECLEN=SOALEN={ref.soalen(cfg)}, 12-real links) "
-                          f"on 32^3x8 ({vcb} sites/cb), {reps} calls x {sec * 1e3:.2f} ms, "
+                          f"on 32^3x64 ({vcb} sites/cb), {reps} calls x {sec * 1e3:.2f} ms, "
                           f"OpenMP {nthr} threads"}
     from oracle import oracle as orc
     dims = (16, 16, 16, 32)

@@ -271,8 +271,8 @@ def test_parity_48_slices_sampled_sites(qamd, orc):
     """48x48x48x16: t-slices of 55,296 sites per checkerboard as at 48^3x96,
     so the automatic traversal has the XCDs sweep t together over z-ranges
     (order 1), with the x/y rows staged in LDS.  Twisted-mass clover dslash
-    and achimbdpsi and the clover dslash, FP32 and FP16 storage, against the
-    FP64 oracle at 4096 sampled sites of the output checkerboard."""
+    and achimbdpsi and the clover dslash, FP32, FP16 and BF16 storage,
+    against the FP64 oracle at 4096 sampled sites of the output checkerboard."""
     dims = (48, 48, 48, 16)
     geom = qamd.Geometry(dims, t_boundary=-1.0)
     assert geom.traversal == 1
@@ -280,7 +280,7 @@ def test_parity_48_slices_sampled_sites(qamd, orc):
     cpu = {k: v.cpu() for k, v in lat.items()}
     rng = np.random.default_rng(5)
     par = parity(dims)
-    for prec, tol in ((1, 1e-6), (2, 5e-3)):
+    for prec, tol in ((1, TOL[1]), (2, TOL[2]), (3, TOL[3])):
         for kind, achi, isign in ((3, False, 1), (3, True, 1), (3, False, -1), (1, False, 1)):
             cb = 0 if not achi else 1
             print(f"case 48^3x16 prec={prec} kind={kind} achi={achi} isign={isign}", flush=True)
