@@ -206,6 +206,19 @@ def copy_bandwidth(dev, nbytes=1 << 30, reps=10):
     return round(2 * nbytes / t / 1e9, 1)
 
 
+def triad_bandwidth(dev, nbytes=1 << 30, reps=10):
+    """STREAM triad c = a + s b on the device (torch's vectorized elementwise
+    kernel, three 1 GiB arrays, 2 reads : 1 write), GB/s."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a).fill_(2.0)
+    c = torch.empty_like(a)
+    t = time_kernel(lambda i: torch.add(a, b, alpha=1.5, out=c), reps)
+    del a, b, c
+    torch.cuda.empty_cache()
+    return round(3 * nbytes / t / 1e9, 1)
+
+
 def sloppy_kernels(q, dev):
     """The CG's sloppy operator kernels (FP16 and BF16 storage, FP32
     arithmetic, 12-real links) at 32^3x64, HIP-event averages."""
@@ -520,6 +533,7 @@ def main():
     del sets
     torch.cuda.empty_cache()
     line["roofline"]["measured_copy_gbs"] = copy_bandwidth(dev)
+    line["roofline"]["measured_triad_gbs"] = triad_bandwidth(dev)
     if not args.no_cg:
         line["cg"] = cg_compare(q, dev)
         line["cg"]["sloppy_kernels"] = sloppy_kernels(q, dev)
