@@ -401,7 +401,10 @@ int qphix_amd_tmclover_blocks(qphix_amd_ctx *ctx, double *out_lex, const double 
    M = F_oo - 1/4 D_oe F_ee^-1 D_eo on the odd checkerboard (cb 1),
    F = A + i mu g5 (mu = 0: Wilson-clover with packed Hermitian blocks).
    The operator keeps an FP64 copy (12-real links) and a sloppy copy in
-   prec_sloppy / recon_sloppy storage; it allocates device memory.       */
+   prec_sloppy / recon_sloppy storage; it allocates device memory.  The
+   force and CG calls on an operator keep their workspaces in it (grow-only,
+   released by qphix_amd_eo_op_destroy), so repeated calls allocate
+   nothing; calls on one operator must not overlap.                      */
 typedef struct qphix_amd_eo_op qphix_amd_eo_op;
 int qphix_amd_eo_op_create(qphix_amd_eo_op **op, qphix_amd_ctx *ctx, const double *u_lex,
                            const double *clov_lex, double mu, int prec_sloppy,

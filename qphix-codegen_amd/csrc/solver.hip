@@ -42,7 +42,21 @@ struct DevBuf {
   hipError_t alloc(size_t bytes)
   {
     n = bytes;
-    return hipMalloc(&p, bytes);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+    }
+    return e;
+  }
+  // at least `bytes`, keeping the current allocation when it is large enough
+  hipError_t reserve(size_t bytes)
+  {
+    if (p && n >= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    return alloc(bytes);
   }
   ~DevBuf()
   {
@@ -65,6 +79,19 @@ struct qphix_amd_eo_op {
   DevBuf gauge[2][2];
   DevBuf dsl[2][2], achi[2][2];
   DevBuf tmp[2];  // cb-0 intermediate per level
+  // Workspaces of the force and CG calls, kept from one call to the next
+  // (grow-only) and released by qphix_amd_eo_op_destroy, so that repeated
+  // solves and forces allocate nothing: three FP64 vectors (the forces' Y,
+  // Z, W and the CG's FP64 residual, M^dag M x and cb-1 intermediate), the
+  // clover force's lexicographic fields and K, the CG's sloppy vectors,
+  // scalars and partial sums, and a pinned block for the scalars.  Calls on
+  // one operator do not overlap, so they share these.
+  DevBuf w64[3], wchi, wpsi, wk, wrs, wp, wap, wxs, wscb, wpart, wpnorm;
+  qamd::CGScalars *hsc = nullptr;
+  ~qphix_amd_eo_op()
+  {
+    if (hsc) (void)hipHostFree(hsc);
+  }
 };
 
 namespace {
@@ -269,8 +296,8 @@ int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
   if (!op || !force_lex || !x || !u_lex) return QPHIX_AMD_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const long n = 24L * op->ctx->g.vcb;
-  DevBuf y, z, w;
-  if (y.alloc(n * 8) != hipSuccess || z.alloc(n * 8) != hipSuccess || w.alloc(n * 8) != hipSuccess)
+  DevBuf &y = op->w64[0], &z = op->w64[1], &w = op->w64[2];
+  if (y.reserve(n * 8) != hipSuccess || z.reserve(n * 8) != hipSuccess || w.reserve(n * 8) != hipSuccess)
     return QPHIX_AMD_ENOMEM;
   // Y = M X, Z = F_ee^-1 D_eo X, W = (F_ee^-1)^dag D_oe^dag Y
   hipError_t e = eo_apply(op, 0, 0, y.p, x, st);
@@ -310,10 +337,11 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
   if (!op || !force_lex || !x || !u_lex) return QPHIX_AMD_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const long n = 24L * op->ctx->g.vcb;
-  DevBuf y, z, w, chi, psi, K;
-  if (y.alloc(n * 8) != hipSuccess || z.alloc(n * 8) != hipSuccess || w.alloc(n * 8) != hipSuccess ||
-      chi.alloc(2 * n * 8) != hipSuccess || psi.alloc(2 * n * 8) != hipSuccess ||
-      K.alloc(2L * op->ctx->g.vcb * 108 * 8) != hipSuccess)
+  DevBuf &y = op->w64[0], &z = op->w64[1], &w = op->w64[2];
+  DevBuf &chi = op->wchi, &psi = op->wpsi, &K = op->wk;
+  if (y.reserve(n * 8) != hipSuccess || z.reserve(n * 8) != hipSuccess || w.reserve(n * 8) != hipSuccess ||
+      chi.reserve(2 * n * 8) != hipSuccess || psi.reserve(2 * n * 8) != hipSuccess ||
+      K.reserve(2L * op->ctx->g.vcb * 108 * 8) != hipSuccess)
     return QPHIX_AMD_ENOMEM;
   // Y = M X, Z = F_ee^-1 D_eo X, W = (F_ee^-1)^dag D_oe^dag Y
   hipError_t e = eo_apply(op, 0, 0, y.p, x, st);
@@ -382,23 +410,23 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   // fused |M p|^2 needs every work-group of the operator kernels full
   const bool fuse = QAMD_CG_FUSED_NORM && vcb % kDslashBlock == 0;
   const long nwg = vcb / kDslashBlock;  // (vcb < 2^31 / 72: nwg / 2048 < kMaxPartials)
-  DevBuf r64, ax64, rs, p, ap, xs, mid, scb, part, pnorm, host;
-  if (fuse && pnorm.alloc(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
-  if (r64.alloc(n * 8) != hipSuccess || ax64.alloc(n * 8) != hipSuccess ||
-      rs.alloc(n * sbytes(ps)) != hipSuccess || p.alloc(n * sbytes(ps)) != hipSuccess ||
-      (!fuse && ap.alloc(n * sbytes(ps)) != hipSuccess) || xs.alloc(n * sbytes(px)) != hipSuccess ||
-      mid.alloc(n * (size_t)sbytes(QPHIX_AMD_F64)) != hipSuccess ||
-      scb.alloc(sizeof(CGScalars)) != hipSuccess || part.alloc(kMaxPartials * 8) != hipSuccess)
+  DevBuf &r64 = op->w64[0], &ax64 = op->w64[1], &mid = op->w64[2];
+  DevBuf &rs = op->wrs, &p = op->wp, &ap = op->wap, &xs = op->wxs;
+  DevBuf &scb = op->wscb, &part = op->wpart, &pnorm = op->wpnorm;
+  if (fuse && pnorm.reserve(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
+  if (r64.reserve(n * 8) != hipSuccess || ax64.reserve(n * 8) != hipSuccess ||
+      rs.reserve(n * sbytes(ps)) != hipSuccess || p.reserve(n * sbytes(ps)) != hipSuccess ||
+      (!fuse && ap.reserve(n * sbytes(ps)) != hipSuccess) || xs.reserve(n * sbytes(px)) != hipSuccess ||
+      mid.reserve(n * (size_t)sbytes(QPHIX_AMD_F64)) != hipSuccess ||
+      scb.reserve(sizeof(CGScalars)) != hipSuccess || part.reserve(kMaxPartials * 8) != hipSuccess)
     return QPHIX_AMD_ENOMEM;
   CGScalars *sc = (CGScalars *)scb.p;
   double *partial = (double *)part.p;
-  CGScalars *hsc = nullptr;
-  if (hipHostMalloc((void **)&hsc, sizeof(CGScalars), hipHostMallocDefault) != hipSuccess)
+  if (!op->hsc && hipHostMalloc((void **)&op->hsc, sizeof(CGScalars), hipHostMallocDefault) != hipSuccess) {
+    op->hsc = nullptr;
     return QPHIX_AMD_ENOMEM;
-  struct HostFree {
-    CGScalars *p;
-    ~HostFree() { (void)hipHostFree(p); }
-  } hf{hsc};
+  }
+  CGScalars *hsc = op->hsc;
 
   auto fetch = [&]() -> hipError_t {
     hipError_t e = hipMemcpyAsync(hsc, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, st);
