@@ -214,30 +214,10 @@ int qphix_amd_eo_op_create(qphix_amd_eo_op **out_op, qphix_amd_ctx *ctx, const d
   op->recon_s = recon_sloppy;
   const long V = (long)ctx->g.vcb * 2;
   const int nreal_cl = op->tm ? 144 : 72;
-  // FP64 lex epilogue fields: [dagger] blocks for the cb-0 dslash (inverse)
-  // and for the cb-1 achimbdpsi (forward)
-  DevBuf lexd[2], lexa[2];
-  int rc = QPHIX_AMD_OK;
-  for (int d = 0; d < 2 && rc == QPHIX_AMD_OK; d++) {
-    if (lexd[d].alloc(V * nreal_cl * 8) != hipSuccess || lexa[d].alloc(V * nreal_cl * 8) != hipSuccess)
-      rc = QPHIX_AMD_ENOMEM;
-  }
-  hipError_t e = hipSuccess;
-  if (rc == QPHIX_AMD_OK) {
-    for (int d = 0; d < 2; d++) {
-      const int sign = d ? -1 : 1;
-      if (op->tm) {
-        e = run_clover_blocks(ctx, (double *)lexd[d].p, clov_lex, 2, mu, sign, st);
-        if (e == hipSuccess) e = run_clover_blocks(ctx, (double *)lexa[d].p, clov_lex, 1, mu, sign, st);
-      } else {
-        e = run_clover_blocks(ctx, (double *)lexd[d].p, clov_lex, 0, 0.0, 1, st);
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(lexa[d].p, clov_lex, V * 72 * 8, hipMemcpyDeviceToDevice, st);
-      }
-      if (e != hipSuccess) break;
-    }
-  }
   const int field = op->tm ? QPHIX_AMD_FIELD_FULLCLOVER : QPHIX_AMD_FIELD_CLOVER;
+  int rc = QPHIX_AMD_OK;
+  hipError_t e = hipSuccess;
+  // the packed fields of both precision levels; links packed right away
   for (int lvl = 0; lvl < 2 && rc == QPHIX_AMD_OK && e == hipSuccess; lvl++) {
     const int prec = level_prec(op, lvl), recon = level_recon(op, lvl);
     const size_t gb = qphix_amd_field_bytes(ctx, QPHIX_AMD_FIELD_GAUGE, prec, recon);
@@ -249,13 +229,30 @@ int qphix_amd_eo_op_create(qphix_amd_eo_op **out_op, qphix_amd_ctx *ctx, const d
       if (op->dsl[lvl][d].alloc(cb) != hipSuccess || op->achi[lvl][d].alloc(cb) != hipSuccess)
         rc = QPHIX_AMD_ENOMEM;
     if (op->tmp[lvl].alloc(sb) != hipSuccess) rc = QPHIX_AMD_ENOMEM;
-    if (rc != QPHIX_AMD_OK) break;
-    for (int c = 0; c < 2 && e == hipSuccess; c++)
+    for (int c = 0; c < 2 && rc == QPHIX_AMD_OK && e == hipSuccess; c++)
       e = run_pack(ctx, QPHIX_AMD_FIELD_GAUGE, prec, recon, op->gauge[lvl][c].p, u_lex, c, st);
-    for (int d = 0; d < 2 && e == hipSuccess; d++) {
-      e = run_pack(ctx, field, prec, 0, op->dsl[lvl][d].p, (const double *)lexd[d].p, 0, st);
-      if (e == hipSuccess)
-        e = run_pack(ctx, field, prec, 0, op->achi[lvl][d].p, (const double *)lexa[d].p, 1, st);
+  }
+  // FP64 lexicographic epilogue blocks, one dagger at a time: the cb-0
+  // dslash takes the inverse blocks, the cb-1 achimbdpsi the forward ones
+  // (with mu = 0 the clover field itself, for both daggers)
+  DevBuf lexd, lexa;
+  if (rc == QPHIX_AMD_OK &&
+      (lexd.alloc(V * nreal_cl * 8) != hipSuccess || (op->tm && lexa.alloc(V * nreal_cl * 8) != hipSuccess)))
+    rc = QPHIX_AMD_ENOMEM;
+  for (int d = 0; d < 2 && rc == QPHIX_AMD_OK && e == hipSuccess; d++) {
+    const int sign = d ? -1 : 1;
+    const double *fwd = clov_lex;
+    if (op->tm) {
+      e = run_clover_blocks(ctx, (double *)lexd.p, clov_lex, 2, mu, sign, st);
+      if (e == hipSuccess) e = run_clover_blocks(ctx, (double *)lexa.p, clov_lex, 1, mu, sign, st);
+      fwd = (const double *)lexa.p;
+    } else if (d == 0) {
+      e = run_clover_blocks(ctx, (double *)lexd.p, clov_lex, 0, 0.0, 1, st);
+    }
+    for (int lvl = 0; lvl < 2 && e == hipSuccess; lvl++) {
+      const int prec = level_prec(op, lvl);
+      e = run_pack(ctx, field, prec, 0, op->dsl[lvl][d].p, (const double *)lexd.p, 0, st);
+      if (e == hipSuccess) e = run_pack(ctx, field, prec, 0, op->achi[lvl][d].p, fwd, 1, st);
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);  // lex temporaries are freed next
