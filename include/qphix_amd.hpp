@@ -318,6 +318,23 @@ class EvenOddOperator {
     check(qphix_amd_cg_solve(op_, x, b, &p, &r, stream_), "qphix_amd_cg_solve");
     return r;
   }
+  /* accumulate scale * dS/dU of S = phi^dag (M^dag M)^-1 phi at
+     x = (M^dag M)^-1 phi (FP64 native, cb 1) into force_lex (lexicographic
+     FP64, 4 links x 18 reals per site): the hopping term; project: su(3)
+     force, else the link gradient */
+  void force(double *force_lex, const void *x, const double *u_lex, double scale = 1.0,
+             bool project = true) const
+  {
+    check(qphix_amd_eo_op_force(op_, force_lex, x, u_lex, scale, project ? 1 : 0, stream_), "qphix_amd_eo_op_force");
+  }
+  /* the same with the clover term, for an operator built on
+     clover_from_links(.., csw) of the same links */
+  void force_clover(double *force_lex, const void *x, const double *u_lex, double csw, double scale = 1.0,
+                    bool project = true) const
+  {
+    check(qphix_amd_eo_op_force_clover(op_, force_lex, x, u_lex, csw, scale, project ? 1 : 0, stream_),
+          "qphix_amd_eo_op_force_clover");
+  }
 
  private:
   qphix_amd_eo_op *op_;

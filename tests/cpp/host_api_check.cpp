@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "qphix_amd.hpp"
@@ -196,6 +197,23 @@ int gpu_mode(const char *in_path, const char *out_path)
   }
   std::printf("cg f64_iters %d f64_rsd %.3e f16_iters %d f16_rsd %.3e f16_reliable %d rel_diff %.3e\n", r64.iters,
               r64.rsd_final, r16.iters, r16.rsd_final, r16.reliable_updates, std::sqrt(num / den));
+
+  // pseudofermion force at the FP64 solution: hopping term alone, then with
+  // the clover term (both projected to su(3)), written to <out>.force
+  Dev fh(nU * 8), fc(nU * 8);
+  HIPCHECK(hipMemset(fh.p, 0, nU * 8));
+  HIPCHECK(hipMemset(fc.p, 0, nU * 8));
+  op.force(static_cast<double *>(fh.p), x64.p, U.d(), 0.5);
+  op.force_clover(static_cast<double *>(fc.p), x64.p, U.d(), 1.0, 0.5);
+  std::vector<double> hf(2 * nU);
+  HIPCHECK(hipMemcpy(hf.data(), fh.p, nU * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(hf.data() + nU, fc.p, nU * 8, hipMemcpyDeviceToHost));
+  const std::string fpath = std::string(out_path) + ".force";
+  FILE *of = std::fopen(fpath.c_str(), "wb");
+  if (!of) return fail("cannot open force output");
+  std::fwrite(hf.data(), sizeof(double), hf.size(), of);
+  std::fclose(of);
+  std::printf("force written\n");
   return 0;
 }
 

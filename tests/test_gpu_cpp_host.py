@@ -6,17 +6,21 @@ host program tests/cpp/host_api_check, built with the library.
 storage with 12-real links, anisotropic coefficients and antiperiodic time,
 against the FP64 oracle (1e-13 / 1e-6 relative); then the even-odd clover
 operator's FP64 CG and FP16 mixed CG to a true residual of 1e-10 and their
-agreement."""
+agreement; its force and force_clover at the FP64 solution against the same
+calls through the Python binding."""
 import os
 import re
 import subprocess
 
 import numpy as np
 import pytest
+import torch
 
 from _util import cb_mask
 
 pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "cpp", "host_api_check")
@@ -77,3 +81,18 @@ def test_cpp_classes_against_oracle(qamd, orc, tmp_path):
     print(r.stdout.strip(), flush=True)
     assert f64_rsd < 1e-10 and f16_rsd < 1e-10
     assert diff < 1e-8
+    # EvenOddOperator::force / force_clover against the same calls through
+    # the Python binding, at the same (deterministic) FP64 CG solution
+    fr = np.fromfile(str(out) + ".force", dtype=np.float64).reshape(2, V, 4, 3, 3, 2)
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    Ud = torch.from_numpy(U).to(DEV)
+    A = qamd.clover_from_links(geom, Ud, 1.0 / (2 * 0.135), 1.0)
+    op = qamd.EvenOddCloverOperator(geom, Ud, A, prec_sloppy=qamd.F16, recon_sloppy=12)
+    x = geom.new_spinor(qamd.F64, cb=1)
+    b = geom.pack_spinor(torch.from_numpy(psi).to(DEV), qamd.F64, 1)
+    qamd.InvCG(op, max_iters=2000)(x, b, 1e-10, mixed=False)
+    for k, csw in ((0, None), (1, 1.0)):
+        F = op.force(torch.zeros_like(Ud), x, Ud, scale=0.5, project=True, csw=csw).cpu().numpy()
+        err = np.abs(fr[k] - F).max() / np.abs(F).max()
+        assert err < 1e-13, (k, err)
+    op.close()
