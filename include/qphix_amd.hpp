@@ -164,6 +164,57 @@ class Geometry {
     check(qphix_amd_full_clover_from_qphix(ctx_, &lay, dst, prec, src, cb, stream),
           "qphix_amd_full_clover_from_qphix");
   }
+  /* bytes of a QPhiX block array of this lattice in layout lay */
+  size_t qphix_field_bytes(const qphix_amd_qphix_layout &lay, int field) const
+  {
+    return qphix_amd_qphix_field_bytes(ctx_, &lay, field);
+  }
+
+  /* faces of a lattice split over several GPUs (set_comm): half-spinors
+     per face, and the projected half-spinors of one face of `in`; each
+     operator class completes a face with face_finish_dslash or
+     face_finish_achimbdpsi */
+  long face_sites(int dim) const { return qphix_amd_face_sites(ctx_, dim); }
+  void face_pack(int prec, int isign, int cb, int dim, int fb, void *buf, const void *in,
+                 void *stream = nullptr) const
+  {
+    check(qphix_amd_face_pack(ctx_, prec, isign, cb, dim, fb, buf, in, stream), "qphix_amd_face_pack");
+  }
+
+  /* differentiation of L = Re<chi, D psi> (no QPhiX counterpart): the
+     adjoint D^dagger chi on checkerboard 1-cb, and the link gradient
+     (project: su(3) force) accumulated into lexicographic FP64 force_lex */
+  void dslash_adjoint(int prec, int recon, int isign, int cb, void *grad_psi, const void *chi,
+                      const void *gauge, double coeff_s = 1.0, double coeff_t_f = 1.0, double coeff_t_b = 1.0,
+                      void *stream = nullptr) const
+  {
+    check(qphix_amd_dslash_adjoint(ctx_, prec, recon, isign, cb, grad_psi, chi, gauge, coeff_s, coeff_t_f,
+                                   coeff_t_b, stream),
+          "qphix_amd_dslash_adjoint");
+  }
+  void dslash_force(int prec, int isign, int cb, double *force_lex, const void *chi, const void *psi,
+                    const double *u_lex, double scale = 1.0, bool project = true, double coeff_s = 1.0,
+                    double coeff_t_f = 1.0, double coeff_t_b = 1.0, void *stream = nullptr) const
+  {
+    check(qphix_amd_dslash_force(ctx_, prec, isign, cb, force_lex, chi, psi, u_lex, coeff_s, coeff_t_f, coeff_t_b,
+                                 scale, project ? 1 : 0, stream),
+          "qphix_amd_dslash_force");
+  }
+  /* link gradient of Re sum_x w(x) chi^dag A psi, A = clover_from_links(.., csw),
+     w = scale_even / scale_odd by site parity (lexicographic FP64 fields) */
+  void clover_force(double *force_lex, const double *chi_lex, const double *psi_lex, const double *u_lex,
+                    double csw, double scale_even = 1.0, double scale_odd = 1.0, bool project = true,
+                    void *stream = nullptr) const
+  {
+    check(qphix_amd_clover_force(ctx_, force_lex, chi_lex, psi_lex, u_lex, csw, scale_even, scale_odd,
+                                 project ? 1 : 0, stream),
+          "qphix_amd_clover_force");
+  }
+  /* project lexicographic FP64 links back onto SU(3), in place */
+  void reunitarize(double *u_lex, void *stream = nullptr) const
+  {
+    check(qphix_amd_reunitarize(ctx_, u_lex, stream), "qphix_amd_reunitarize");
+  }
 
  private:
   qphix_amd_ctx *ctx_;
@@ -193,6 +244,20 @@ class Dslash {
                                       beta * ct_, beta * ct_, stream_),
           "qphix_amd_dslash_achimbdpsi");
   }
+  /* complete the sites of face (dim, fb) from a received face buffer:
+     after dslash (beta = -coeff; each family has its own), and after
+     dslashAChiMinusBDPsi (beta = +coeff, the same for every family) */
+  void face_finish_dslash(void *res, const void *buf, const void *u, double beta, int isign, int cb, int dim,
+                          int fb) const
+  {
+    face_finish_achimbdpsi(res, buf, u, beta, isign, cb, dim, fb);
+  }
+  void face_finish_achimbdpsi(void *res, const void *buf, const void *u, double beta, int isign, int cb, int dim,
+                              int fb) const
+  {
+    check(qphix_amd_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_),
+          "qphix_amd_face_finish");
+  }
 
  protected:
   qphix_amd_ctx *ctx_;
@@ -219,6 +284,14 @@ class ClovDslash : public Dslash {
     check(qphix_amd_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, clov, beta * cs_,
                                            beta * ct_, beta * ct_, prec_mass_rho, stream_),
           "qphix_amd_clov_dslash_achimbdpsi");
+  }
+  /* face completion of dslash (beta = +coeff) */
+  void face_finish_dslash(void *res, const void *buf, const void *u, const void *invclov, double beta, int isign,
+                          int cb, int dim, int fb) const
+  {
+    check(qphix_amd_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invclov, beta,
+                                     stream_),
+          "qphix_amd_clov_face_finish");
   }
 };
 
@@ -251,6 +324,15 @@ class TMDslash : public Dslash {
                                          beta * ct_, beta * ct_, mu_phys_ / alpha, stream_),
           "qphix_amd_tm_dslash_achimbdpsi");
   }
+  /* face completion of dslash (beta = -coeff), with the twisted epilogue */
+  void face_finish_dslash(void *res, const void *buf, const void *u, double beta, int isign, int cb, int dim,
+                          int fb) const
+  {
+    const double mu = mu_phys_ / alpha_, muinv = alpha_ / (alpha_ * alpha_ + mu_phys_ * mu_phys_);
+    check(qphix_amd_tm_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu, muinv,
+                                   stream_),
+          "qphix_amd_tm_face_finish");
+  }
 
  private:
   double alpha_, mu_phys_;
@@ -275,6 +357,14 @@ class TMClovDslash : public Dslash {
     check(qphix_amd_tm_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, fclov,
                                               beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_),
           "qphix_amd_tm_clov_dslash_achimbdpsi");
+  }
+  /* face completion of dslash (beta = +coeff) */
+  void face_finish_dslash(void *res, const void *buf, const void *u, const void *invfclov, double beta, int isign,
+                          int cb, int dim, int fb) const
+  {
+    check(qphix_amd_tm_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invfclov, beta,
+                                        stream_),
+          "qphix_amd_tm_clov_face_finish");
   }
 };
 

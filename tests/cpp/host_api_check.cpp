@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -91,6 +92,36 @@ int cpu_mode()
     threw = e.code() == QPHIX_AMD_EINVAL;
   }
   if (!threw) return fail("recon 11 did not throw EINVAL");
+  // the face, differentiation and link wrappers reach their C entry points
+  // (bad arguments are rejected there, before any device work)
+  if (g.face_sites(0) != 8 * 8 * 16 / 2 || g.face_sites(3) != 16 * 16 * 8 / 2 || g.face_sites(4) != 0)
+    return fail("face_sites");
+  double dummy[4];
+  TMDslash T(g, QPHIX_AMD_F32, 12, 0.1, 0.1);
+  TMClovDslash TC(g, 7, 12);
+  const struct {
+    const char *what;
+    std::function<void()> call;
+  } bad[] = {
+      {"face_pack dim 4", [&] { g.face_pack(QPHIX_AMD_F32, 1, 0, 4, 0, dummy, dummy + 2); }},
+      {"dslash_adjoint cb 2", [&] { g.dslash_adjoint(QPHIX_AMD_F32, 12, 1, 2, dummy, dummy, dummy); }},
+      {"dslash_force null force", [&] { g.dslash_force(QPHIX_AMD_F64, 1, 0, nullptr, dummy, dummy, dummy); }},
+      {"clover_force null force", [&] { g.clover_force(nullptr, dummy, dummy, dummy, 1.0); }},
+      {"reunitarize null links", [&] { g.reunitarize(nullptr); }},
+      {"face_finish_achimbdpsi isign 0", [&] { D.face_finish_achimbdpsi(dummy, dummy, dummy, 1.0, 0, 0, 0, 0); }},
+      {"clov face_finish_dslash recon 11", [&] { C.face_finish_dslash(dummy, dummy, dummy, dummy, 1.0, 1, 0, 0, 0); }},
+      {"tm face_finish_dslash dim 5", [&] { T.face_finish_dslash(dummy, dummy, dummy, -1.0, 1, 0, 5, 0); }},
+      {"tm_clov face_finish_dslash prec 7", [&] { TC.face_finish_dslash(dummy, dummy, dummy, dummy, 1.0, 1, 0, 0, 0); }},
+  };
+  for (const auto &b : bad) {
+    threw = false;
+    try {
+      b.call();
+    } catch (const Error &e) {
+      threw = e.code() == QPHIX_AMD_EINVAL;
+    }
+    if (!threw) return fail(b.what);
+  }
   std::printf("cpu ok\n");
   return 0;
 }
