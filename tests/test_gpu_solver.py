@@ -170,6 +170,40 @@ def test_cg_repeatable(qamd, lat):
         op.close()
 
 
+def test_workspace_reuse_across_calls(qamd, lat):
+    """One operator keeps its CG and force workspaces between calls: a
+    mixed FP16 solve, an FP64 solve (larger sloppy vectors), the force
+    (which shares the FP64 vectors) and another mixed solve give exactly
+    what fresh operators give."""
+    dims, U, b = lat
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+
+    def solve(op, mixed):
+        x = geom.new_spinor(qamd.F64, cb=1)
+        qamd.InvCG(op, max_iters=5000)(x, bf, 1e-10, mixed=mixed)
+        return x
+
+    def fresh(mixed):
+        op = qamd.EvenOddCloverOperator(geom, Ud, A, mu=0.1, prec_sloppy=qamd.F16, recon_sloppy=12)
+        x = solve(op, mixed)
+        F = op.force(torch.zeros_like(Ud), x, Ud, project=True, csw=CSW) if not mixed else None
+        op.close()
+        return x.data.clone(), F
+
+    x16, _ = fresh(True)
+    x64, F64 = fresh(False)
+    op = qamd.EvenOddCloverOperator(geom, Ud, A, mu=0.1, prec_sloppy=qamd.F16, recon_sloppy=12)
+    assert torch.equal(solve(op, True).data, x16)
+    y = solve(op, False)
+    assert torch.equal(y.data, x64)
+    assert torch.equal(op.force(torch.zeros_like(Ud), y, Ud, project=True, csw=CSW), F64)
+    assert torch.equal(solve(op, True).data, x16)
+    op.close()
+
+
 def test_cg_without_fused_norm(qamd, orc):
     """4x4x4x6: 192 sites per checkerboard, not a whole number of 128-site
     work-groups, so the CG takes <p, M^dag M p> from a separate dot product
