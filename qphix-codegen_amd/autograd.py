@@ -20,14 +20,22 @@ su3_force(...)       the traceless anti-Hermitian su(3) force of either loss,
 
 import torch
 
-from ._lib import F64, check, lib
+from ._lib import F32, F64, FIELD_SPINOR, QPhiXAMDError, check, lib
 from .dslash import Dslash, _ptr, _stream  # noqa: F401
-from .solver import EvenOddCloverOperator, InvCG, clover_from_links
+from .solver import LEX_GAUGE, EvenOddCloverOperator, InvCG, _lex64, clover_from_links
 
 
 def dslash_force(geom, force, chi, psi, U, isign, cb, scale=1.0, project=True):
     """Accumulate scale * (link gradient of Re<chi, D psi>) into force (lex
     FP64 (V,4,3,3,2)); chi, psi native fields on cb, 1-cb (FP64 or FP32)."""
+    if chi.prec != psi.prec or chi.prec not in (F64, F32):
+        raise QPhiXAMDError("chi and psi must share a precision, FP64 or FP32")
+    for f, what in ((chi, "chi"), (psi, "psi")):
+        if f.kind != FIELD_SPINOR or f.data.numel() != 24 * geom.vcb:
+            raise QPhiXAMDError(f"{what}: not a spinor of this geometry")
+    if chi.cb not in (None, cb) or psi.cb not in (None, 1 - cb):
+        raise QPhiXAMDError("chi lives on cb and psi on 1 - cb")
+    force, U = _lex64(geom, (force, LEX_GAUGE, "force", True), (U, LEX_GAUGE, "U", False))
     check(lib().qphix_amd_dslash_force(geom.ctx, chi.prec, isign, cb, _ptr(force), chi.ptr(), psi.ptr(),
                                        _ptr(U), 1.0, 1.0, 1.0, float(scale), int(bool(project)),
                                        _stream()),

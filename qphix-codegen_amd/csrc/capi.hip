@@ -82,8 +82,16 @@ int qphix_amd_ctx_create(qphix_amd_ctx **ctx, int Lx, int Ly, int Lz, int Lt, do
   // Every extent even: the wrap-around hop must change the site parity.
   if (Lx < 2 || Ly < 2 || Lz < 2 || Lt < 2) return QPHIX_AMD_EGEOM;
   if ((Lx | Ly | Lz | Lt) & 1) return QPHIX_AMD_EGEOM;
-  const long vcb = (long)Lx * Ly * Lz * Lt / 2;
-  if (vcb * 72L >= (1L << 31)) return QPHIX_AMD_EGEOM;  // 32-bit vector indices
+  // Every field index must fit 32 bits: 72 reals x sites per checkerboard
+  // < 2^31.  The bound is applied to each partial product, so no product of
+  // extents can overflow on the way (each factor is below 2^31 and each
+  // partial product below 2^31 / 72).
+  const long kMaxVcb = ((1L << 31) - 1) / 72;
+  long vcb = Lx / 2;
+  for (int L : {Ly, Lz, Lt}) {
+    if (vcb > kMaxVcb / L) return QPHIX_AMD_EGEOM;
+    vcb *= L;
+  }
   qphix_amd_ctx *c = new (std::nothrow) qphix_amd_ctx();
   if (!c) return QPHIX_AMD_ENOMEM;
   c->g.Lx = Lx;

@@ -276,6 +276,7 @@ int qphix_amd_eo_op_apply(qphix_amd_eo_op *op, int prec, int dagger, void *out, 
                           void *stream)
 {
   if (!op || !out || !in || out == in) return QPHIX_AMD_EINVAL;
+  if (op->ctx->comm_mask) return QPHIX_AMD_EINVAL;  // split after the operator was built
   int lvl;
   if (prec == QPHIX_AMD_F64)
     lvl = 0;
@@ -291,6 +292,7 @@ int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
                           const double *u_lex, double scale, int project, void *stream)
 {
   if (!op || !force_lex || !x || !u_lex) return QPHIX_AMD_EINVAL;
+  if (op->ctx->comm_mask) return QPHIX_AMD_EINVAL;  // needs the whole lattice
   hipStream_t st = (hipStream_t)stream;
   const long n = 24L * op->ctx->g.vcb;
   DevBuf &y = op->w64[0], &z = op->w64[1], &w = op->w64[2];
@@ -332,6 +334,7 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
                                  void *stream)
 {
   if (!op || !force_lex || !x || !u_lex) return QPHIX_AMD_EINVAL;
+  if (op->ctx->comm_mask) return QPHIX_AMD_EINVAL;  // needs the whole lattice
   hipStream_t st = (hipStream_t)stream;
   const long n = 24L * op->ctx->g.vcb;
   DevBuf &y = op->w64[0], &z = op->w64[1], &w = op->w64[2];
@@ -388,6 +391,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
 {
   if (!op || !x || !b || !prm || !res || x == b) return QPHIX_AMD_EINVAL;
   if (!(prm->rsd_target > 0.0) || prm->max_iters <= 0) return QPHIX_AMD_EINVAL;
+  if (op->ctx->comm_mask) return QPHIX_AMD_EINVAL;  // split after the operator was built
   hipStream_t st = (hipStream_t)stream;
   auto t0 = std::chrono::steady_clock::now();
   const long vcb = op->ctx->g.vcb;

@@ -48,6 +48,11 @@ def test_ctx_validation(qamd):
     # every field index must fit 32 bits: 72 reals x sites per checkerboard < 2^31
     assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 128, 128, 128, 128, -1.0) == -2
     assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), 96, 96, 96, 96, -1.0) == -2
+    # extents whose product overflows 32 or 64 bits are refused, not wrapped
+    # (65536^4 / 2 = 2^63 wraps a signed 64-bit product to a negative number)
+    for dims in ((65536, 65536, 65536, 65536), (65536, 65536, 2, 2), (2, 2, 65536, 65536),
+                 (1 << 30, 2, 2, 2), (2, 2, 2, 1 << 30), (2, 65534, 65536, 2)):
+        assert lib.qphix_amd_ctx_create(ctypes.byref(ctx), *dims, -1.0) == -2, dims
     big = ctypes.c_void_p()  # 64^4: 72 x 8,388,608 < 2^31
     assert lib.qphix_amd_ctx_create(ctypes.byref(big), 64, 64, 64, 64, -1.0) == 0
     lib.qphix_amd_ctx_destroy(big)

@@ -285,3 +285,37 @@ def test_block_inverses_of_indefinite_hermitian_blocks(qamd):
         rf = torch.linalg.inv(twisted)
         gz = torch.complex(blk[..., 0], blk[..., 1]).reshape(V, 2, 6, 6)
         assert (gz - rf).abs().max().item() / rf.abs().max().item() < 1e-13
+
+
+def test_split_after_create_refused(qamd, lat):
+    """An operator built on a whole lattice refuses to run once the context
+    is marked split (qphix_amd_set_comm): it would otherwise drop the
+    wrap-around hops and return a wrong M.  Clearing the mask restores it."""
+    import ctypes
+    dims, U, b = lat
+    geom = qamd.Geometry(dims)
+    Ud = U.to(DEV)
+    A = qamd.clover_from_links(geom, Ud, ALPHA, CSW)
+    op = qamd.EvenOddCloverOperator(geom, Ud, A, prec_sloppy=qamd.F32, recon_sloppy=12)
+    bf = geom.pack_spinor(b.to(DEV), qamd.F64, 1)
+    ref = geom.new_spinor(qamd.F64, cb=1)
+    op.apply(ref, bf)
+    lib = qamd.lib()
+    geom.set_comm((0, 0, 0, 1))
+    out = geom.new_spinor(qamd.F64, cb=1)
+    force = torch.zeros_like(Ud)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.qphix_amd_eo_op_apply(op._op, qamd.F64, 0, out.ptr(), bf.ptr(), st) == -1
+    assert lib.qphix_amd_eo_op_force(op._op, ctypes.c_void_p(force.data_ptr()), bf.ptr(),
+                                     ctypes.c_void_p(Ud.data_ptr()), 1.0, 1, st) == -1
+    assert lib.qphix_amd_eo_op_force_clover(op._op, ctypes.c_void_p(force.data_ptr()), bf.ptr(),
+                                            ctypes.c_void_p(Ud.data_ptr()), CSW, 1.0, 1, st) == -1
+    prm = qamd._lib.CGParams(1e-10, 100, 0.0, 0)
+    res = qamd._lib.CGResult()
+    assert lib.qphix_amd_cg_solve(op._op, out.ptr(), bf.ptr(), ctypes.byref(prm), ctypes.byref(res), st) == -1
+    geom.set_comm((0, 0, 0, 0))
+    op.apply(out, bf)
+    torch.cuda.synchronize()
+    assert torch.equal(out.data, ref.data)
+    assert torch.count_nonzero(force) == 0
+    op.close()
