@@ -11,14 +11,17 @@ timing starts; two input/output sets alternate so the Infinity Cache cannot
 hold a step's working set.
 
 Prints one JSON line: metric, value = GFLOP/s of the step, the roofline of
-the dominant kernel from HIP-event timing, "cg" = FP64 vs mixed-precision CG
-time to a true residual of 1e-10 (the 32^3x64 Wilson-clover solve), and
-the CPU baseline (the
-reference's own AVX-512/AVX2 kernel on the host cores).
+the dominant kernel from HIP-event timing, every reduced-precision kernel's
+error against the FP64 kernel, "cg" = FP64 vs mixed-precision CG time to a
+true residual of 1e-10 (the 32^3x64 Wilson-clover solve), "cfg4" = the
+48^3x96 twisted-mass clover case (dslash throughput and error, FP64 vs
+FP16-mixed CG, the pseudofermion force and its error), and the CPU baseline
+(the reference's own AVX-512/AVX2 kernels on the host cores).
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -52,88 +55,138 @@ def time_kernel(fn, n):
     return s.elapsed_time(e) / n * 1e-3
 
 
-def _host_isa():
+def _host_flags():
     try:
-        flags = open("/proc/cpuinfo").read()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return set(line.split(":", 1)[1].split())
     except OSError:
-        return None
-    if "avx512f" in flags:
-        return "avx512_f32"
-    if "avx2" in flags and "fma" in flags:
-        return "avx2_f32"
-    return None
+        pass
+    return set()
 
 
-def host_triad_gbs(nthreads, n=1 << 26):
-    """STREAM-style triad c = a + s b on the host (torch's CPU kernels on
-    nthreads threads, 3 x 256 MB FP32 arrays, best of 5): 12 bytes per
-    element counted."""
-    import torch
-    prev = torch.get_num_threads()
-    torch.set_num_threads(nthreads)
+def host_cpu():
+    """The host CPUs this process may run on: model, logical CPUs and
+    physical cores (distinct (package, core) pairs among them)."""
     try:
-        a, b, c = torch.rand(n), torch.rand(n), torch.empty(n)
-        best = float("inf")
-        for _ in range(5):
-            t0 = time.perf_counter()
-            torch.add(a, b, alpha=1.5, out=c)
-            best = min(best, time.perf_counter() - t0)
-        return 12.0 * n / best / 1e9
-    finally:
-        torch.set_num_threads(prev)
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = set(range(os.cpu_count() or 1))
+    model, cores = None, set()
+    try:
+        text = open("/proc/cpuinfo").read()
+    except OSError:
+        text = ""
+    for block in text.split("\n\n"):
+        d = {}
+        for line in block.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                d[k.strip()] = v.strip()
+        if "processor" not in d or int(d["processor"]) not in allowed:
+            continue
+        model = model or d.get("model name")
+        cores.add((d.get("physical id", "0"), d.get("core id", d["processor"])))
+    return {"model": model, "logical_cpus": len(allowed), "physical_cores": len(cores) or len(allowed)}
 
 
-def cpu_baseline(seconds=10.0):
-    """The reference's own generated FP32 clover kernel (oracle/_ref, built
-    from the qphix-codegen sources) under an OpenMP block loop on the host
-    cores, on the whole 32^3 x 64 lattice (about 0.9 GB of fields per call,
-    so the kernel streams from host memory rather than from caches), timed
-    over about `seconds` of calls.  The inputs are generated on the GPU when
-    there is one (generation is not timed).  Falls back to the FP64 C oracle
-    (one thread, a bounded sample of sites) if the reference build is
-    absent."""
+# The reference's ISA configurations timed as the CPU baseline (VECLEN =
+# SOALEN, codegen/jinja/isa.js): name, host flags needed, storage bytes.
+CPU_CONFIGS = (("avx512_f32", {"avx512f", "fma"}, 4), ("avx512_f64", {"avx512f", "fma"}, 8),
+               ("avx2_f32", {"avx2", "fma"}, 4))
+
+
+def _cpu_baseline_child(seconds_per_config, dims):
+    """Runs in a child process that loads no torch (and so no second OpenMP
+    runtime) with OMP_PROC_BIND=close, OMP_PLACES=cores: the reference's
+    generated clover kernel under an OpenMP block loop, for each of the
+    AVX-512 FP32, AVX-512 FP64 and AVX2 FP32 builds the host supports, about
+    `seconds_per_config` of calls each (median call time), and a STREAM
+    triad on the same threads.  The inputs are Gaussian arrays of the
+    workload's shapes (FP64, converted by the reference driver): the
+    kernels' instruction and memory streams do not depend on the values."""
     import numpy as np
-    import torch
     from oracle import ref
+    flags = _host_flags()
+    cfgs = [(c, b) for c, need, b in CPU_CONFIGS if need <= flags and ref.available(c)]
+    nthr = int(os.environ["OMP_NUM_THREADS"])
+    if not cfgs:
+        return None
+    rng = np.random.default_rng(5)
+    V = int(np.prod(dims))
+    U = rng.standard_normal((V, 4, 3, 3, 2))
+    psi = rng.standard_normal((V, 4, 3, 2))
+    inv = rng.standard_normal((V, 2, 36))
+    vcb = V // 2
+    triad = ref.stream_triad(cfgs[0][0], nthr)
+    by = {}
+    for cfg, sb in cfgs:
+        _, t = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=3)
+        reps = max(5, int(seconds_per_config / max(float(np.median(t)), 1e-6)))
+        _, t = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=reps)
+        med = float(np.median(t))
+        gbs = vcb * (24 + 96 + 72 + 24) * sb / med / 1e9  # algorithmic bytes, 12-real links
+        by[cfg] = {"gflops": round(vcb * FLOP_CLOVER / med / 1e9, 2), "ms_median": round(med * 1e3, 3),
+                   "ms_min": round(float(np.min(t)) * 1e3, 3), "calls": reps,
+                   "veclen": ref.soalen(cfg), "gbs": round(gbs, 1),
+                   "host_roofline_frac": round(gbs / triad, 3)}
+    return {"triad": triad, "by": by}
+
+
+def cpu_baseline(seconds_per_config=6.0, dims=DIMS):
+    """The reference's own generated clover kernel (oracle/_ref, built from
+    the qphix-codegen sources) on the host cores, on the whole 32^3 x 64
+    lattice (0.9-1.8 GB of fields per call, so the kernel streams from host
+    memory rather than from caches), in the AVX-512 FP32, AVX-512 FP64 and
+    AVX2 FP32 builds the host supports; the headline value is AVX-512 FP32
+    (else the first available).
+    It runs in a child process with OMP_PROC_BIND=close, OMP_PLACES=cores
+    and one thread per CPU this process may use (OMP_NUM_THREADS when set):
+    in this process torch's own OpenMP runtime would, under those settings,
+    pin the main thread to one core as it loads, and the reference's
+    separate runtime would then see a one-core affinity mask.
+    Falls back to the FP64 C oracle (one thread, a bounded sample of sites)
+    if no reference build is present."""
+    import numpy as np
+    cpu = host_cpu()
+    nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cpu["logical_cpus"]
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(nthr))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+           "--cpu-seconds", str(seconds_per_config), "--lattice", ",".join(map(str, dims))]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    child = json.loads(r.stdout.strip().splitlines()[-1])
+    if child is not None:
+        by, triad = child["by"], child["triad"]
+        vcb = int(np.prod(dims)) // 2
+        head = "avx512_f32" if "avx512_f32" in by else next(iter(by))
+        h = by[head]
+        return {"value": h["gflops"], "unit": "GFLOP/s", "cores": nthr, "kind": "reference",
+                "physical_cores": cpu["physical_cores"], "logical_cpus": cpu["logical_cpus"],
+                "cpu_model": cpu["model"], "omp_proc_bind": "close", "omp_places": "cores",
+                "host_triad_gbs": round(triad, 1), "host_roofline_frac": h["host_roofline_frac"],
+                "by_config": by,
+                "sample": f"reference {head} clov_dslash (VECLEN=SOALEN={h['veclen']}, 12-real links) "
+                          f"on {'x'.join(map(str, dims))} ({vcb} sites/cb), median of {h['calls']} calls "
+                          f"({h['ms_median']:.2f} ms), OpenMP {nthr} threads pinned one per core place; "
+                          f"also " + ", ".join(c for c in by if c != head)}
     import __graft_entry__ as ge
-    q = ge.load_package()
-    L = q.lattice
-    cfg = _host_isa()
-    if cfg and ref.available(cfg):
-        dims = (32, 32, 32, 64)
-        gen = "cuda:0" if torch.cuda.is_available() else "cpu"
-        U = L.random_gauge(dims, 0.3, 5, device=gen).cpu().numpy()
-        psi = L.random_spinor(dims, 6, device=gen).cpu().numpy()
-        inv = L.invert_clover(L.random_clover(dims, 7, device=gen)).cpu().numpy()
-        if gen != "cpu":
-            torch.cuda.empty_cache()
-        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-        vcb = U.shape[0] // 2
-        _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=3)
-        reps = max(3, int(seconds / max(sec, 1e-6)))
-        _, sec = ref.apply(cfg, dims, psi, U, 1, 0, kind=1, clov=inv, recon=12, nthreads=nthr, reps=reps)
-        triad = host_triad_gbs(nthr)
-        kernel_gbs = vcb * 864 / sec / 1e9  # algorithmic bytes per site, FP32 12-real links
-        return {"value": round(vcb * FLOP_CLOVER / sec / 1e9, 2), "unit": "GFLOP/s", "cores": nthr,
-                "kind": "reference", "host_triad_gbs": round(triad, 1),
-                "host_roofline_frac": round(kernel_gbs / triad, 3),
-                "sample": f"reference {cfg} clov_dslash (VECLEN=SOALEN={ref.soalen(cfg)}, 12-real links) "
-                          f"on 32^3x64 ({vcb} sites/cb), {reps} calls x {sec * 1e3:.2f} ms, "
-                          f"OpenMP {nthr} threads"}
     from oracle import oracle as orc
+    L = ge.load_package().lattice
     dims = (16, 16, 16, 32)
     U = L.random_gauge(dims, 0.3, 5).numpy()
     psi = L.random_spinor(dims, 6).numpy()
     clov = L.invert_clover(L.random_clover(dims, 7)).numpy()
     sites = np.arange(0, U.shape[0], 2, dtype=np.int64)
     done, t0, n = 0, time.perf_counter(), 2048
-    while time.perf_counter() - t0 < seconds:
+    while time.perf_counter() - t0 < seconds_per_config:
         chunk = sites[(done % len(sites)):(done % len(sites)) + n]
         orc.dslash_sites(dims, chunk, psi, U, 1, kind=orc.CLOVER, clov=clov)
         done += len(chunk)
     dt = time.perf_counter() - t0
     return {"value": round(done * FLOP_CLOVER / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
-            "kind": "port",
+            "kind": "port", "physical_cores": cpu["physical_cores"], "cpu_model": cpu["model"],
             "sample": f"FP64 C oracle, clover dslash at {done} sites of a 16^3x32 lattice, "
                       f"{dt:.1f} s, 1 thread"}
 
@@ -170,28 +223,41 @@ def tm_kernel(q, geom, s, dev, n):
             "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4), "flop_per_site": 1392}
 
 
-def f32_error_vs_f64(q, geom, s, dev):
-    """Per-site max-abs difference of the FP32 clov_dslash from the FP64 one
-    on the same inputs (the FP64 kernels equal the FP64 oracle to 1e-13 in
-    the tests), absolute and relative to the largest output element."""
+def _err(out, ref):
+    d = (out - ref).abs().max().item()
+    return {"max_abs": d, "max_abs_rel": d / ref.abs().max().item()}
+
+
+def errors_vs_f64(q, geom, dev):
+    """Per-site max-abs difference of each reduced-precision kernel from the
+    FP64 kernel on the same inputs (the FP64 kernels equal the FP64 oracle
+    to 1e-13 in the tests), absolute and relative to the largest output
+    element: clov_dslash in FP32 and in the CG's sloppy FP16 / BF16 storage
+    (12-real links), and the twisted-mass tm_dslash (mu 0.1, kappa 0.1350)
+    in FP32."""
     import torch
     L = q.lattice
     U = L.random_gauge(DIMS, 0.3, 1000, device=dev)
-    clov = L.random_clover(DIMS, 1003, device=dev)
-    inv = L.invert_clover(clov)
+    inv = L.invert_clover(L.random_clover(DIMS, 1003, device=dev))
     psi = L.random_spinor(DIMS, 1001, device=dev)
-    op = q.ClovDslash(geom)
-    outs = []
-    for prec in (q.F64, q.F32):
-        t = geom.new_spinor(prec, cb=0)
-        op.dslash(t, geom.pack_spinor(psi, prec, 1), geom.pack_gauge(U, prec, 12, 0),
-                  geom.pack_clover(inv, prec, 0), 1, 0)
-        outs.append(geom.unpack_spinor(t))
-    d = (outs[1] - outs[0]).abs().max().item()
-    m = outs[0].abs().max().item()
-    del U, clov, inv, psi, outs
+    alpha, mu = 1.0 / (2 * 0.1350), 0.1
+    ops = {"clov_dslash": (q.ClovDslash(geom), True), "tm_dslash": (q.TMDslash(geom, alpha - 4.0, mu), False)}
+    precs = {"clov_dslash": (("f32", q.F32), ("f16", q.F16), ("bf16", q.BF16)), "tm_dslash": (("f32", q.F32),)}
+    out = {}
+    for name, (op, clover) in ops.items():
+        res = {}
+        for pname, prec in (("f64", q.F64),) + precs[name]:
+            t = geom.new_spinor(prec, cb=0)
+            args = (t, geom.pack_spinor(psi, prec, 1), geom.pack_gauge(U, prec, 12, 0))
+            if clover:
+                args = args + (geom.pack_clover(inv, prec, 0),)
+            op.dslash(*args, 1, 0)
+            res[pname] = geom.unpack_spinor(t)
+        out[name] = {p: _err(res[p], res["f64"]) for p, _ in precs[name]}
+        del res
+    del U, inv, psi
     torch.cuda.empty_cache()
-    return {"max_abs": d, "max_abs_rel": d / m}
+    return out
 
 
 def copy_bandwidth(dev, nbytes=1 << 30, reps=10):
@@ -295,7 +361,8 @@ def cg_compare(q, dev):
         x, res, sec = _solve(q, geom, op, bf, prec != q.F64)
         sols[name] = x.data
         out[name] = {"iters": res["iters"], "seconds": round(sec, 4),
-                     "reliable_updates": res["reliable_updates"], "rsd_final": res["rsd_final"]}
+                     "reliable_updates": res["reliable_updates"], "rsd_final": res["rsd_final"],
+                     "rsd_true_recomputed": true_residual(q, geom, op, x, bf)}
         op.close()
     for name in ("f16", "bf16"):
         out[name]["speedup_vs_f64"] = round(out["f64"]["seconds"] / out[name]["seconds"], 3)
@@ -306,16 +373,49 @@ def cg_compare(q, dev):
     return out
 
 
-def tmclover48(args, q, dev):
+def true_residual(q, geom, op, x, b):
+    """|b - M^dag M x| / |b| recomputed in FP64 through the operator's own
+    FP64 application (outside the CG's loop)."""
+    mx, mdmx = geom.new_spinor(q.F64, cb=1), geom.new_spinor(q.F64, cb=1)
+    op.apply(mx, x)
+    op.apply(mdmx, mx, dagger=True)
+    return float(((b.data - mdmx.data).norm() / b.data.norm()).item())
+
+
+FORCE_REPEATS = 3
+
+
+def _force(q, op, U, x, csw):
+    """The whole pseudofermion force at x (hopping + clover terms, FP64 force
+    kernels): one call that sizes the operator's workspaces, then the median
+    wall time of FORCE_REPEATS calls (each synchronises on return)."""
+    import torch
+    F = op.force(torch.zeros_like(U), x, U, project=True, csw=csw)
+    times = []
+    for _ in range(FORCE_REPEATS):
+        G = torch.zeros_like(U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.force(G, x, U, project=True, csw=csw)
+        times.append(time.perf_counter() - t0)
+        del G
+    return F, sorted(times)[len(times) // 2]
+
+
+def cfg4(q, dev, cg=True):
     """The 48^3x96 (5,308,416 sites/cb) twisted-mass clover workload, mu = 0.05,
-    kappa = 0.1355, csw = 1.5, links sigma = 0.2.
-      * dslash throughput in half storage (FP16 spinors, 12-real FP16 links,
-        FP16 full clover blocks, FP32 arithmetic) and in FP32;
+    kappa = 0.1355, csw = 1.5 (clover from the links), links sigma = 0.2.
+      * tm_clov_dslash and its achimbdpsi form in half storage (FP16 spinors,
+        12-real FP16 links, FP16 full clover blocks, FP32 arithmetic) and in
+        FP32: HIP-event time, GFLOP/s, fraction of 8 TB/s by algorithmic
+        bytes, and the per-site error against the FP64 kernel;
       * FP64 CG vs mixed-precision CG (FP16 storage) to a true residual of
         1e-10 on M^dag M of the even-odd TM-clover operator (median of
-        CG_REPEATS solves);
-      * the pseudofermion force at the solution (hopping and clover terms):
-        from the mixed solve vs from the FP64 solve (FP64 force kernels)."""
+        CG_REPEATS solves), each solution's true residual recomputed in FP64
+        outside the solver, and the two solutions' difference;
+      * the pseudofermion force at each solution (hopping and clover terms,
+        median of FORCE_REPEATS calls), and the force from the mixed
+        solution against the force from the FP64 solution."""
     import torch
     L = q.lattice
     dims = (48, 48, 48, 96)
@@ -326,60 +426,71 @@ def tmclover48(args, q, dev):
     U = L.random_gauge(dims, 0.2, 4000, device=dev)
     A = q.clover_from_links(geom, U, alpha, csw)
     phi = L.random_spinor(dims, 4001, device=dev)
-    out = {"workload": "48^3x96 twisted-mass clover, mu=0.05, kappa=0.1355, csw=1.5",
-           "sites_per_cb": vcb}
-    # --- dslash throughput
+    out = {"workload": "48^3x96 twisted-mass clover, mu=0.05, kappa=0.1355, csw=1.5 (clover from links), "
+                       "sigma=0.2", "sites_per_cb": vcb}
+    # --- dslash throughput and error against FP64
     flop_d, flop_a = 1872, 1920
     op_k = q.TMClovDslash(geom)
-    for name, prec in (("f16", q.F16), ("f32", q.F32)):
-        B = q.tmclover_blocks(geom, A, mu, 1, True)
-        F = q.tmclover_blocks(geom, A, mu, 1, False)
+    B = q.tmclover_blocks(geom, A, mu, 1, True)
+    F = q.tmclover_blocks(geom, A, mu, 1, False)
+    res = {}
+    for name, prec in (("f64", q.F64), ("f16", q.F16), ("f32", q.F32)):
         f = dict(g0=geom.pack_gauge(U, prec, 12, 0), g1=geom.pack_gauge(U, prec, 12, 1),
                  B0=geom.pack_full_clover(B, prec, 0), F1=geom.pack_full_clover(F, prec, 1),
                  psi=geom.pack_spinor(phi, prec, 1), t=geom.new_spinor(prec, cb=0),
                  o=geom.new_spinor(prec, cb=1))
-        del B, F
+        op_k.dslash(f["t"], f["psi"], f["g0"], f["B0"], 1, 0)
+        res[name] = geom.unpack_spinor(f["t"])
+        if name != "f64":
+            def kd(i):
+                op_k.dslash(f["t"], f["psi"], f["g0"], f["B0"], 1, 0)
 
-        def kd(i):
-            op_k.dslash(f["t"], f["psi"], f["g0"], f["B0"], 1, 0)
+            def ka(i):
+                op_k.dslashAChiMinusBDPsi(f["o"], f["t"], f["psi"], f["g1"], f["F1"], 0.25, 1, 1)
 
-        def ka(i):
-            op_k.dslashAChiMinusBDPsi(f["o"], f["t"], f["psi"], f["g1"], f["F1"], 0.25, 1, 1)
-
-        for k in (kd, ka):
-            for i in range(3):
-                k(i)
-        td, ta = time_kernel(kd, 50), time_kernel(ka, 50)
-        sb = 2 if prec == q.F16 else 4
-        bd = (24 + 96 + 144 + 24) * sb * vcb
-        ba = (24 + 96 + 144 + 24 + 24) * sb * vcb
-        out[f"dslash_{name}"] = {
-            "tm_clov_dslash_us": round(td * 1e6, 1), "tm_clov_dslash_gflops": round(vcb * flop_d / td / 1e9, 1),
-            "tm_clov_dslash_hbm_frac": round(bd / td / 1e9 / PEAK_HBM_GBS, 4),
-            "tm_clov_achimbdpsi_us": round(ta * 1e6, 1),
-            "tm_clov_achimbdpsi_gflops": round(vcb * flop_a / ta / 1e9, 1),
-            "tm_clov_achimbdpsi_hbm_frac": round(ba / ta / 1e9 / PEAK_HBM_GBS, 4)}
+            for k in (kd, ka):
+                for i in range(3):
+                    k(i)
+            td, ta = time_kernel(kd, 50), time_kernel(ka, 50)
+            sb = 2 if prec == q.F16 else 4
+            bd = (24 + 96 + 144 + 24) * sb * vcb
+            ba = (24 + 96 + 144 + 24 + 24) * sb * vcb
+            out[f"dslash_{name}"] = {
+                "tm_clov_dslash_us": round(td * 1e6, 1),
+                "tm_clov_dslash_gflops": round(vcb * flop_d / td / 1e9, 1),
+                "tm_clov_dslash_hbm_frac": round(bd / td / 1e9 / PEAK_HBM_GBS, 4),
+                "tm_clov_achimbdpsi_us": round(ta * 1e6, 1),
+                "tm_clov_achimbdpsi_gflops": round(vcb * flop_a / ta / 1e9, 1),
+                "tm_clov_achimbdpsi_hbm_frac": round(ba / ta / 1e9 / PEAK_HBM_GBS, 4),
+                "tm_clov_dslash_error_vs_f64": _err(res[name], res["f64"])}
         del f
         torch.cuda.empty_cache()
+    del B, F, res
+    torch.cuda.empty_cache()
+    if not cg:
+        return out
     # --- CG: FP64 vs mixed (FP16 storage), then the force at each solution
     b = geom.pack_spinor(phi, q.F64, 1)
-    forces = {}
+    forces, sols = {}, {}
     for name, prec in (("f64", q.F64), ("f16", q.F16)):
         op = q.EvenOddCloverOperator(geom, U, A, mu=mu, prec_sloppy=prec, recon_sloppy=12)
-        x, res, sec = _solve(q, geom, op, b, prec != q.F64)
-        out[f"cg_{name}"] = {"iters": res["iters"], "seconds": round(sec, 4),
-                             "reliable_updates": res["reliable_updates"], "rsd_final": res["rsd_final"]}
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        forces[name] = op.force(torch.zeros_like(U), x, U, project=True, csw=csw)
-        torch.cuda.synchronize()
-        out[f"cg_{name}"]["force_seconds"] = round(time.perf_counter() - t0, 4)
+        x, r, sec = _solve(q, geom, op, b, prec != q.F64)
+        out[f"cg_{name}"] = {"iters": r["iters"], "seconds": round(sec, 4),
+                             "reliable_updates": r["reliable_updates"], "rsd_final": r["rsd_final"],
+                             "rsd_true_recomputed": true_residual(q, geom, op, x, b)}
+        forces[name], ft = _force(q, op, U, x, csw)
+        out[f"cg_{name}"]["force_seconds"] = round(ft, 4)
+        sols[name] = x.data
         op.close()
         del x
         torch.cuda.empty_cache()
     out["cg_f16"]["speedup_vs_f64"] = round(out["cg_f64"]["seconds"] / out["cg_f16"]["seconds"], 3)
+    out["cg_f16"]["rel_err_vs_f64_solution"] = float(
+        ((sols["f16"] - sols["f64"]).norm() / sols["f64"].norm()).item())
     out["force_rel_err_mixed_vs_f64"] = float(
         ((forces["f16"] - forces["f64"]).abs().max() / forces["f64"].abs().max()).item())
+    del forces, sols, U, A, phi, b
+    torch.cuda.empty_cache()
     return out
 
 
@@ -389,10 +500,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG solves (32^3x64 and 48^3x96)")
+    ap.add_argument("--no-cfg4", action="store_true", help="skip the 48^3x96 twisted-mass clover block")
     ap.add_argument("--workload", default="clover32", choices=["clover32", "tmclover48"],
-                    help="clover32: the 32^3x64 clover operator (the headline line); "
-                         "tmclover48: the 48^3x96 twisted-mass clover case")
+                    help="clover32: the 32^3x64 clover operator (the headline line, with the "
+                         "48^3x96 block); tmclover48: the 48^3x96 twisted-mass clover block alone")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch clov_dslash, clov_dslash_achimbdpsi and Wilson dslash 10 times "
                          "each plus a 1 GiB device copy (for rocprofv3 --pmc runs and their "
@@ -401,7 +513,13 @@ def main():
                     help="with --kernels-only: Lx,Ly,Lz,Lt (default 32,32,32,64)")
     ap.add_argument("--traversal", type=int, default=-1,
                     help="with --kernels-only: work-group traversal order (-1 automatic)")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_child:  # the CPU-baseline leg's own process (no torch)
+        dims = tuple(int(v) for v in args.lattice.split(",")) if args.lattice else DIMS
+        print(json.dumps(_cpu_baseline_child(args.cpu_seconds, dims)), flush=True)
+        return
     if (args.lattice or args.traversal != -1) and not args.kernels_only:
         print("--lattice and --traversal apply to --kernels-only runs", file=sys.stderr)
         sys.exit(2)
@@ -417,7 +535,7 @@ def main():
     dev = "cuda:0"
     torch.cuda.set_device(0)
     if args.workload == "tmclover48":
-        print(json.dumps(tmclover48(args, q, dev)), flush=True)
+        print(json.dumps(cfg4(q, dev, cg=not args.no_cg)), flush=True)
         return
     prec, recon = q.F32, 12
     dims = tuple(int(v) for v in args.lattice.split(",")) if args.lattice else DIMS
@@ -483,7 +601,7 @@ def main():
     t_d = time_kernel(k_dslash, args.steps)
     t_a = time_kernel(k_achi, args.steps)
     tm = tm_kernel(q, geom, sets[0], dev, args.steps)
-    err = f32_error_vs_f64(q, geom, sets[0], dev)
+    err = errors_vs_f64(q, geom, dev)
     # naive count (every neighbour spinor read from memory), as often quoted
     b_naive = (8 * 24 + 8 * recon + 72 + 24) * BYTES_F32 * vcb
     b_d = algorithmic_bytes(recon, 72, False, BYTES_F32) * vcb
@@ -518,11 +636,13 @@ def main():
                                 "clov_dslash_achimbdpsi": round(b_a / t_a / 1e9 / PEAK_HBM_GBS, 4)},
             "parallelism": "single GPU",
             "tm_dslash": tm,
-            "clov_dslash_f32_error_vs_f64": err,
+            "errors_vs_f64": err,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": profile_traffic(),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, "
+                                       "profiles/clov_dslash_f32_32x64_pmc.txt (not measured in this run)",
                      "kernel": "clov_dslash f32 recon12",
                      "algorithmic_bytes_per_launch": b_d,
                      "naive_bytes_per_launch": b_naive,
@@ -537,6 +657,8 @@ def main():
     if not args.no_cg:
         line["cg"] = cg_compare(q, dev)
         line["cg"]["sloppy_kernels"] = sloppy_kernels(q, dev)
+    if not args.no_cfg4:
+        line["cfg4"] = cfg4(q, dev, cg=not args.no_cg)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

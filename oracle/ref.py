@@ -37,12 +37,20 @@ def lib(config):
         l.ref_pack_field.argtypes = [i, i, i, i, i, i, i, _dp, ctypes.c_void_p]
         l.ref_pack_field.restype = ctypes.c_long
         l.ref_soalen.restype = i
+        l.ref_stream_triad.argtypes = [ctypes.c_long, i, i]
+        l.ref_stream_triad.restype = d
         _libs[config] = l
     return _libs[config]
 
 
 def soalen(config):
     return lib(config).ref_soalen()
+
+
+def stream_triad(config, nthreads, n=1 << 25, reps=5):
+    """Host STREAM triad (GB/s) on nthreads OpenMP threads of the same
+    runtime the reference's kernels run under (three FP64 arrays of n)."""
+    return lib(config).ref_stream_triad(int(n), int(nthreads), int(reps))
 
 
 def _p(a):
@@ -54,8 +62,10 @@ def _p(a):
 def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=None, recon=18,
           coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.0, t_boundary=-1.0, alpha=0.0, mu=0.0,
           muinv=0.0, rho=0.0, nthreads=1, reps=0, tbc_phases=None):
-    """Run one reference kernel over checkerboard cb.  Returns (out, seconds
-    per repetition or None); out is (V,4,3,2) with the cb sites set.
+    """Run one reference kernel over checkerboard cb.  Returns (out, times):
+    out is (V,4,3,2) with the cb sites set; with reps > 0 the block loop
+    (not the packing) runs reps more times and times holds each call's
+    seconds (a numpy array), else None.
     tbc_phases: four complex forward-hop phases (x, y, z, t) -> the bodies
     generated with twisted boundary conditions in every dimension."""
     Lx, Ly, Lz, Lt = dims
@@ -63,16 +73,16 @@ def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=No
     c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
     psi, U, chi, clov = c(psi), c(U), c(chi), c(clov)
     out = np.zeros((V, 4, 3, 2))
-    sec = ctypes.c_double(0.0)
+    sec = np.zeros(max(int(reps), 1))
     ph = None
     if tbc_phases is not None:
         ph = np.array([v for z in tbc_phases for v in (complex(z).real, complex(z).imag)], dtype=np.float64)
     rc = lib(config).ref_apply(kind, int(achi), isign, recon, cb, Lx, Ly, Lz, Lt, _p(psi), _p(chi),
                                _p(U), _p(clov), _p(out), coeff_s, coeff_t_f, coeff_t_b, t_boundary,
-                               alpha, mu, muinv, rho, nthreads, reps, ctypes.byref(sec), _p(ph))
+                               alpha, mu, muinv, rho, nthreads, reps, _p(sec), _p(ph))
     if rc != 0:
         raise ValueError(f"reference kernel rejected the call (config {config}, dims {dims})")
-    return out, (sec.value if reps > 0 else None)
+    return out, (sec if reps > 0 else None)
 
 
 def pack(config, field, dims, src, cb, recon=18):

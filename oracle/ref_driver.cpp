@@ -262,6 +262,40 @@ long ref_pack_field(int field, int recon, int cb, int Lx, int Ly, int Lz, int Lt
   return n;
 }
 
+// STREAM triad c = a + 1.5 b over three FP64 arrays of n elements on
+// nthreads OpenMP threads (arrays first touched by the same static schedule,
+// so each thread's pages sit on its own memory node): the host's roofline
+// for the kernels above.  Returns the best of reps in GB/s, counting 24
+// bytes per element; -1 if the arrays cannot be allocated.
+double ref_stream_triad(long n, int nthreads, int reps)
+{
+  double *a = (double *)aligned_alloc(64, ((n * 8 + 63) / 64) * 64);
+  double *b = (double *)aligned_alloc(64, ((n * 8 + 63) / 64) * 64);
+  double *c = (double *)aligned_alloc(64, ((n * 8 + 63) / 64) * 64);
+  double best = -1.0;
+  if (a && b && c) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long i = 0; i < n; i++) {
+      a[i] = 1.0;
+      b[i] = 2.0;
+      c[i] = 0.0;
+    }
+    double tmin = 1e300;
+    for (int r = 0; r < reps; r++) {
+      const double t0 = omp_get_wtime();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+      for (long i = 0; i < n; i++) c[i] = a[i] + 1.5 * b[i];
+      const double dt = omp_get_wtime() - t0;
+      tmin = dt < tmin ? dt : tmin;
+    }
+    best = 24.0 * (double)n / tmin / 1e9;
+  }
+  free(a);
+  free(b);
+  free(c);
+  return best;
+}
+
 const char *ref_config(void) { return REF_CONFIG_NAME; }
 int ref_veclen(void) { return VECLEN; }
 int ref_soalen(void) { return SOALEN; }
@@ -278,7 +312,8 @@ int ref_lowprec(void)
 // kind: 0 wilson, 1 clover, 2 tm, 3 tm-clover.  Inputs are lexicographic
 // FP64 arrays; psi is read on checkerboard 1-cb, chi on cb, and out is
 // written on cb.  Returns 0, or -1 for an unsupported geometry/argument.
-// reps > 0 times the block loop (not the packing) and stores seconds/rep.
+// reps > 0 times the block loop (not the packing) reps more times and
+// stores each call's seconds in seconds[0 .. reps-1].
 // tbc_phases (8 doubles, (re, im) per dimension x y z t) selects the bodies
 // generated with twisted boundary conditions in every dimension and passes
 // the phases (codegen/dslash.cc:206-230); NULL selects the simple bodies.
@@ -372,10 +407,10 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
     }
   };
   run();
-  if (reps > 0) {
-    double t0 = omp_get_wtime();
-    for (int r = 0; r < reps; r++) run();
-    if (seconds) *seconds = (omp_get_wtime() - t0) / reps;
+  for (int r = 0; r < reps; r++) {
+    const double t0 = omp_get_wtime();
+    run();
+    if (seconds) seconds[r] = omp_get_wtime() - t0;
   }
   unpack_spinor(out, o, g, cb);
   free(in);
