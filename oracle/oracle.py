@@ -43,6 +43,10 @@ def lib():
                                           _dp, _dp, i, i, i, i, i, i, i, d, d, d, d, _ip, _dp, d, d,
                                           d, d]
         l.oracle_force.argtypes = [_dp, _dp, _dp, _dp, i, i, i, i, i, i, d, d, d, d, _ip, _dp, i]
+        l.oracle_force_links.argtypes = [_dp, ctypes.POINTER(ctypes.c_long), ctypes.c_long, _dp, _dp, _dp,
+                                         i, i, i, i, i, i, d, d, d, d, _ip, _dp, i]
+        l.oracle_clover_from_links_sites.argtypes = [_dp, ctypes.POINTER(ctypes.c_long), ctypes.c_long,
+                                                     _dp, i, i, i, i, d, d]
         l.oracle_clover_apply.argtypes = [_dp, _dp, _dp, i, i, i, i, i, i]
         l.oracle_clover_from_links.argtypes = [_dp, _dp, i, i, i, i, d, d]
         l.oracle_clover_invert.argtypes = [_dp, _dp, ctypes.c_long]
@@ -122,6 +126,37 @@ def force(dims, chi, psi, U, isign, cb, coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.
     t, p = _bc(tbc, tbc_phases)
     lib().oracle_force(po, pc, pp, pu, Lx, Ly, Lz, Lt, isign, cb, coeff_s, coeff_t_f, coeff_t_b,
                        t_boundary, t, p, int(bool(project)))
+    return out
+
+
+def force_links(dims, links, chi, psi, U, isign, cb, coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.0,
+                t_boundary=-1.0, tbc=None, tbc_phases=None, project=True):
+    """force() at the listed links only (link = 4 * lexicographic site + mu)
+    -> (n,3,3,2).  Inputs are full-lattice arrays; only the sites the listed
+    links' hops touch are read."""
+    Lx, Ly, Lz, Lt = dims
+    links = np.ascontiguousarray(links, dtype=np.int64)
+    chi, pc = _c(chi)
+    psi, pp = _c(psi)
+    U, pu = _c(U)
+    out = np.zeros((len(links), 3, 3, 2))
+    out, po = _c(out)
+    t, p = _bc(tbc, tbc_phases)
+    lib().oracle_force_links(po, links.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), len(links), pc, pp,
+                             pu, Lx, Ly, Lz, Lt, isign, cb, coeff_s, coeff_t_f, coeff_t_b, t_boundary, t,
+                             p, int(bool(project)))
+    return out
+
+
+def clover_from_links_sites(dims, sites, U, alpha, csw):
+    """clover_from_links() at the listed lexicographic sites only -> (n,2,36)."""
+    Lx, Ly, Lz, Lt = dims
+    sites = np.ascontiguousarray(sites, dtype=np.int64)
+    U, pu = _c(U)
+    out = np.zeros((len(sites), 2, 36))
+    out, po = _c(out)
+    lib().oracle_clover_from_links_sites(po, sites.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                         len(sites), pu, Lx, Ly, Lz, Lt, alpha, csw)
     return out
 
 

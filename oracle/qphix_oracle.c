@@ -445,13 +445,11 @@ static void field_strength(const double *U, const lat_t *g, long s, int mu,
   for (int i = 0; i < 3; i++) F[i][i] -= tr;
 }
 
-void oracle_clover_from_links(double *clov, const double *U, int Lx, int Ly,
-                              int Lz, int Lt, double alpha, double csw)
+/* A(x) at site s: alpha + (csw/2) sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
+   sigma_{mu nu} = (i/2)(g_mu g_nu - g_nu g_mu), stored packed at blk. */
+static void clover_site(double *blk0, const double *U, const lat_t *g, long s, double alpha,
+                        double csw)
 {
-  lat_t g;
-  lat_init(&g, Lx, Ly, Lz, Lt);
-  init_gamma();
-  /* sigma_{mu nu} = (i/2)(g_mu g_nu - g_nu g_mu) */
   cplx sigma[4][4][4][4];
   for (int mu = 0; mu < 4; mu++)
     for (int nu = 0; nu < 4; nu++)
@@ -463,33 +461,50 @@ void oracle_clover_from_links(double *clov, const double *U, int Lx, int Ly,
                  gamma_mat[nu][a][k] * gamma_mat[mu][k][b];
           sigma[mu][nu][a][b] = 0.5 * I * v;
         }
-  for (long s = 0; s < g.V; s++) {
-    cplx A[2][6][6] = {{{0}}};
-    for (int b = 0; b < 2; b++)
-      for (int k = 0; k < 6; k++) A[b][k][k] = alpha;
-    for (int mu = 0; mu < 4; mu++)
-      for (int nu = mu + 1; nu < 4; nu++) {
-        cplx F[3][3];
-        field_strength(U, &g, s, mu, nu, F);
-        for (int b = 0; b < 2; b++)
-          for (int r = 0; r < 6; r++)
-            for (int c = 0; c < 6; c++) {
-              int s1 = 2 * b + r / 3, s2 = 2 * b + c / 3;
-              A[b][r][c] += 0.5 * csw * sigma[mu][nu][s1][s2] * F[r % 3][c % 3];
-            }
-      }
-    for (int b = 0; b < 2; b++) {
-      double *blk = clov + s * 72 + b * 36;
-      for (int r = 0; r < 6; r++) {
-        blk[r] = creal(A[b][r][r]);
-        for (int c = 0; c < r; c++) {
-          int od = r * (r - 1) / 2 + c;
-          blk[6 + 2 * od] = creal(A[b][r][c]);
-          blk[6 + 2 * od + 1] = cimag(A[b][r][c]);
-        }
+  cplx A[2][6][6] = {{{0}}};
+  for (int b = 0; b < 2; b++)
+    for (int k = 0; k < 6; k++) A[b][k][k] = alpha;
+  for (int mu = 0; mu < 4; mu++)
+    for (int nu = mu + 1; nu < 4; nu++) {
+      cplx F[3][3];
+      field_strength(U, g, s, mu, nu, F);
+      for (int b = 0; b < 2; b++)
+        for (int r = 0; r < 6; r++)
+          for (int c = 0; c < 6; c++) {
+            int s1 = 2 * b + r / 3, s2 = 2 * b + c / 3;
+            A[b][r][c] += 0.5 * csw * sigma[mu][nu][s1][s2] * F[r % 3][c % 3];
+          }
+    }
+  for (int b = 0; b < 2; b++) {
+    double *blk = blk0 + b * 36;
+    for (int r = 0; r < 6; r++) {
+      blk[r] = creal(A[b][r][r]);
+      for (int c = 0; c < r; c++) {
+        int od = r * (r - 1) / 2 + c;
+        blk[6 + 2 * od] = creal(A[b][r][c]);
+        blk[6 + 2 * od + 1] = cimag(A[b][r][c]);
       }
     }
   }
+}
+
+void oracle_clover_from_links(double *clov, const double *U, int Lx, int Ly,
+                              int Lz, int Lt, double alpha, double csw)
+{
+  lat_t g;
+  lat_init(&g, Lx, Ly, Lz, Lt);
+  init_gamma();
+  for (long s = 0; s < g.V; s++) clover_site(clov + s * 72, U, &g, s, alpha, csw);
+}
+
+/* The same at the listed lexicographic sites only: clov[n][72]. */
+void oracle_clover_from_links_sites(double *clov, const long *sites, long nsites, const double *U,
+                                    int Lx, int Ly, int Lz, int Lt, double alpha, double csw)
+{
+  lat_t g;
+  lat_init(&g, Lx, Ly, Lz, Lt);
+  init_gamma();
+  for (long k = 0; k < nsites; k++) clover_site(clov + k * 72, U, &g, sites[k], alpha, csw);
 }
 
 /* Dense 6x6 complex inverse by Gauss-Jordan with partial pivoting. */
@@ -579,6 +594,80 @@ int oracle_tmclover_blocks(double *out, const double *clov, long nsites,
  *   project = 1: TA(U G) (traceless anti-Hermitian, d/de L(e^{ieT}U) =
  *                Re tr(i T F)),  project = 0: G^dagger.
  * ---------------------------------------------------------------------- */
+/* G of link (y, mu): the one hop term that uses it.  y on cb: the forward
+   hop at x = y (n = y + mu); y on 1-cb: the backward hop at x = y + mu
+   (n = y).  Gl[i*3+j] is G[i][j] with L = sum_links Re tr(U G). */
+static void force_link(cplx Gl[9], const double *chi, const double *psi, const lat_t *g, long y,
+                       int mu, int isign, int cb, double coeff_s, double coeff_t_f,
+                       double coeff_t_b, double t_boundary, const int *tbc,
+                       const double *tbc_phases)
+{
+  int cy[4];
+  site_coords(g, y, cy);
+  const int fb = ((cy[0] + cy[1] + cy[2] + cy[3]) & 1) == cb;
+  const long x = fb ? y : neighbour(g, y, mu, +1);
+  const long n = fb ? neighbour(g, y, mu, +1) : y;
+  int c[4];
+  site_coords(g, x, c);
+  cplx ph = 1.0;
+  if (tbc && tbc[mu]) ph = tbc_phases[2 * mu] + I * tbc_phases[2 * mu + 1];
+  double beta;
+  if (mu < 3)
+    beta = coeff_s;
+  else if (fb)
+    beta = coeff_t_f * (c[3] == g->L[3] - 1 ? t_boundary : 1.0);
+  else
+    beta = coeff_t_b * (c[3] == 0 ? t_boundary : 1.0);
+  double sg = fb ? -(double)isign : (double)isign;
+  /* eta = (1 + sg g_mu)^+ chi(x) = (1 + sg g_mu) chi(x) (g Hermitian) */
+  cplx eta[4][3];
+  for (int sp = 0; sp < 4; sp++)
+    for (int i = 0; i < 3; i++) {
+      cplx v = spin(chi, x, sp, i);
+      for (int k = 0; k < 4; k++) v += sg * gamma_mat[mu][sp][k] * spin(chi, x, k, i);
+      eta[sp][i] = v;
+    }
+  /* term = beta * sum_s sum_ij conj(eta_si) V_ij psi(n)_sj with
+     V = ph U_mu(x) (fb = 1) or conj(ph) U_mu(n)^+ (fb = 0).
+     fb = 1: term = tr(U_mu(x) Gx), Gx[j][i] += beta ph psi_j conj(eta_i)
+     fb = 0: V_ij = conj(ph) conj(U_mu(n)_ji):
+             Re term = Re tr(U_mu(n) Gn), Gn[i][j] += beta ph eta_i conj(psi_j) */
+  for (int k = 0; k < 9; k++) Gl[k] = 0;
+  for (int sp = 0; sp < 4; sp++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        if (fb)
+          Gl[j * 3 + i] += beta * ph * spin(psi, n, sp, j) * conj(eta[sp][i]);
+        else
+          Gl[i * 3 + j] += beta * ph * eta[sp][i] * conj(spin(psi, n, sp, j));
+      }
+}
+
+/* Output of one link from its G: G^dagger, or TA(U G). */
+static void force_out(double *out, const cplx Gl[9], const double *U, long s, int mu, int project)
+{
+  if (!project) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) st(out + (i * 3 + j) * 2, conj(Gl[j * 3 + i]));
+    return;
+  }
+  cplx W[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      cplx v = 0;
+      for (int k = 0; k < 3; k++) v += link(U, s, mu, i, k) * Gl[k * 3 + j];
+      W[i][j] = v;
+    }
+  cplx tr = 0;
+  for (int i = 0; i < 3; i++) tr += W[i][i] - conj(W[i][i]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      cplx v = 0.5 * (W[i][j] - conj(W[j][i]));
+      if (i == j) v -= tr / 6.0;
+      st(out + (i * 3 + j) * 2, v);
+    }
+}
+
 void oracle_force(double *F, const double *chi, const double *psi, const double *U,
                   int Lx, int Ly, int Lz, int Lt, int isign, int cb, double coeff_s,
                   double coeff_t_f, double coeff_t_b, double t_boundary,
@@ -587,74 +676,28 @@ void oracle_force(double *F, const double *chi, const double *psi, const double 
   lat_t g;
   lat_init(&g, Lx, Ly, Lz, Lt);
   init_gamma();
-  cplx *G = (cplx *)calloc((size_t)g.V * 4 * 9, sizeof(cplx));
-  for (long x = 0; x < g.V; x++) {
-    int c[4];
-    site_coords(&g, x, c);
-    if (((c[0] + c[1] + c[2] + c[3]) & 1) != cb) continue;
-    for (int mu = 0; mu < 4; mu++) {
-      cplx ph = 1.0;
-      if (tbc && tbc[mu]) ph = tbc_phases[2 * mu] + I * tbc_phases[2 * mu + 1];
-      for (int fb = 0; fb < 2; fb++) {
-        long n = neighbour(&g, x, mu, fb ? +1 : -1);
-        double beta;
-        if (mu < 3)
-          beta = coeff_s;
-        else if (fb)
-          beta = coeff_t_f * (c[3] == Lt - 1 ? t_boundary : 1.0);
-        else
-          beta = coeff_t_b * (c[3] == 0 ? t_boundary : 1.0);
-        double sg = fb ? -(double)isign : (double)isign;
-        /* eta = (1 + sg g_mu)^+ chi(x) = (1 + sg g_mu) chi(x) (g Hermitian) */
-        cplx eta[4][3];
-        for (int sp = 0; sp < 4; sp++)
-          for (int i = 0; i < 3; i++) {
-            cplx v = spin(chi, x, sp, i);
-            for (int k = 0; k < 4; k++) v += sg * gamma_mat[mu][sp][k] * spin(chi, x, k, i);
-            eta[sp][i] = v;
-          }
-        /* term = beta * sum_s sum_ij conj(eta_si) V_ij psi(n)_sj with
-           V = ph U_mu(x) (fb = 1) or conj(ph) U_mu(n)^+ (fb = 0).
-           fb = 1: term = tr(U_mu(x) Gx), Gx[j][i] += beta ph psi_j conj(eta_i)
-           fb = 0: V_ij = conj(ph) conj(U_mu(n)_ji):
-                   Re term = Re tr(U_mu(n) Gn), Gn[i][j] += beta ph eta_i conj(psi_j) */
-        long l = fb ? x : n;
-        for (int sp = 0; sp < 4; sp++)
-          for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-              if (fb)
-                G[(l * 4 + mu) * 9 + j * 3 + i] += beta * ph * spin(psi, n, sp, j) * conj(eta[sp][i]);
-              else
-                G[(l * 4 + mu) * 9 + i * 3 + j] += beta * ph * eta[sp][i] * conj(spin(psi, n, sp, j));
-            }
-      }
-    }
-  }
   for (long l = 0; l < g.V * 4; l++) {
-    cplx *Gl = G + l * 9;
-    double *out = F + l * 18;
-    if (!project) {
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) st(out + (i * 3 + j) * 2, conj(Gl[j * 3 + i]));
-      continue;
-    }
-    long s = l / 4;
-    int mu = (int)(l % 4);
-    cplx W[3][3];
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) {
-        cplx v = 0;
-        for (int k = 0; k < 3; k++) v += link(U, s, mu, i, k) * Gl[k * 3 + j];
-        W[i][j] = v;
-      }
-    cplx tr = 0;
-    for (int i = 0; i < 3; i++) tr += W[i][i] - conj(W[i][i]);
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) {
-        cplx v = 0.5 * (W[i][j] - conj(W[j][i]));
-        if (i == j) v -= tr / 6.0;
-        st(out + (i * 3 + j) * 2, v);
-      }
+    cplx Gl[9];
+    force_link(Gl, chi, psi, &g, l / 4, (int)(l % 4), isign, cb, coeff_s, coeff_t_f, coeff_t_b,
+               t_boundary, tbc, tbc_phases);
+    force_out(F + l * 18, Gl, U, l / 4, (int)(l % 4), project);
   }
-  free(G);
+}
+
+/* The same for the listed links only (link = 4 * site + mu): F[n][18]. */
+void oracle_force_links(double *F, const long *links, long nlinks, const double *chi,
+                        const double *psi, const double *U, int Lx, int Ly, int Lz, int Lt,
+                        int isign, int cb, double coeff_s, double coeff_t_f, double coeff_t_b,
+                        double t_boundary, const int *tbc, const double *tbc_phases, int project)
+{
+  lat_t g;
+  lat_init(&g, Lx, Ly, Lz, Lt);
+  init_gamma();
+  for (long k = 0; k < nlinks; k++) {
+    const long l = links[k];
+    cplx Gl[9];
+    force_link(Gl, chi, psi, &g, l / 4, (int)(l % 4), isign, cb, coeff_s, coeff_t_f, coeff_t_b,
+               t_boundary, tbc, tbc_phases);
+    force_out(F + k * 18, Gl, U, l / 4, (int)(l % 4), project);
+  }
 }

@@ -90,3 +90,27 @@ def test_torch_clover_restatement_equals_oracle_builder(qamd, orc):
     assert A[:, :6, 6:].abs().max().item() < 1e-15
     ref = orc.clover_from_links(dims, U.numpy(), 3.7, 1.3)
     assert np.abs(ct.pack_blocks(A).numpy() - ref).max() < 1e-14
+
+
+def test_oracle_sampled_entry_points_equal_whole_lattice(qamd, orc):
+    """oracle.force_links and oracle.clover_from_links_sites (used by the
+    48^3x96 GPU checks, where whole-lattice oracle calls are too large) give
+    exactly the whole-lattice results at the listed links / sites, with
+    anisotropic coefficients, antiperiodic time and twisted phases."""
+    L = qamd.lattice
+    dims = (4, 6, 4, 8)
+    U = L.random_gauge(dims, 0.3, 71).numpy()
+    chi = L.random_spinor(dims, 72).numpy()
+    psi = L.random_spinor(dims, 73).numpy()
+    V = U.shape[0]
+    rng = np.random.default_rng(7)
+    links = rng.choice(4 * V, 200, replace=False)
+    kw = dict(coeff_s=0.9, coeff_t_f=1.3, coeff_t_b=0.7, t_boundary=-1.0, tbc=(1, 0, 1, 1),
+              tbc_phases=(np.exp(0.3j), 1, np.exp(-0.7j), np.exp(1.1j)))
+    for isign, cb, project in ((1, 0, True), (-1, 1, True), (1, 1, False)):
+        full = orc.force(dims, chi, psi, U, isign, cb, project=project, **kw).reshape(4 * V, 3, 3, 2)
+        part = orc.force_links(dims, links, chi, psi, U, isign, cb, project=project, **kw)
+        assert np.array_equal(part, full[links])
+    sites = rng.choice(V, 100, replace=False)
+    A = orc.clover_from_links(dims, U, 3.7, 1.5)
+    assert np.array_equal(orc.clover_from_links_sites(dims, sites, U, 3.7, 1.5), A[sites])
