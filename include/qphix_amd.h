@@ -42,6 +42,7 @@ extern "C" {
 #define QPHIX_AMD_ENOMEM -4     /* device allocation failed                */
 #define QPHIX_AMD_ENOCONV -5    /* solver did not converge                 */
 #define QPHIX_AMD_EALIGN -6     /* pointer not 16-byte aligned             */
+#define QPHIX_AMD_ESCRATCH -7   /* lattice too large for the static scratch */
 
 /* ---- enums ------------------------------------------------------------ */
 /* storage precision; arithmetic is FP64 for F64 and FP32 otherwise
@@ -463,6 +464,24 @@ int qphix_amd_clover_force(qphix_amd_ctx *ctx, double *force_lex,
                            const double *chi_lex, const double *psi_lex,
                            const double *u_lex, double csw, double scale_even,
                            double scale_odd, int project, void *stream);
+
+/* Scratch-free qphix_amd_clover_force: the same arguments and result
+   (the same kernels), with its workspace -- the clover weights, 108 doubles
+   per lattice site -- in a static device array of
+   QPHIX_AMD_NOALLOC_SCRATCH_BYTES (64 MiB) per GPU instead of a per-call
+   hipMalloc, so the call allocates nothing and does not synchronise.  No
+   counterpart in the reference; it is for callers that cannot allocate
+   device memory (code under HIP graph capture, tight HMC loops, harnesses
+   that forbid hipMalloc).  Lattices of more than 77,672 sites (2 * vcb *
+   108 * 8 bytes > 64 MiB; e.g. 16^4 fits, 16^3x32 does not) return
+   QPHIX_AMD_ESCRATCH without launching anything.  Every call on one GPU
+   shares that GPU's array: calls that overlap on one GPU race.           */
+#define QPHIX_AMD_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int qphix_amd_clover_force_noalloc(qphix_amd_ctx *ctx, double *force_lex,
+                                   const double *chi_lex,
+                                   const double *psi_lex, const double *u_lex,
+                                   double csw, double scale_even,
+                                   double scale_odd, int project, void *stream);
 
 /* Solves M^dag M x = b; x, b are FP64 native spinors on cb 1.  Returns
    QPHIX_AMD_ENOCONV (with res filled) when max_iters is reached.         */

@@ -107,6 +107,8 @@ SIGNATURES = {
     "qphix_amd_eo_op_force": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _c_int, _vp]),
     "qphix_amd_eo_op_force_clover": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _c_double, _c_int, _vp]),
     "qphix_amd_clover_force": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_double, _c_double, _c_double, _c_int, _vp]),
+    "qphix_amd_clover_force_noalloc": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_double, _c_double, _c_double, _c_int,
+                                                _vp]),
     "qphix_amd_cg_solve": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(CGParams), ctypes.POINTER(CGResult), _vp]),
 }
 

@@ -174,6 +174,7 @@ const char *qphix_amd_error_string(int code)
     case QPHIX_AMD_ENOMEM: return "out of memory";
     case QPHIX_AMD_ENOCONV: return "solver did not converge";
     case QPHIX_AMD_EALIGN: return "field pointer not 16-byte aligned";
+    case QPHIX_AMD_ESCRATCH: return "lattice too large for the static scratch of a no-allocation call";
   }
   return "unknown error";
 }
