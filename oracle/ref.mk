@@ -22,6 +22,10 @@
 #   avx2_f32    avx2    FP32   8
 #   avx512_f16  avx512  FP16 storage, FP32 arithmetic (ENABLE_LOW_PRECISION,
 #               codegen/CMakeLists.txt:78-83)  16
+# and two xy-packed ones (VECLEN = 2 SOALEN: one gauge vector spans two rows
+# of y; soalens of codegen/jinja/isa.js):
+#   avx512_f64_s4   avx512  FP64   VECLEN 8, SOALEN 4
+#   avx512_f32_s8   avx512  FP32   VECLEN 16, SOALEN 8
 REF ?= /root/reference/codegen
 OUT := $(abspath _ref)
 CXX ?= g++
@@ -42,6 +46,8 @@ DEFS_avx2_f64 := $(YES) -DNO_HW_MASKING -DAVX2 -DPRECISION=2 -DVECLEN=4 -DSOALEN
 DEFS_avx2_f32 := $(YES) -DNO_HW_MASKING -DAVX2 -DPRECISION=1 -DVECLEN=8 -DSOALEN=8
 DEFS_avx512_f16 := $(YES) -DAVX512 -DPRECISION=1 -DVECLEN=16 -DSOALEN=16 -DENABLE_LOW_PRECISION \
                    -DUSE_LP_SPINOR -DUSE_LP_GAUGE -DUSE_LP_CLOVER
+DEFS_avx512_f64_s4 := $(YES) -DAVX512 -DPRECISION=2 -DVECLEN=8 -DSOALEN=4
+DEFS_avx512_f32_s8 := $(YES) -DAVX512 -DPRECISION=1 -DVECLEN=16 -DSOALEN=8
 
 ARCH_scalar_f64 := scalar
 ARCH_avx512_f64 := avx512
@@ -49,6 +55,8 @@ ARCH_avx512_f32 := avx512
 ARCH_avx2_f64 := avx2
 ARCH_avx2_f32 := avx2
 ARCH_avx512_f16 := avx512
+ARCH_avx512_f64_s4 := avx512
+ARCH_avx512_f32_s8 := avx512
 
 MARCH_scalar_f64 := -march=x86-64
 MARCH_avx512_f64 := -mavx512f -mfma
@@ -56,6 +64,8 @@ MARCH_avx512_f32 := -mavx512f -mfma
 MARCH_avx2_f64 := -mavx2 -mfma -mf16c
 MARCH_avx2_f32 := -mavx2 -mfma -mf16c
 MARCH_avx512_f16 := -mavx512f -mfma -mf16c
+MARCH_avx512_f64_s4 := -mavx512f -mfma
+MARCH_avx512_f32_s8 := -mavx512f -mfma
 
 FT_scalar_f64 := double
 FT_avx512_f64 := double
@@ -63,14 +73,22 @@ FT_avx512_f32 := float
 FT_avx2_f64 := double
 FT_avx2_f32 := float
 FT_avx512_f16 := half
+FT_avx512_f64_s4 := double
+FT_avx512_f32_s8 := float
 VL_scalar_f64 := 1
 VL_avx512_f64 := 8
 VL_avx512_f32 := 16
 VL_avx2_f64 := 4
 VL_avx2_f32 := 8
 VL_avx512_f16 := 16
+VL_avx512_f64_s4 := 8
+VL_avx512_f32_s8 := 16
+# SOALEN = VECLEN unless set
+SOA_avx512_f64_s4 := 4
+SOA_avx512_f32_s8 := 8
+soa = $(if $(SOA_$(1)),$(SOA_$(1)),$(VL_$(1)))
 
-CONFIGS := scalar_f64 avx512_f64 avx512_f32 avx2_f64 avx2_f32 avx512_f16
+CONFIGS := scalar_f64 avx512_f64 avx512_f32 avx2_f64 avx2_f32 avx512_f16 avx512_f64_s4 avx512_f32_s8
 
 all: $(foreach c,$(CONFIGS),$(OUT)/libref_$(c).so)
 
@@ -88,7 +106,7 @@ $(OUT)/%/generated.stamp: $(OUT)/%/codegen
 	touch $@
 
 $(OUT)/%/bodies.inc: $(OUT)/%/generated.stamp ref_wrappers.py
-	python3 ref_wrappers.py $@ $(FT_$*) $(VL_$*) $(VL_$*)
+	python3 ref_wrappers.py $@ $(FT_$*) $(VL_$*) $(call soa,$*)
 
 $(OUT)/libref_%.so: $(OUT)/%/bodies.inc ref_driver.cpp
 	$(CXX) -std=c++14 -O3 -fopenmp -fPIC -shared -w $(MARCH_$*) $(DEFS_$*) \

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-CONFIGS = ("scalar_f64", "avx512_f64", "avx512_f32", "avx2_f64", "avx2_f32", "avx512_f16")
+CONFIGS = ("scalar_f64", "avx512_f64", "avx512_f32", "avx2_f64", "avx2_f32", "avx512_f16",
+           "avx512_f64_s4", "avx512_f32_s8")
 _libs = {}
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -29,7 +30,7 @@ def lib(config):
         l = ctypes.CDLL(path(config))
         i, d = ctypes.c_int, ctypes.c_double
         l.ref_apply.argtypes = [i, i, i, i, i, i, i, i, i, _dp, _dp, _dp, _dp, _dp, d, d, d, d, d,
-                                d, d, d, i, i, _dp, _dp]
+                                d, d, d, i, i, _dp, _dp, ctypes.POINTER(ctypes.c_ubyte)]
         l.ref_apply.restype = i
         l.ref_veclen.restype = i
         l.ref_precision.restype = i
@@ -47,6 +48,10 @@ def soalen(config):
     return lib(config).ref_soalen()
 
 
+def veclen(config):
+    return lib(config).ref_veclen()
+
+
 def stream_triad(config, nthreads, n=1 << 25, reps=5):
     """Host STREAM triad (GB/s) on nthreads OpenMP threads of the same
     runtime the reference's kernels run under (three FP64 arrays of n)."""
@@ -61,13 +66,16 @@ def _p(a):
 
 def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=None, recon=18,
           coeff_s=1.0, coeff_t_f=1.0, coeff_t_b=1.0, t_boundary=-1.0, alpha=0.0, mu=0.0,
-          muinv=0.0, rho=0.0, nthreads=1, reps=0, tbc_phases=None):
+          muinv=0.0, rho=0.0, nthreads=1, reps=0, tbc_phases=None, mask=None):
     """Run one reference kernel over checkerboard cb.  Returns (out, times):
     out is (V,4,3,2) with the cb sites set; with reps > 0 the block loop
     (not the packing) runs reps more times and times holds each call's
     seconds (a numpy array), else None.
     tbc_phases: four complex forward-hop phases (x, y, z, t) -> the bodies
-    generated with twisted boundary conditions in every dimension."""
+    generated with twisted boundary conditions in every dimension.
+    mask: optional uint8 array of one byte per lexicographic site, read at
+    the output sites: bit d = 2 mu + fb enables direction d there (the
+    driver turns it into the bodies' per-lane accumulate[d] masks)."""
     Lx, Ly, Lz, Lt = dims
     V = Lx * Ly * Lz * Lt
     c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
@@ -77,9 +85,15 @@ def apply(config, dims, psi, U, isign, cb, kind=0, achi=False, chi=None, clov=No
     ph = None
     if tbc_phases is not None:
         ph = np.array([v for z in tbc_phases for v in (complex(z).real, complex(z).imag)], dtype=np.float64)
+    pm = None
+    if mask is not None:
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        if mask.size != V:
+            raise ValueError("mask needs one byte per lattice site")
+        pm = mask.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
     rc = lib(config).ref_apply(kind, int(achi), isign, recon, cb, Lx, Ly, Lz, Lt, _p(psi), _p(chi),
                                _p(U), _p(clov), _p(out), coeff_s, coeff_t_f, coeff_t_b, t_boundary,
-                               alpha, mu, muinv, rho, nthreads, reps, _p(sec), _p(ph))
+                               alpha, mu, muinv, rho, nthreads, reps, _p(sec), _p(ph), pm)
     if rc != 0:
         raise ValueError(f"reference kernel rejected the call (config {config}, dims {dims})")
     return out, (sec if reps > 0 else None)

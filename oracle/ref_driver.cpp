@@ -16,9 +16,14 @@
 //     (codegen/data_types.cc:140-198, loadSplitSOAFVec in inst_sp_vec16.cc);
 //   * passes coeff_t_b / coeff_t_f multiplied by the time boundary factor on
 //     the first / last time slice (codegen/dslash.cc:79-86);
-//   * runs the block loop with OpenMP, accumulate[] = all lanes.
-// Requires SOALEN == VECLEN (one x-row segment per vector) and Lx/2 a
-// multiple of SOALEN.
+//   * runs the block loop with OpenMP, one body call per SU3MatrixBlock:
+//     VECLEN lanes = NGY = VECLEN / SOALEN rows of y, each row segment of
+//     SOALEN sites addressed through the offset arrays at lane r * SOALEN
+//     (the bodies load row segment r from base + offs[r * SOALEN],
+//     codegen/data_types.cc:64-119, codegen/address_types.h:281-312);
+//   * builds accumulate[d] from an optional per-site direction mask: bit l
+//     of accumulate[d] is bit d of lane l's site (all lanes when no mask).
+// Requires Lx/2 a multiple of SOALEN and Ly a multiple of NGY.
 #include <immintrin.h>
 #include <omp.h>
 
@@ -57,8 +62,8 @@ static inline FT to_ft(double v) { return (FT)v; }
 static inline double from_ft(FT v) { return (double)v; }
 #endif
 
-#if SOALEN != VECLEN
-#error "ref_driver supports SOALEN == VECLEN only"
+#if VECLEN % SOALEN != 0
+#error "VECLEN must be a multiple of SOALEN"
 #endif
 
 struct RefArgs {
@@ -104,20 +109,28 @@ using namespace QPhiX;
 namespace {
 
 constexpr int S = SOALEN;
-constexpr int BLK = 24 * S;  // FT elements per FourSpinorBlock
+constexpr int NGY = VECLEN / SOALEN;  // rows of y per gauge / clover vector
+constexpr int BLK = 24 * S;           // FT elements per FourSpinorBlock
 
 struct Lat {
   int L[4], Lxh, nxb;
   long V;
+  // FourSpinorBlock of row (t, z, y), x-segment xb
   long block(int t, int z, int y, int xb) const
   {
     return (((long)t * L[2] + z) * L[1] + y) * nxb + xb;
+  }
+  // SU3MatrixBlock / CloverBlock holding rows y / NGY * NGY .. + NGY - 1
+  long gblock(int t, int z, int y, int xb) const
+  {
+    return (((long)t * L[2] + z) * (L[1] / NGY) + y / NGY) * nxb + xb;
   }
   long lex(int x, int y, int z, int t) const
   {
     return (((long)t * L[2] + z) * L[1] + y) * L[0] + x;
   }
   int xof(int xh, int y, int z, int t, int cb) const { return 2 * xh + ((cb + y + z + t) & 1); }
+  long nblk() const { return (long)L[3] * L[2] * L[1] * nxb; }
 };
 
 template <typename T> T *amalloc(size_t n)
@@ -162,8 +175,9 @@ void unpack_spinor(double *dst, const FT *src, const Lat &g, int cb)
         }
 }
 
-// Gauge blocks FT[8][nrow][3][2][V]: S[d][i][j] = M_d[j][i], with
-// M_{2mu} = U_mu(x - mu) (applied as its adjoint) and M_{2mu+1} = U_mu(x).
+// Gauge blocks FT[8][nrow][3][2][VECLEN]: S[d][i][j] = M_d[j][i], with
+// M_{2mu} = U_mu(x - mu) (applied as its adjoint) and M_{2mu+1} = U_mu(x);
+// the site of row y, lane i sits in lane (y % NGY) * S + i.
 void pack_gauge(FT *dst, const double *U, const Lat &g, int cb, int nrow)
 {
   const long gblk = 8L * nrow * 3 * 2 * VECLEN;
@@ -171,8 +185,9 @@ void pack_gauge(FT *dst, const double *U, const Lat &g, int cb, int nrow)
     for (int z = 0; z < g.L[2]; z++)
       for (int y = 0; y < g.L[1]; y++)
         for (int xb = 0; xb < g.nxb; xb++) {
-          FT *b = dst + g.block(t, z, y, xb) * gblk;
+          FT *b = dst + g.gblock(t, z, y, xb) * gblk;
           for (int i = 0; i < S; i++) {
+            const int lane = (y % NGY) * S + i;
             int c4[4] = {g.xof(xb * S + i, y, z, t, cb), y, z, t};
             for (int d = 0; d < 8; d++) {
               int mu = d / 2, fb = d % 2;
@@ -183,7 +198,7 @@ void pack_gauge(FT *dst, const double *U, const Lat &g, int cb, int nrow)
               for (int r = 0; r < nrow; r++)
                 for (int col = 0; col < 3; col++)
                   for (int ri = 0; ri < 2; ri++)
-                    b[(((d * nrow + r) * 3 + col) * 2 + ri) * VECLEN + i] =
+                    b[(((d * nrow + r) * 3 + col) * 2 + ri) * VECLEN + lane] =
                         to_ft(m[(col * 3 + r) * 2 + ri]);
             }
           }
@@ -198,16 +213,17 @@ void pack_clover(FT *dst, const double *cl, const Lat &g, int cb)
     for (int z = 0; z < g.L[2]; z++)
       for (int y = 0; y < g.L[1]; y++)
         for (int xb = 0; xb < g.nxb; xb++) {
-          FT *b = dst + g.block(t, z, y, xb) * cblk;
+          FT *b = dst + g.gblock(t, z, y, xb) * cblk;
           for (int i = 0; i < S; i++) {
+            const int lane = (y % NGY) * S + i;
             long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
             for (int blk = 0; blk < 2; blk++) {
               const double *src = cl + s * 72 + blk * 36;
               FT *diag = b + blk * 36 * VECLEN;
               FT *off = diag + 6 * VECLEN;
-              for (int k = 0; k < 6; k++) diag[k * VECLEN + i] = to_ft(src[k]);
+              for (int k = 0; k < 6; k++) diag[k * VECLEN + lane] = to_ft(src[k]);
               for (int k = 0; k < 15; k++)
-                for (int ri = 0; ri < 2; ri++) off[(k * 2 + ri) * VECLEN + i] = to_ft(src[6 + 2 * k + ri]);
+                for (int ri = 0; ri < 2; ri++) off[(k * 2 + ri) * VECLEN + lane] = to_ft(src[6 + 2 * k + ri]);
             }
           }
         }
@@ -221,12 +237,27 @@ void pack_full_clover(FT *dst, const double *cl, const Lat &g, int cb)
     for (int z = 0; z < g.L[2]; z++)
       for (int y = 0; y < g.L[1]; y++)
         for (int xb = 0; xb < g.nxb; xb++) {
-          FT *b = dst + g.block(t, z, y, xb) * cblk;
+          FT *b = dst + g.gblock(t, z, y, xb) * cblk;
           for (int i = 0; i < S; i++) {
+            const int lane = (y % NGY) * S + i;
             long s = g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t);
-            for (int k = 0; k < 144; k++) b[k * VECLEN + i] = to_ft(cl[s * 144 + k]);
+            for (int k = 0; k < 144; k++) b[k * VECLEN + lane] = to_ft(cl[s * 144 + k]);
           }
         }
+}
+
+bool make_lat(Lat &g, int Lx, int Ly, int Lz, int Lt)
+{
+  if ((Lx | Ly | Lz | Lt) & 1) return false;
+  g.L[0] = Lx;
+  g.L[1] = Ly;
+  g.L[2] = Lz;
+  g.L[3] = Lt;
+  g.Lxh = Lx / 2;
+  if (g.Lxh % S || Ly % NGY) return false;
+  g.nxb = g.Lxh / S;
+  g.V = (long)Lx * Ly * Lz * Lt;
+  return true;
 }
 
 }  // namespace
@@ -240,18 +271,13 @@ extern "C" {
 long ref_pack_field(int field, int recon, int cb, int Lx, int Ly, int Lz, int Lt, const double *src,
                     void *dst)
 {
-  if ((Lx | Ly | Lz | Lt) & 1 || (Lx / 2) % S || field < 0 || field > 3) return -1;
   Lat g;
-  g.L[0] = Lx;
-  g.L[1] = Ly;
-  g.L[2] = Lz;
-  g.L[3] = Lt;
-  g.Lxh = Lx / 2;
-  g.nxb = g.Lxh / S;
-  const long nblk = (long)Lt * Lz * Ly * g.nxb;
+  if (!make_lat(g, Lx, Ly, Lz, Lt) || field < 0 || field > 3) return -1;
+  const long nblk = g.nblk();
   const int nrow = recon == 12 ? 2 : 3;
-  const long per[4] = {BLK, 8L * nrow * 3 * 2 * VECLEN, 72L * VECLEN, 144L * VECLEN};
-  const long n = nblk * per[field];
+  const long per[4] = {nblk * BLK, nblk / NGY * 8L * nrow * 3 * 2 * VECLEN, nblk / NGY * 72L * VECLEN,
+                       nblk / NGY * 144L * VECLEN};
+  const long n = per[field];
   if (!dst) return n;
   FT *d = (FT *)dst;
   memset(d, 0, n * sizeof(FT));
@@ -317,31 +343,32 @@ int ref_lowprec(void)
 // tbc_phases (8 doubles, (re, im) per dimension x y z t) selects the bodies
 // generated with twisted boundary conditions in every dimension and passes
 // the phases (codegen/dslash.cc:206-230); NULL selects the simple bodies.
+// mask (optional, one byte per lexicographic site, read at the output
+// sites): bit d enables direction d = 2 mu + fb; lane l of a body call's
+// accumulate[d] is bit d of the site in lane l.  The bodies apply these
+// lane masks to the x hops (and to the y hops when SOALEN < VECLEN) and
+// otherwise run a direction for the whole vector when any lane's bit is set
+// (codegen/dslash.cc:109,248,285-290,570-578).
 int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, int Lz, int Lt,
               const double *psi, const double *chi, const double *U, const double *clov,
               double *out, double coeff_s, double coeff_t_f, double coeff_t_b,
               double t_boundary, double alpha, double mu, double muinv, double rho,
-              int nthreads, int reps, double *seconds, const double *tbc_phases)
+              int nthreads, int reps, double *seconds, const double *tbc_phases,
+              const unsigned char *mask)
 {
   if (kind < 0 || kind > 3 || (recon != 12 && recon != 18) || (cb != 0 && cb != 1)) return -1;
-  if ((Lx | Ly | Lz | Lt) & 1) return -1;
   Lat g;
-  g.L[0] = Lx;
-  g.L[1] = Ly;
-  g.L[2] = Lz;
-  g.L[3] = Lt;
-  g.Lxh = Lx / 2;
-  if (g.Lxh % S) return -1;
-  g.nxb = g.Lxh / S;
-  const long nblk = (long)Lt * Lz * Ly * g.nxb;
+  if (!make_lat(g, Lx, Ly, Lz, Lt)) return -1;
+  const long nblk = g.nblk();
+  const long ngblk = nblk / NGY;
   const int nrow = recon == 12 ? 2 : 3;
   FT *in = amalloc<FT>(nblk * BLK);
   FT *chib = amalloc<FT>(nblk * BLK);
   FT *o = amalloc<FT>(nblk * BLK);
   const long gblk = 8L * nrow * 3 * 2 * VECLEN;
-  FT *gb = amalloc<FT>(nblk * gblk);
+  FT *gb = amalloc<FT>(ngblk * gblk);
   const long cblk = (kind == 3 ? 144L : 72L) * VECLEN;
-  FT *cl = (kind == 1 || kind == 3) ? amalloc<FT>(nblk * cblk) : nullptr;
+  FT *cl = (kind == 1 || kind == 3) ? amalloc<FT>(ngblk * cblk) : nullptr;
   pack_spinor(in, psi, g, 1 - cb);
   if (achi) pack_spinor(chib, chi, g, cb);
   pack_gauge(gb, U, g, cb, nrow);
@@ -351,46 +378,65 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
   ref_body_fn fn = (tbc_phases ? ref_table_tbc : ref_table)[kind][achi ? 1 : 0][isign > 0 ? 0 : 1]
                                                           [recon == 12 ? 0 : 1];
   const unsigned int all = VECLEN >= 32 ? 0xffffffffu : ((1u << VECLEN) - 1u);
+  const int Lyg = Ly / NGY;
 
   auto run = [&]() {
 #pragma omp parallel for num_threads(nthreads) schedule(static)
-    for (long bi = 0; bi < nblk; bi++) {
-      long r = bi;
+    for (long gi = 0; gi < ngblk; gi++) {
+      long r = gi;
       const int xb = (int)(r % g.nxb);
       r /= g.nxb;
-      const int y = (int)(r % Ly);
-      r /= Ly;
+      const int yg = (int)(r % Lyg);
+      r /= Lyg;
       const int z = (int)(r % Lz);
       const int t = (int)(r / Lz);
-      const int p = (cb + y + z + t) & 1;  // x = 2 xh + p on this row
+      const int y0 = yg * NGY;
+      const long b0 = g.block(t, z, y0, xb);  // the vector's first row segment
       RefArgs A;
       memset(&A, 0, sizeof(A));
-      const FT *blk = in + bi * BLK;
-      A.xyBase = blk;
-      const long xprev = g.block(t, z, y, (xb + g.nxb - 1) % g.nxb);
-      const long xnext = g.block(t, z, y, (xb + 1) % g.nxb);
-      if (p == 1) {  // x-1 = 2 xh: same lanes; x+1 = 2 xh + 2: shift by one
-        A.xbOffs[0] = 0;
-        if (S > 1) A.xbOffs[1] = 1;
-        A.xfOffs[0] = 1;
-        A.xfOffs[S - 1] = (int)((xnext - bi) * BLK);
-      } else {  // x-1 = 2 xh - 1: shift by one; x+1 = 2 xh + 1: same lanes
-        A.xbOffs[0] = (int)((xprev - bi) * BLK + (S - 1));
-        if (S > 1) A.xbOffs[1] = 0;
-        A.xfOffs[0] = 0;
-        A.xfOffs[S - 1] = S - 1;
+      for (int row = 0; row < NGY; row++) {
+        const int y = y0 + row, L0 = row * S;
+        const long bi = g.block(t, z, y, xb);
+        const int ro = (int)((bi - b0) * BLK);  // this row's segment, from xyBase
+        const int p = (cb + y + z + t) & 1;    // x = 2 xh + p on this row
+        const long xprev = g.block(t, z, y, (xb + g.nxb - 1) % g.nxb);
+        const long xnext = g.block(t, z, y, (xb + 1) % g.nxb);
+        A.offs[L0] = ro;
+        // X split loads (codegen/inst_sp_vec16.cc:609-636): the backward
+        // neighbour's lanes 1..S-1 come from xyBase + xbOffs[L0 + 1] - 1 and
+        // lane 0 from xyBase + xbOffs[L0]; the forward neighbour's lanes
+        // 0..S-2 from xyBase + xfOffs[L0] and lane S-1 from xfOffs[L0 + S-1].
+        if (p == 1) {  // x-1 = 2 xh: same lanes; x+1 = 2 xh + 2: shift by one
+          A.xbOffs[L0] = ro;
+          if (S > 1) A.xbOffs[L0 + 1] = ro + 1;
+          A.xfOffs[L0] = ro + 1;
+          A.xfOffs[L0 + S - 1] = (int)((xnext - b0) * BLK);
+        } else {  // x-1 = 2 xh - 1: shift by one; x+1 = 2 xh + 1: same lanes
+          A.xbOffs[L0] = (int)((xprev - b0) * BLK + (S - 1));
+          if (S > 1) A.xbOffs[L0 + 1] = ro;
+          A.xfOffs[L0] = ro;
+          A.xfOffs[L0 + S - 1] = ro + S - 1;
+        }
+        A.ybOffs[L0] = (int)((g.block(t, z, (y + Ly - 1) % Ly, xb) - b0) * BLK);
+        A.yfOffs[L0] = (int)((g.block(t, z, (y + 1) % Ly, xb) - b0) * BLK);
+        for (int i = 0; i < S; i++)
+          for (int d = 0; d < 8; d++) {
+            const bool on = mask ? (mask[g.lex(g.xof(xb * S + i, y, z, t, cb), y, z, t)] >> d) & 1 : true;
+            if (on) A.accumulate[d] |= 1u << (L0 + i);
+          }
       }
-      A.ybOffs[0] = (int)((g.block(t, z, (y + Ly - 1) % Ly, xb) - bi) * BLK);
-      A.yfOffs[0] = (int)((g.block(t, z, (y + 1) % Ly, xb) - bi) * BLK);
-      A.zbBase = in + g.block(t, (z + Lz - 1) % Lz, y, xb) * BLK;
-      A.zfBase = in + g.block(t, (z + 1) % Lz, y, xb) * BLK;
-      A.tbBase = in + g.block((t + Lt - 1) % Lt, z, y, xb) * BLK;
-      A.tfBase = in + g.block((t + 1) % Lt, z, y, xb) * BLK;
-      A.chiBase = chib + bi * BLK;
-      A.oBase = o + bi * BLK;
-      A.gBase = gb + bi * gblk;
-      A.clBase = cl ? cl + bi * cblk : nullptr;
-      for (int d = 0; d < 8; d++) A.accumulate[d] = all;
+      for (int d = 0; d < 8; d++) A.accumulate[d] &= all;
+      // z / t neighbours: the neighbouring row-(y0) segment; rows further in
+      // the vector follow at the same offs as in the vector's own plane
+      A.xyBase = in + b0 * BLK;
+      A.zbBase = in + g.block(t, (z + Lz - 1) % Lz, y0, xb) * BLK;
+      A.zfBase = in + g.block(t, (z + 1) % Lz, y0, xb) * BLK;
+      A.tbBase = in + g.block((t + Lt - 1) % Lt, z, y0, xb) * BLK;
+      A.tfBase = in + g.block((t + 1) % Lt, z, y0, xb) * BLK;
+      A.chiBase = chib + b0 * BLK;
+      A.oBase = o + b0 * BLK;
+      A.gBase = gb + gi * gblk;
+      A.clBase = cl ? cl + gi * cblk : nullptr;
       A.alpha = to_ft(alpha);
       A.coeff_s = to_ft(coeff_s);
       A.coeff_t_b = to_ft(t == 0 ? coeff_t_b * t_boundary : coeff_t_b);
@@ -407,10 +453,10 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
     }
   };
   run();
-  for (int r = 0; r < reps; r++) {
+  for (int rr = 0; rr < reps; rr++) {
     const double t0 = omp_get_wtime();
     run();
-    if (seconds) seconds[r] = omp_get_wtime() - t0;
+    if (seconds) seconds[rr] = omp_get_wtime() - t0;
   }
   unpack_spinor(out, o, g, cb);
   free(in);

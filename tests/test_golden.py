@@ -69,7 +69,8 @@ def test_oracle_matches_reference_avx512_f32_fixture(qamd, orc):
 
 
 REF_DIMS = {"scalar_f64": (8, 8, 8, 8), "avx512_f64": (16, 4, 4, 8), "avx512_f32": (32, 4, 4, 4),
-            "avx2_f64": (8, 8, 8, 8), "avx2_f32": (16, 4, 4, 8)}
+            "avx2_f64": (8, 8, 8, 8), "avx2_f32": (16, 4, 4, 8),
+            "avx512_f64_s4": (16, 4, 4, 8), "avx512_f32_s8": (32, 4, 4, 4)}
 
 
 def _cpu_has(flag):
@@ -200,3 +201,93 @@ def test_f16_agreement_criterion_rejects_real_errors():
     near0[3] = np.float16(1e-4 + 4e-7)  # about 9 FP16 ulps of 1e-4
     assert f16_within(near0, want).all()
     assert not f16_within(np.float16(1e-4 + 2e-4) * np.ones(4, np.float16), want)[3]
+
+
+def accumulate_mask(dims, cb, soalen, veclen, seed):
+    """A per-site direction mask (one byte per lexicographic site, bit d =
+    2 mu + fb) that the reference's bodies can express: random per site in
+    x, and in y when soalen < veclen (the directions they mask lane by lane,
+    codegen/dslash.cc:109,285-290,570-578); the same for every site of a
+    body call's vector (ngy rows x soalen lanes) otherwise, since the bodies
+    gate those directions per call with `if (accumulate[d])`."""
+    Lx, Ly, Lz, Lt = dims
+    ngy, nvecs = veclen // soalen, Lx // 2 // soalen
+    rng = np.random.default_rng(seed)
+    t, z, y, xh = np.meshgrid(np.arange(Lt), np.arange(Lz), np.arange(Ly), np.arange(Lx // 2),
+                              indexing="ij")
+    t, z, y, xh = t.ravel(), z.ravel(), y.ravel(), xh.ravel()
+    lex = ((t * Lz + z) * Ly + y) * Lx + 2 * xh + ((cb + y + z + t) & 1)
+    vec = ((t * Lz + z) * (Ly // ngy) + y // ngy) * nvecs + xh // soalen
+    nvec = vec.max() + 1
+    mask = np.zeros(Lx * Ly * Lz * Lt, dtype=np.uint8)
+    for d in range(8):
+        per_site = d < 2 or (d < 4 and soalen < veclen)
+        bits = rng.integers(0, 2, len(lex)) if per_site else rng.integers(0, 2, nvec)[vec]
+        mask[lex] |= (bits << d).astype(np.uint8)
+    return mask
+
+
+def masked_oracle(orc, dims, psi, U, isign, cb, mask, **kw):
+    """The operator with direction d dropped at the sites whose mask bit d is
+    clear: the oracle's result split exactly into its 8 direction terms
+    (links of one dimension kept, the others zeroed; forward and backward
+    hops separated by a twisted phase i in that dimension, forward e^{i pi/2},
+    backward its conjugate), then summed under the mask."""
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    base = c(orc.dslash(dims, psi, np.zeros_like(U), isign, cb, **kw))  # the a chi part
+    total = base.copy()
+    bits = (mask.astype(np.int64)[:, None] >> np.arange(8)[None, :]) & 1  # (V, 8)
+    for mu in range(4):
+        Um = np.zeros_like(U)
+        Um[:, mu] = U[:, mu]
+        tbc = [int(m == mu) for m in range(4)]
+        h0 = c(orc.dslash(dims, psi, Um, isign, cb, **kw)) - base
+        h1 = c(orc.dslash(dims, psi, Um, isign, cb, tbc=tbc, tbc_phases=[1j if m == mu else 1 for m in range(4)],
+                          **kw)) - base
+        fwd, bwd = (h0 - 1j * h1) / 2, (h0 + 1j * h1) / 2
+        total += bits[:, 2 * mu, None, None] * bwd + bits[:, 2 * mu + 1, None, None] * fwd
+    return np.stack([total.real, total.imag], axis=-1)
+
+
+@pytest.mark.parametrize("config", ["avx512_f64", "avx512_f64_s4", "avx512_f32_s8", "avx2_f64"])
+def test_reference_accumulate_masks_live(qamd, orc, config):
+    """The reference's bodies with non-trivial accumulate[] masks (driver
+    oracle/ref_driver.cpp) against the oracle's exact direction-by-direction
+    decomposition summed under the same per-site mask: every family,
+    dslash and achimbdpsi, both isigns and checkerboards, 12/18-real links,
+    soalen == veclen and the xy-packed soalen = veclen / 2 (where y is masked
+    lane by lane too), hardware-masked (AVX-512) and blended (AVX2) builds."""
+    from oracle import ref
+    if not ref.available(config):
+        pytest.skip("oracle/_ref not built (needs the reference sources at build time)")
+    if "avx512" in config and not _cpu_has("avx512f"):
+        pytest.skip("host CPU has no AVX-512")
+    if "avx2" in config and not _cpu_has("avx2"):
+        pytest.skip("host CPU has no AVX2")
+    L = qamd.lattice
+    dims = REF_DIMS[config]
+    S, V = ref.soalen(config), ref.veclen(config)
+    U = L.random_gauge(dims, 0.3, 51).numpy()
+    psi = L.random_spinor(dims, 52).numpy()
+    chi = L.random_spinor(dims, 53).numpy()
+    clt = L.random_clover(dims, 54)
+    tol = 1e-13 if "f64" in config else 1e-6
+    kw0 = dict(alpha=3.7, mu=0.027, muinv=0.26, rho=0.05, coeff_s=0.9, coeff_t_f=1.1, coeff_t_b=1.2,
+               t_boundary=-1.0)
+    for kind in range(4):
+        for achi in (0, 1):
+            for isign, cb in ((1, 0), (-1, 1)):
+                kw = dict(kw0, kind=kind, achi=bool(achi), chi=chi)
+                if kind == 1:
+                    kw["clov"] = clt.numpy() if achi else L.invert_clover(clt).numpy()
+                if kind == 3:
+                    kw["clov"] = L.tmclover_blocks(clt, 0.1, isign, not achi).numpy()
+                mask = accumulate_mask(dims, cb, S, V, 100 * kind + 10 * achi + cb)
+                want = masked_oracle(orc, dims, psi, U, isign, cb, mask, **kw)
+                full = orc.dslash(dims, psi, U, isign, cb, **kw)
+                m = cb_mask(dims, cb)
+                assert np.abs(want[m] - full[m]).max() > 0.1 * np.abs(full[m]).max()  # the mask matters
+                for recon in (12, 18):
+                    b, _ = ref.apply(config, dims, psi, U, isign, cb, recon=recon, mask=mask, **kw)
+                    err = np.abs(want[m] - b[m]).max() / np.abs(want[m]).max()
+                    assert err < tol, (kind, achi, isign, cb, recon, err)

@@ -1,10 +1,12 @@
 """CPU checks of the QPhiX block-layout restatement (tests/qphix_blocks.py)
 and of the converter entry points' host-side validation.
 
-* At soalen == veclen the restated block arrays equal, element for element,
-  the arrays the reference driver hands to the reference's own kernels
-  (oracle/_ref, ref.pack), for spinor, gauge (12 and 18 rows), clover and
-  full clover on both checkerboards, in every built configuration.  Those
+* The restated block arrays equal, element for element, the arrays the
+  reference driver hands to the reference's own kernels (oracle/_ref,
+  ref.pack), for spinor, gauge (12 and 18 rows), clover and full clover on
+  both checkerboards, in every built configuration: soalen == veclen, and
+  the xy-packed soalen = veclen / 2 builds (one gauge vector over two rows
+  of y).  Those
   kernels reproduce the FP64 oracle to rounding (tests/test_golden.py), so
   this pins the layout the converters must read.
 * qphix_amd_qphix_field_bytes matches the restatement's block counts and
@@ -17,7 +19,7 @@ import pytest
 
 import qphix_blocks as qb
 
-CONFIGS = ("scalar_f64", "avx2_f64", "avx2_f32", "avx512_f64", "avx512_f32")
+CONFIGS = ("scalar_f64", "avx2_f64", "avx2_f32", "avx512_f64", "avx512_f32", "avx512_f64_s4", "avx512_f32_s8")
 
 
 def _ref():
@@ -30,7 +32,7 @@ def test_restated_blocks_equal_reference_driver(qamd, config):
     ref = _ref()
     if not ref.available(config):
         pytest.skip(f"oracle/_ref/libref_{config}.so not built")
-    S = ref.soalen(config)
+    S, V = ref.soalen(config), ref.veclen(config)
     ft = 1 if ref.lib(config).ref_precision() == 1 else 0
     dims = (2 * S * 2, 4, 2, 4)
     L = qamd.lattice
@@ -40,10 +42,10 @@ def test_restated_blocks_equal_reference_driver(qamd, config):
     fcl = L.tmclover_blocks(L.random_clover(dims, 73), 0.1, 1, False).numpy()
     for cb in (0, 1):
         for recon in (12, 18):
-            lay = qb.Layout(ft, S, S, compress12=recon == 12)
+            lay = qb.Layout(ft, V, S, compress12=recon == 12)
             got = qb.pack_gauge(U, lay, dims, cb)
             np.testing.assert_array_equal(got, ref.pack(config, 1, dims, U, cb, recon))
-        lay = qb.Layout(ft, S, S)
+        lay = qb.Layout(ft, V, S)
         np.testing.assert_array_equal(qb.pack_spinor(psi, lay, dims, cb), ref.pack(config, 0, dims, psi, cb))
         np.testing.assert_array_equal(qb.pack_clover(clov, lay, dims, cb), ref.pack(config, 2, dims, clov, cb))
         np.testing.assert_array_equal(qb.pack_full_clover(fcl, lay, dims, cb),
