@@ -186,6 +186,91 @@ int qphix_amd_full_clover_from_qphix(qphix_amd_ctx *ctx,
                                      void *dst, int prec, const void *src,
                                      int cb, void *stream);
 
+/* ---- operators on QPhiX FourSpinorBlock arrays, in place ---------------
+   The operators below, with in, chi and out read and written where a QPhiX
+   host keeps them: FourSpinorBlock arrays of the layout `lay` (FT = lay->ft,
+   F64, F32 or F16; soalen a power of two; the block order above, padding
+   included) instead of native spinors -- no conversion pass per call.  The
+   links and clover blocks are native fields in the same precision
+   (converted once, e.g. with qphix_amd_gauge_from_qphix), `recon` as
+   there.  Same semantics, arguments and errors as the native entries of
+   the same family; QPhiX arrays need FT alignment only.
+   accumulate (device, optional): one byte per site of the output array, in
+   its site order (block * soalen + lane, padding slots included and
+   ignored); bit d = 2 mu + fb set accumulates direction d at that site,
+   clear drops it.  This is the reference's per-call accumulate[8]
+   (codegen/jinja/dslash_decl.h.j2:6-43) transposed to one mask per site:
+   QPhiX sets lane bits for the x hops (and the y hops when soalen <
+   veclen) and whole-vector flags otherwise (codegen/dslash.cc:109,248,
+   285-290,570-578); any such pattern is a per-site mask.  NULL accumulates
+   every direction.  The context's split dimensions (qphix_amd_set_comm)
+   apply as well.
+   Each replaces the per-block template named in the native entry below:
+     qphix_amd_qphix_dslash                 dslash_{plus,minus}_vec
+     qphix_amd_qphix_dslash_achimbdpsi      dslash_achimbdpsi_{plus,minus}_vec
+     qphix_amd_qphix_clov_dslash            clov_dslash_{plus,minus}_vec
+     qphix_amd_qphix_clov_dslash_achimbdpsi clov_dslash_achimbdpsi_{plus,minus}_vec
+     qphix_amd_qphix_tm_dslash              tm_dslash_{plus,minus}_vec
+     qphix_amd_qphix_tm_dslash_achimbdpsi   tm_dslash_achimbdpsi_{plus,minus}_vec
+     qphix_amd_qphix_tm_clov_dslash         tm_clov_dslash_{plus,minus}_vec
+     qphix_amd_qphix_tm_clov_dslash_achimbdpsi
+                                            tm_clov_dslash_achimbdpsi_{plus,minus}_vec
+   (codegen/jinja/{,clov_,tm_,tm_clov_}dslash_decl.h.j2:6-94).            */
+int qphix_amd_qphix_dslash(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay,
+                           int recon, int isign, int cb, void *out,
+                           const void *in, const void *gauge,
+                           const unsigned char *accumulate, double coeff_s,
+                           double coeff_t_f, double coeff_t_b, void *stream);
+int qphix_amd_qphix_dslash_achimbdpsi(qphix_amd_ctx *ctx,
+                                      const qphix_amd_qphix_layout *lay,
+                                      int recon, int isign, int cb, void *out,
+                                      const void *in, const void *chi,
+                                      const void *gauge,
+                                      const unsigned char *accumulate,
+                                      double alpha, double coeff_s,
+                                      double coeff_t_f, double coeff_t_b,
+                                      void *stream);
+int qphix_amd_qphix_clov_dslash(qphix_amd_ctx *ctx,
+                                const qphix_amd_qphix_layout *lay, int recon,
+                                int isign, int cb, void *out, const void *in,
+                                const void *gauge, const void *clover,
+                                const unsigned char *accumulate,
+                                double coeff_s, double coeff_t_f,
+                                double coeff_t_b, void *stream);
+int qphix_amd_qphix_clov_dslash_achimbdpsi(
+    qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+    int isign, int cb, void *out, const void *in, const void *chi,
+    const void *gauge, const void *clover, const unsigned char *accumulate,
+    double coeff_s, double coeff_t_f, double coeff_t_b, double prec_mass_rho,
+    void *stream);
+int qphix_amd_qphix_tm_dslash(qphix_amd_ctx *ctx,
+                              const qphix_amd_qphix_layout *lay, int recon,
+                              int isign, int cb, void *out, const void *in,
+                              const void *gauge,
+                              const unsigned char *accumulate, double coeff_s,
+                              double coeff_t_f, double coeff_t_b, double mu,
+                              double muinv, void *stream);
+int qphix_amd_qphix_tm_dslash_achimbdpsi(
+    qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+    int isign, int cb, void *out, const void *in, const void *chi,
+    const void *gauge, const unsigned char *accumulate, double alpha,
+    double coeff_s, double coeff_t_f, double coeff_t_b, double mu,
+    void *stream);
+int qphix_amd_qphix_tm_clov_dslash(qphix_amd_ctx *ctx,
+                                   const qphix_amd_qphix_layout *lay,
+                                   int recon, int isign, int cb, void *out,
+                                   const void *in, const void *gauge,
+                                   const void *fclover,
+                                   const unsigned char *accumulate,
+                                   double coeff_s, double coeff_t_f,
+                                   double coeff_t_b, void *stream);
+int qphix_amd_qphix_tm_clov_dslash_achimbdpsi(
+    qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+    int isign, int cb, void *out, const void *in, const void *chi,
+    const void *gauge, const void *fclover, const unsigned char *accumulate,
+    double coeff_s, double coeff_t_f, double coeff_t_b, double prec_mass_rho,
+    void *stream);
+
 /* ---- operators ----------------------------------------------------------
    out (checkerboard cb) from in (checkerboard 1-cb).  hop(x) =
      sum_mu b_mu(x) (1 + s g_mu) U_mu(x-mu)^+ psi(x-mu)

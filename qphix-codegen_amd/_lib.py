@@ -54,6 +54,25 @@ SIGNATURES = {
     "qphix_amd_gauge_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "qphix_amd_clover_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _vp, _c_int, _vp]),
     "qphix_amd_full_clover_from_qphix": (_c_int, [_vp, _lay, _vp, _c_int, _vp, _c_int, _vp]),
+    "qphix_amd_qphix_dslash": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                        _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_qphix_dslash_achimbdpsi": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
+                                                   _c_double, _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_qphix_clov_dslash": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
+                                             _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_qphix_clov_dslash_achimbdpsi": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                                        _vp, _vp, _c_double, _c_double, _c_double, _c_double,
+                                                        _vp]),
+    "qphix_amd_qphix_tm_dslash": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_double,
+                                           _c_double, _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_qphix_tm_dslash_achimbdpsi": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                                      _vp, _c_double, _c_double, _c_double, _c_double,
+                                                      _c_double, _vp]),
+    "qphix_amd_qphix_tm_clov_dslash": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
+                                                _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_qphix_tm_clov_dslash_achimbdpsi": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                                           _vp, _vp, _vp, _c_double, _c_double, _c_double,
+                                                           _c_double, _vp]),
     "qphix_amd_dslash": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                   _c_double, _c_double, _c_double, _vp]),
     "qphix_amd_dslash_achimbdpsi": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,

@@ -33,6 +33,38 @@ int run(qphix_amd_ctx *ctx, const DslashCall &c, void *stream)
   return hip_status(qamd::run_dslash_any(ctx, c, static_cast<hipStream_t>(stream)));
 }
 
+// An operator call on QPhiX FourSpinorBlock arrays: in, chi and out in the
+// layout lay (FT = lay->ft, also the storage of the native links and clover
+// blocks), with the optional per-site direction mask acc.
+int run_qphix(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, const unsigned char *acc, DslashCall c,
+              void *stream)
+{
+  if (!ctx || !lay) return QPHIX_AMD_EINVAL;
+  qamd::QSpinorGeom q;
+  if (!qamd::qphix_spinor_geometry(ctx, lay, q)) return QPHIX_AMD_EINVAL;
+  if ((c.isign != 1 && c.isign != -1) || (c.cb != 0 && c.cb != 1)) return QPHIX_AMD_EINVAL;
+  if (c.recon != 12 && c.recon != 18) return QPHIX_AMD_EINVAL;
+  if (!c.out || !c.in || !c.gauge || c.out == c.in || (c.achi && (!c.chi || c.chi == c.out)))
+    return QPHIX_AMD_EINVAL;
+  const bool need_clov = c.op == qamd::OP_CLOVER || c.op == qamd::OP_TMCLOVER;
+  if (need_clov && !c.clov) return QPHIX_AMD_EINVAL;
+  // QPhiX arrays are read element by element: FT alignment suffices; the
+  // native links and clover blocks are read as 16-byte vectors
+  const uintptr_t ftm = (uintptr_t)storage_bytes(lay->ft) - 1;
+  if ((reinterpret_cast<uintptr_t>(c.out) & ftm) || (reinterpret_cast<uintptr_t>(c.in) & ftm) ||
+      (c.chi && (reinterpret_cast<uintptr_t>(c.chi) & ftm)) || !aligned16(c.gauge) ||
+      (c.clov && !aligned16(c.clov)))
+    return QPHIX_AMD_EALIGN;
+  c.prec = lay->ft;
+  c.spl = 1;
+  c.q_sh = q.sh;
+  c.q_nvecs = q.nvecs;
+  c.q_pxy = q.pxy;
+  c.q_pxyz = q.pxyz;
+  c.acc = acc;
+  return hip_status(qamd::run_dslash_any(ctx, c, static_cast<hipStream_t>(stream)));
+}
+
 DslashCall make_call(int prec, int recon, int op, bool achi, int isign, int cb, void *out,
                      const void *in, const void *chi, const void *gauge, const void *clov,
                      double cs, double ctf, double ctb)
@@ -359,6 +391,98 @@ int qphix_amd_ndtm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec, int reco
 {
   return qphix_amd_tm_clov_dslash_achimbdpsi(ctx, prec, recon, isign, cb, out, in, chi, gauge, fclover,
                                              coeff_s, coeff_t_f, coeff_t_b, prec_mass_rho, stream);
+}
+
+// ---- operators on QPhiX FourSpinorBlock arrays, in place -------------------
+int qphix_amd_qphix_dslash(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon, int isign,
+                           int cb, void *out, const void *in, const void *gauge,
+                           const unsigned char *accumulate, double coeff_s, double coeff_t_f,
+                           double coeff_t_b, void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_WILSON, false, isign, cb, out, in, nullptr, gauge, nullptr,
+                           coeff_s, coeff_t_f, coeff_t_b);
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_dslash_achimbdpsi(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+                                      int isign, int cb, void *out, const void *in, const void *chi,
+                                      const void *gauge, const unsigned char *accumulate, double alpha,
+                                      double coeff_s, double coeff_t_f, double coeff_t_b, void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_WILSON, true, isign, cb, out, in, chi, gauge, nullptr, coeff_s,
+                           coeff_t_f, coeff_t_b);
+  c.alpha = alpha;
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_clov_dslash(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon, int isign,
+                                int cb, void *out, const void *in, const void *gauge, const void *clover,
+                                const unsigned char *accumulate, double coeff_s, double coeff_t_f,
+                                double coeff_t_b, void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_CLOVER, false, isign, cb, out, in, nullptr, gauge, clover,
+                           coeff_s, coeff_t_f, coeff_t_b);
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+                                           int isign, int cb, void *out, const void *in, const void *chi,
+                                           const void *gauge, const void *clover,
+                                           const unsigned char *accumulate, double coeff_s, double coeff_t_f,
+                                           double coeff_t_b, double prec_mass_rho, void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_CLOVER, true, isign, cb, out, in, chi, gauge, clover, coeff_s,
+                           coeff_t_f, coeff_t_b);
+  c.rho = prec_mass_rho;
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_tm_dslash(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon, int isign,
+                              int cb, void *out, const void *in, const void *gauge,
+                              const unsigned char *accumulate, double coeff_s, double coeff_t_f,
+                              double coeff_t_b, double mu, double muinv, void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_TM, false, isign, cb, out, in, nullptr, gauge, nullptr, coeff_s,
+                           coeff_t_f, coeff_t_b);
+  c.mu = mu;
+  c.muinv = muinv;
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_tm_dslash_achimbdpsi(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+                                         int isign, int cb, void *out, const void *in, const void *chi,
+                                         const void *gauge, const unsigned char *accumulate, double alpha,
+                                         double coeff_s, double coeff_t_f, double coeff_t_b, double mu,
+                                         void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_TM, true, isign, cb, out, in, chi, gauge, nullptr, coeff_s,
+                           coeff_t_f, coeff_t_b);
+  c.alpha = alpha;
+  c.mu = mu;
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_tm_clov_dslash(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+                                   int isign, int cb, void *out, const void *in, const void *gauge,
+                                   const void *fclover, const unsigned char *accumulate, double coeff_s,
+                                   double coeff_t_f, double coeff_t_b, void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_TMCLOVER, false, isign, cb, out, in, nullptr, gauge, fclover,
+                           coeff_s, coeff_t_f, coeff_t_b);
+  return run_qphix(ctx, lay, accumulate, c, stream);
+}
+
+int qphix_amd_qphix_tm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay,
+                                              int recon, int isign, int cb, void *out, const void *in,
+                                              const void *chi, const void *gauge, const void *fclover,
+                                              const unsigned char *accumulate, double coeff_s,
+                                              double coeff_t_f, double coeff_t_b, double prec_mass_rho,
+                                              void *stream)
+{
+  DslashCall c = make_call(0, recon, qamd::OP_TMCLOVER, true, isign, cb, out, in, chi, gauge, fclover,
+                           coeff_s, coeff_t_f, coeff_t_b);
+  c.rho = prec_mass_rho;
+  return run_qphix(ctx, lay, accumulate, c, stream);
 }
 
 }  // extern "C"

@@ -56,7 +56,17 @@ template <> __device__ __forceinline__ __hip_bfloat16 from_real<__hip_bfloat16>(
 template <typename S> __device__ __forceinline__ S from_realf(float v);
 template <> __device__ __forceinline__ double from_realf<double>(float v) { return v; }
 template <> __device__ __forceinline__ float from_realf<float>(float v) { return v; }
-template <> __device__ __forceinline__ __half from_realf<__half>(float v) { return __float2half(v); }
+// FP16 stores round the FP32 result to FP16: the value is made opaque first,
+// so the compiler cannot fuse the last multiply-add with the conversion into
+// one rounding straight to FP16 (v_fma_mixlo_f16).  The reference's
+// low-precision kernels compute in FP32 and then convert
+// (codegen/inst_sp_vec16.cc:129-135), and a fused form would round some
+// elements differently, depending on which outputs the compiler fuses.
+template <> __device__ __forceinline__ __half from_realf<__half>(float v)
+{
+  asm("" : "+v"(v));
+  return __float2half(v);
+}
 template <> __device__ __forceinline__ __hip_bfloat16 from_realf<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
 
 template <typename S> __device__ __forceinline__ S to_storage(double v) { return from_real<S>(v); }

@@ -71,6 +71,21 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.one = (R)1;
   p.norm = c.norm;
   p.step = c.step;
+  p.q_sh = c.q_sh;
+  p.q_nvecs = c.q_nvecs;
+  p.q_pxy = c.q_pxy;
+  p.q_pxyz = c.q_pxyz;
+  p.acc = c.acc;
+  if (c.spl) {
+    // QPhiX FourSpinorBlock arrays (FP64, FP32 or FP16, the reference's
+    // types): the x/y rows are not staged, their site order being blocked
+    if constexpr (std::is_same<S, __hip_bfloat16>::value) {
+      return hipErrorInvalidValue;
+    } else {
+      p.lds = 0;
+      return dispatch_dslash<S, 1>(c.recon, c.op, c.achi, c.isign, p, st);
+    }
+  }
   return dispatch_dslash<S>(c.recon, c.op, c.achi, c.isign, p, st);
 }
 

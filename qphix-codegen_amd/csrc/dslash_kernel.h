@@ -48,6 +48,11 @@ template <typename R> struct DslashParams {
   // when set (with norm): out holds a CG residual r, and the launch stores
   // r - (*step) * result in its place; norm then sums |r|^2
   const double *step;
+  // QPhiX FourSpinorBlock arrays (kernel template SPL = 1): soalen 1 << q_sh,
+  // block strides q_nvecs (y), q_pxy (z), q_pxyz (t)
+  int q_sh, q_nvecs, q_pxy, q_pxyz;
+  // per-site direction mask (bit d = 2 mu + fb), QPhiX site order, or null
+  const unsigned char *acc;
 };
 
 // What a launch does with its result at a site: store it; store it and sum
@@ -264,6 +269,91 @@ __device__ __forceinline__ void load_nbr_lds(const Vec16 *lds, int j, NbrSpinor<
   }
 }
 
+// ---- QPhiX FourSpinorBlock arrays, FT[3 colour][4 spin][2][soalen] -------
+// (codegen/include/decl_common.h:11; block t pxyz + z pxy + y nvecs +
+// xh / soalen, lane xh % soalen).  Element (spin s, colour c, re/im r) of the
+// site whose element 0 sits at q is at q + ((c * 4 + s) * 2 + r) * soalen: a
+// wave's load of one element reads soalen consecutive values per block.
+template <typename R>
+__device__ __forceinline__ int qphix_site(const DslashParams<R> &p, int xh, int y, int z, int t)
+{
+  const int blk = t * p.q_pxyz + z * p.q_pxy + y * p.q_nvecs + (xh >> p.q_sh);
+  return ((blk * 24) << p.q_sh) + (xh & ((1 << p.q_sh) - 1));
+}
+
+template <typename S, typename R>
+__device__ __forceinline__ void load_nbr_q(const S *__restrict__ in, int q, int sh, NbrSpinor<S, R> &n)
+{
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        const S v = in[q + (((c * 4 + sp) * 2 + r) << sh)];
+        if constexpr (std::is_same<S, __half>::value)
+          n.v[(sp * 3 + c) * 2 + r] = v;
+        else
+          n.v[(sp * 3 + c) * 2 + r] = (R)to_real(v);
+      }
+}
+
+// Non-temporal scalar access; 16-bit storage types go through their bits.
+template <typename S> __device__ __forceinline__ S nt_load(const S *a)
+{
+  if constexpr (sizeof(S) == 2) {
+    const unsigned short u = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(a));
+    S v;
+    __builtin_memcpy(&v, &u, 2);
+    return v;
+  } else {
+    return __builtin_nontemporal_load(a);
+  }
+}
+
+template <typename S> __device__ __forceinline__ void nt_store(S v, S *a)
+{
+  if constexpr (sizeof(S) == 2) {
+    unsigned short u;
+    __builtin_memcpy(&u, &v, 2);
+    __builtin_nontemporal_store(u, reinterpret_cast<unsigned short *>(a));
+  } else {
+    __builtin_nontemporal_store(v, a);
+  }
+}
+
+template <typename S, typename R, bool NT = false>
+__device__ __forceinline__ void load_q_reals(const S *__restrict__ base, int q, int sh, R *out)
+{
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        const S *a = base + q + (((c * 4 + sp) * 2 + r) << sh);
+        out[(sp * 3 + c) * 2 + r] = (R)to_real((NT && QAMD_NT_STREAM) ? nt_load(a) : *a);
+      }
+}
+
+template <typename S, typename R>
+__device__ __forceinline__ void store_q_reals(S *__restrict__ base, int q, int sh, const R *in)
+{
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        S *a = base + q + (((c * 4 + sp) * 2 + r) << sh);
+        const S v = to_storage<S>(in[(sp * 3 + c) * 2 + r]);
+        if (QAMD_NT_STREAM)
+          nt_store(v, a);
+        else
+          *a = v;
+      }
+}
+
 // (1 + SGN*gamma_MU) psi from raw FP16 components: every projected real is
 // a + s b = fma(b, s, a) with s = +-one, exact products and one rounding,
 // the same value as the FP32 add of the converted components.
@@ -321,17 +411,25 @@ __device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const NbrSpinor<S, R>
   reconstruct<MU, SGN>(acc, w, beta);
 }
 
-template <typename S, int RECON, int MU, int ISIGN, typename R>
+// nb, nf: the neighbours' native site indices, or (SPL) their element-0
+// offsets in a QPhiX FourSpinorBlock array of soalen 1 << sh
+template <typename S, int RECON, int MU, int ISIGN, int SPL = 0, typename R>
 __device__ __forceinline__ void hop_mu(cx<R> acc[4][3], const S *__restrict__ in,
                                        const S *__restrict__ gauge, int vcb, int site, int nb,
-                                       int nf, R bb, R bf, bool tbc, cx<R> phase, R one)
+                                       int nf, R bb, R bf, bool tbc, cx<R> phase, R one, int sh = 0)
 {
   cx<R> ub[3][3], uf[3][3];
   load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf, -R(0) * one);
   NbrSpinor<S, R> n;
-  load_nbr(in, vcb, nb, n);
+  if constexpr (SPL)
+    load_nbr_q(in, nb, sh, n);
+  else
+    load_nbr(in, vcb, nb, n);
   hop_apply<S, MU, 0, ISIGN>(acc, n, ub, bb, tbc, conj(phase), one);
-  load_nbr(in, vcb, nf, n);
+  if constexpr (SPL)
+    load_nbr_q(in, nf, sh, n);
+  else
+    load_nbr(in, vcb, nf, n);
   hop_apply<S, MU, 1, ISIGN>(acc, n, uf, bf, tbc, phase, one);
 }
 
@@ -440,7 +538,11 @@ __device__ __forceinline__ void twist(const cx<R> in[4][3], cx<R> out[4][3], R a
 // for the clover and twisted-mass-clover achimbdpsi kernels, the second half
 // of the even-odd operator M: EPI_NORM on M p gives the CG's
 // <p, M^dag M p> = |M p|^2, EPI_RESIDUAL on M^dag (M p) its residual update.
-template <typename S, int RECON, int OP, bool ACHI, int ISIGN, int EPI = EPI_STORE>
+// SPL = 1 reads in and chi and writes out as QPhiX FourSpinorBlock arrays
+// (the links and clover blocks stay native), with the optional per-site
+// direction mask p.acc: a QPhiX host's arrays used in place, the reference's
+// body with its accumulate[] lane masks (codegen/dslash.cc:248,285-290).
+template <typename S, int RECON, int OP, bool ACHI, int ISIGN, int EPI = EPI_STORE, int SPL = 0>
 __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {
   using R = typename Traits<S>::R;
@@ -504,14 +606,33 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   // reference's accumulate[d] = 0 on boundary blocks, codegen/dslash.cc:248).
   const R bs = p.coeff_s;
   const int cm = p.comm;
-  const R bxb = ((cm & 1) && c.x == 0) ? R(0) : bs;
-  const R bxf = ((cm & 1) && c.x == g.Lx - 1) ? R(0) : bs;
-  const R byb = ((cm & 2) && c.y == 0) ? R(0) : bs;
-  const R byf = ((cm & 2) && c.y == g.Ly - 1) ? R(0) : bs;
-  const R bzb = ((cm & 4) && c.z == 0) ? R(0) : bs;
-  const R bzf = ((cm & 4) && c.z == g.Lz - 1) ? R(0) : bs;
-  const R btb = c.t == 0 ? ((cm & 8) ? R(0) : p.coeff_tb * p.tbound) : p.coeff_tb;
-  const R btf = c.t == g.Lt - 1 ? ((cm & 8) ? R(0) : p.coeff_tf * p.tbound) : p.coeff_tf;
+  R bxb = ((cm & 1) && c.x == 0) ? R(0) : bs;
+  R bxf = ((cm & 1) && c.x == g.Lx - 1) ? R(0) : bs;
+  R byb = ((cm & 2) && c.y == 0) ? R(0) : bs;
+  R byf = ((cm & 2) && c.y == g.Ly - 1) ? R(0) : bs;
+  R bzb = ((cm & 4) && c.z == 0) ? R(0) : bs;
+  R bzf = ((cm & 4) && c.z == g.Lz - 1) ? R(0) : bs;
+  R btb = c.t == 0 ? ((cm & 8) ? R(0) : p.coeff_tb * p.tbound) : p.coeff_tb;
+  R btf = c.t == g.Lt - 1 ? ((cm & 8) ? R(0) : p.coeff_tf * p.tbound) : p.coeff_tf;
+  // QPhiX arrays: element-0 offsets of this site and of its neighbours
+  int q0 = 0;
+  if constexpr (SPL) {
+    q0 = qphix_site(p, c.xh, c.y, c.z, c.t);
+    if (p.acc) {
+      // the reference's accumulate[d]: a cleared bit drops direction d here
+      // (a zero coefficient adds exact zeros, the neighbour being finite)
+      const int blk = p.q_pxyz * c.t + p.q_pxy * c.z + p.q_nvecs * c.y + (c.xh >> p.q_sh);
+      const unsigned m = p.acc[(blk << p.q_sh) + (c.xh & ((1 << p.q_sh) - 1))];
+      bxb = (m & 1) ? bxb : R(0);
+      bxf = (m & 2) ? bxf : R(0);
+      byb = (m & 4) ? byb : R(0);
+      byf = (m & 8) ? byf : R(0);
+      bzb = (m & 16) ? bzb : R(0);
+      bzf = (m & 32) ? bzf : R(0);
+      btb = (m & 64) ? btb : R(0);
+      btf = (m & 128) ? btf : R(0);
+    }
+  }
 
   cx<R> acc[4][3];
 #pragma unroll
@@ -522,6 +643,20 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const int tm = p.tbc_mask;
   const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
   const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
+  if constexpr (SPL) {
+    const int yb = c.y == 0 ? g.Ly - 1 : c.y - 1, yf = c.y == g.Ly - 1 ? 0 : c.y + 1;
+    const int zb = c.z == 0 ? g.Lz - 1 : c.z - 1, zf = c.z == g.Lz - 1 ? 0 : c.z + 1;
+    const int tb = c.t == 0 ? g.Lt - 1 : c.t - 1, tf = c.t == g.Lt - 1 ? 0 : c.t + 1;
+    const int sh = p.q_sh;
+    hop_mu<S, RECON, 0, ISIGN, 1>(acc, in, gauge, vcb, idx, qphix_site(p, xb >> 1, c.y, c.z, c.t),
+                                  qphix_site(p, xf >> 1, c.y, c.z, c.t), bxb, bxf, tm & 1, ph0, p.one, sh);
+    hop_mu<S, RECON, 1, ISIGN, 1>(acc, in, gauge, vcb, idx, qphix_site(p, c.xh, yb, c.z, c.t),
+                                  qphix_site(p, c.xh, yf, c.z, c.t), byb, byf, tm & 2, ph1, p.one, sh);
+    hop_mu<S, RECON, 2, ISIGN, 1>(acc, in, gauge, vcb, idx, qphix_site(p, c.xh, c.y, zb, c.t),
+                                  qphix_site(p, c.xh, c.y, zf, c.t), bzb, bzf, tm & 4, ph2, p.one, sh);
+    hop_mu<S, RECON, 3, ISIGN, 1>(acc, in, gauge, vcb, idx, qphix_site(p, c.xh, c.y, c.z, tb),
+                                  qphix_site(p, c.xh, c.y, c.z, tf), btb, btf, tm & 8, ph3, p.one, sh);
+  } else {
 #if QAMD_LDS_XY
   // A work-group inside one (z, t) plane stages its x rows y0 .. y1 and the
   // rows y0 - 1 and y1 + 1 (y periodic) once; its x and y hops then read
@@ -566,6 +701,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   }
   hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bzb, bzf, tm & 4, ph2, p.one);
   hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3, p.one);
+  }
 
   cx<R> res[4][3];
   if (!ACHI) {
@@ -583,7 +719,10 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     }
   } else {
     R chr[24];
-    load_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
+    if constexpr (SPL)
+      load_q_reals<S, R, true>(static_cast<const S *>(p.chi), q0, p.q_sh, chr);
+    else
+      load_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
     cx<R> chi[4][3];
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -632,7 +771,10 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
 #pragma unroll
     for (int k = 0; k < 24; k++) o[k] = rv[k] - a * o[k];
   }
-  store_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
+  if constexpr (SPL)
+    store_q_reals<S, R>(static_cast<S *>(p.out), q0, p.q_sh, o);
+  else
+    store_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
   if constexpr (EPI != EPI_STORE) {
     // |out|^2 of the stored (rounded) values: lanes in FP32/FP64, the wave
     // and the work-group in FP64 in a fixed order
@@ -658,10 +800,16 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
 }
 
 // Host-side launch for one instantiation.
-template <typename S, int RECON, int OP, bool ACHI, int ISIGN>
+template <typename S, int RECON, int OP, bool ACHI, int ISIGN, int SPL = 0>
 hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream_t stream)
 {
   const int grid = xcd_grid(p.nblocks);
+  if constexpr (SPL) {  // QPhiX arrays: plain store only
+    if (p.norm || p.step) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_STORE, 1>), dim3(grid), dim3(kDslashBlock),
+                       0, stream, p);
+    return hipGetLastError();
+  }
   if constexpr (ACHI && (OP == OP_CLOVER || OP == OP_TMCLOVER)) {
     if (p.norm && !p.step) {
       hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_NORM>), dim3(grid), dim3(kDslashBlock),
@@ -683,14 +831,15 @@ hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream
   return hipGetLastError();
 }
 
-// Run-time dispatch over (recon, op, achi, isign) for one storage type.
-template <typename S>
+// Run-time dispatch over (recon, op, achi, isign) for one storage type;
+// SPL = 1 for QPhiX spinor arrays.
+template <typename S, int SPL = 0>
 hipError_t dispatch_dslash(int recon, int op, bool achi, int isign,
                            const DslashParams<typename Traits<S>::R> &p, hipStream_t st)
 {
 #define QAMD_ISIGN(RC, OPV, AC)                                              \
-  return isign > 0 ? launch_dslash<S, RC, OPV, AC, 1>(p, st)                 \
-                   : launch_dslash<S, RC, OPV, AC, -1>(p, st);
+  return isign > 0 ? launch_dslash<S, RC, OPV, AC, 1, SPL>(p, st)            \
+                   : launch_dslash<S, RC, OPV, AC, -1, SPL>(p, st);
 #define QAMD_ACHI(RC, OPV)                                                   \
   if (achi) {                                                                \
     QAMD_ISIGN(RC, OPV, true)                                                \

@@ -28,7 +28,25 @@ struct DslashCall {
   // optional, with norm, for the clover achimbdpsi kernels with isign -1:
   // out holds a residual r and receives r - (*step) * result
   const double *step;
+  // spl = 1: in, chi and out are QPhiX FourSpinorBlock arrays of the
+  // geometry q (converted from a qphix_amd_qphix_layout); acc, when set, is
+  // the per-site direction mask in their site order (block * soalen + lane)
+  int spl;
+  int q_sh, q_nvecs, q_pxy, q_pxyz;
+  const unsigned char *acc;
 };
+
+// FourSpinorBlock geometry of a QPhiX layout on a lattice, for the
+// operator kernels that read and write those arrays in place: soalen =
+// 1 << sh, block strides nvecs (y), pxy (z), pxyz (t) in blocks, and the
+// number of site slots (padding included).
+struct QSpinorGeom {
+  int sh, nvecs, pxy, pxyz;
+  long sites;
+};
+// false for a layout the lattice does not admit, a soalen that is not a
+// power of two, or arrays of 2^31 elements or more
+bool qphix_spinor_geometry(const qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, QSpinorGeom &out);
 
 // The work-group traversal order the operator calls use on this geometry.
 int dslash_traversal(const qphix_amd_ctx *ctx);

@@ -295,6 +295,24 @@ int entry(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int field, int 
 }
 
 }  // namespace
+
+bool qphix_spinor_geometry(const qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, QSpinorGeom &out)
+{
+  QLay q;
+  if (!make_layout(ctx, lay, q)) return false;
+  int sh = 0;
+  while ((1 << sh) < q.soalen) sh++;
+  if ((1 << sh) != q.soalen) return false;  // the operator kernels shift by log2(soalen)
+  const long elems = (long)ctx->g.Lt * q.pxyz * 24 * q.soalen;
+  if (elems >= (1L << 31)) return false;  // 32-bit element offsets
+  out.sh = sh;
+  out.nvecs = q.nvecs;
+  out.pxy = (int)q.pxy;
+  out.pxyz = (int)q.pxyz;
+  out.sites = elems / 24;
+  return true;
+}
+
 }  // namespace qamd
 
 extern "C" {

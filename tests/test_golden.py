@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from _util import cb_mask, f16_ulps, f16_within, recon12_links
+from _util import accumulate_mask, cb_mask, f16_ulps, f16_within, masked_oracle, recon12_links
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -201,52 +201,6 @@ def test_f16_agreement_criterion_rejects_real_errors():
     near0[3] = np.float16(1e-4 + 4e-7)  # about 9 FP16 ulps of 1e-4
     assert f16_within(near0, want).all()
     assert not f16_within(np.float16(1e-4 + 2e-4) * np.ones(4, np.float16), want)[3]
-
-
-def accumulate_mask(dims, cb, soalen, veclen, seed):
-    """A per-site direction mask (one byte per lexicographic site, bit d =
-    2 mu + fb) that the reference's bodies can express: random per site in
-    x, and in y when soalen < veclen (the directions they mask lane by lane,
-    codegen/dslash.cc:109,285-290,570-578); the same for every site of a
-    body call's vector (ngy rows x soalen lanes) otherwise, since the bodies
-    gate those directions per call with `if (accumulate[d])`."""
-    Lx, Ly, Lz, Lt = dims
-    ngy, nvecs = veclen // soalen, Lx // 2 // soalen
-    rng = np.random.default_rng(seed)
-    t, z, y, xh = np.meshgrid(np.arange(Lt), np.arange(Lz), np.arange(Ly), np.arange(Lx // 2),
-                              indexing="ij")
-    t, z, y, xh = t.ravel(), z.ravel(), y.ravel(), xh.ravel()
-    lex = ((t * Lz + z) * Ly + y) * Lx + 2 * xh + ((cb + y + z + t) & 1)
-    vec = ((t * Lz + z) * (Ly // ngy) + y // ngy) * nvecs + xh // soalen
-    nvec = vec.max() + 1
-    mask = np.zeros(Lx * Ly * Lz * Lt, dtype=np.uint8)
-    for d in range(8):
-        per_site = d < 2 or (d < 4 and soalen < veclen)
-        bits = rng.integers(0, 2, len(lex)) if per_site else rng.integers(0, 2, nvec)[vec]
-        mask[lex] |= (bits << d).astype(np.uint8)
-    return mask
-
-
-def masked_oracle(orc, dims, psi, U, isign, cb, mask, **kw):
-    """The operator with direction d dropped at the sites whose mask bit d is
-    clear: the oracle's result split exactly into its 8 direction terms
-    (links of one dimension kept, the others zeroed; forward and backward
-    hops separated by a twisted phase i in that dimension, forward e^{i pi/2},
-    backward its conjugate), then summed under the mask."""
-    c = lambda a: a[..., 0] + 1j * a[..., 1]
-    base = c(orc.dslash(dims, psi, np.zeros_like(U), isign, cb, **kw))  # the a chi part
-    total = base.copy()
-    bits = (mask.astype(np.int64)[:, None] >> np.arange(8)[None, :]) & 1  # (V, 8)
-    for mu in range(4):
-        Um = np.zeros_like(U)
-        Um[:, mu] = U[:, mu]
-        tbc = [int(m == mu) for m in range(4)]
-        h0 = c(orc.dslash(dims, psi, Um, isign, cb, **kw)) - base
-        h1 = c(orc.dslash(dims, psi, Um, isign, cb, tbc=tbc, tbc_phases=[1j if m == mu else 1 for m in range(4)],
-                          **kw)) - base
-        fwd, bwd = (h0 - 1j * h1) / 2, (h0 + 1j * h1) / 2
-        total += bits[:, 2 * mu, None, None] * bwd + bits[:, 2 * mu + 1, None, None] * fwd
-    return np.stack([total.real, total.imag], axis=-1)
 
 
 @pytest.mark.parametrize("config", ["avx512_f64", "avx512_f64_s4", "avx512_f32_s8", "avx2_f64"])

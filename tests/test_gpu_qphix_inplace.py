@@ -1,0 +1,220 @@
+"""GPU tests of the operators on QPhiX FourSpinorBlock arrays in place
+(qphix_amd_qphix_*), called through the C ABI the way a QPhiX host would:
+the spinor arrays are the reference driver's own blocks (oracle/ref_driver.cpp
+packs them for the reference's kernels) handed over as they are, the links
+and clover blocks are converted once from the driver's SU3MatrixBlock /
+CloverBlock / FullCloverBlock arrays, and the result is written into a
+FourSpinorBlock array.
+
+* live against the reference's AVX-512 kernels (oracle/_ref) with
+  non-trivial accumulate masks (x lane by lane, y too when soalen < veclen,
+  z and t per vector, the patterns its bodies express): every family x
+  {dslash, achimbdpsi} x isign x 12/18-real links, soalen == veclen and the
+  xy-packed soalen = veclen / 2, FP64 (1e-13 relative) and FP32 (1e-6);
+  FP16 storage against the reference's low-precision build (within one
+  FP16 unit in the last place, and >= 99% of the elements bit-identical);
+* without a mask, bit-identical to the native-layout kernels on the same
+  data (the in-place path changes addressing only).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import qphix_blocks as qb
+from _util import accumulate_mask, f16_ulps, f16_within
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+DIMS = (32, 8, 4, 4)
+# scalars exact in FP16, so the low-precision build (which reads its scalars
+# as FP16) sees the same values
+KW0 = dict(alpha=3.75, mu=0.03125, muinv=0.25, rho=0.125, coeff_s=0.875, coeff_t_f=0.75, coeff_t_b=1.25,
+           t_boundary=-1.0)
+TORCH_FT = {0: torch.float64, 1: torch.float32, 2: torch.float16}
+NATIVE = {(0, False): "qphix_amd_dslash", (0, True): "qphix_amd_dslash_achimbdpsi",
+          (1, False): "qphix_amd_clov_dslash", (1, True): "qphix_amd_clov_dslash_achimbdpsi",
+          (2, False): "qphix_amd_tm_dslash", (2, True): "qphix_amd_tm_dslash_achimbdpsi",
+          (3, False): "qphix_amd_tm_clov_dslash", (3, True): "qphix_amd_tm_clov_dslash_achimbdpsi"}
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def call(qamd, geom, kind, achi, recon, isign, cb, out, psi, chi, g, cl, lay=None, acc=None):
+    """One operator call through the C ABI: on QPhiX arrays in place when
+    lay is given (qphix_amd_qphix_*), else on native spinors."""
+    l, kw = qamd.lib(), KW0
+    cs, ctf, ctb = kw["coeff_s"], kw["coeff_t_f"], kw["coeff_t_b"]
+    if lay is not None:
+        fn = getattr(l, NATIVE[(kind, achi)].replace("qphix_amd_", "qphix_amd_qphix_"))
+        head = (geom.ctx, ctypes.byref(lay), recon, isign, cb, _p(out), _p(psi))
+        a = _p(acc)
+    else:
+        fn = getattr(l, NATIVE[(kind, achi)])
+        head = (geom.ctx, g.prec, recon, isign, cb, _p(out), _p(psi))
+    mid = []
+    if achi:
+        mid.append(_p(chi))
+    mid.append(g.ptr())
+    if kind in (1, 3):
+        mid.append(cl.ptr())
+    if lay is not None:
+        mid.append(a)
+    if kind == 0:
+        tail = ((kw["alpha"],) if achi else ()) + (cs, ctf, ctb)
+    elif kind == 2:
+        tail = (kw["alpha"], cs, ctf, ctb, kw["mu"]) if achi else (cs, ctf, ctb, kw["mu"], kw["muinv"])
+    else:
+        tail = (cs, ctf, ctb) + ((kw["rho"],) if achi else ())
+    st = fn(*head, *mid, *tail, _stream())
+    assert st == 0, (NATIVE[(kind, achi)], lay is not None, st)
+
+
+def epilogue(L, clt, kind, achi, isign):
+    if kind == 1:
+        return clt.numpy() if achi else L.invert_clover(clt).numpy()
+    if kind == 3:
+        return L.tmclover_blocks(clt, 0.1, isign, not achi).numpy()
+    return None
+
+
+def qphix_mask(lay, dims, cb, mask):
+    """The per-site mask in the output array's site order (block * soalen + lane)."""
+    blk, lane, lex = qb.spinor_slots(lay, dims, cb)
+    out = np.zeros(lay.n_spinor_blocks(dims) * lay.soalen, dtype=np.uint8)
+    out[blk * lay.soalen + lane] = mask[lex]
+    return out
+
+
+def to_dev(a, ft):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV).view(TORCH_FT[ft])
+
+
+@pytest.fixture(scope="module")
+def fields(qamd):
+    L = qamd.lattice
+    return dict(U=L.random_gauge(DIMS, 0.3, 61).numpy(), psi=L.random_spinor(DIMS, 62).numpy(),
+                chi=L.random_spinor(DIMS, 63).numpy(), clt=L.random_clover(DIMS, 64))
+
+
+CONFIGS = ["avx512_f64", "avx512_f64_s4", "avx512_f32", "avx512_f32_s8", "avx512_f16"]
+
+
+@pytest.mark.parametrize("config", CONFIGS)
+def test_inplace_equals_reference_kernels_with_masks(qamd, fields, config):
+    from oracle import ref
+    if not ref.available(config):
+        pytest.skip("oracle/_ref not built")
+    L = qamd.lattice
+    S, V = ref.soalen(config), ref.veclen(config)
+    ft = 2 if ref.lib(config).ref_lowprec() else (1 if ref.lib(config).ref_precision() == 1 else 0)
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    U, psi, chi, clt = fields["U"], fields["psi"], fields["chi"], fields["clt"]
+    worst = 0.0
+    for kind in range(4):
+        for achi in (False, True):
+            for isign, cb in ((1, 0), (-1, 1)):
+                clov = epilogue(L, clt, kind, achi, isign)
+                kw = dict(KW0, kind=kind, achi=achi, chi=chi, clov=clov)
+                mask = accumulate_mask(DIMS, cb, S, V, 7 * kind + 3 * achi + cb)
+                lay_q = qb.Layout(ft, V, S)
+                acc = torch.from_numpy(qphix_mask(lay_q, DIMS, cb, mask)).to(DEV)
+                psi_q = to_dev(ref.pack(config, 0, DIMS, psi, 1 - cb), ft)
+                chi_q = to_dev(ref.pack(config, 0, DIMS, chi, cb), ft)
+                for recon in (12, 18):
+                    print(f"case in-place {config} kind={kind} achi={achi} isign={isign} recon={recon}",
+                          flush=True)
+                    lay = qamd.qphix_layout(ft, V, S, compress12=recon == 12)
+                    g = geom.gauge_from_qphix(to_dev(ref.pack(config, 1, DIMS, U, cb, recon), ft), lay, ft,
+                                              recon, cb)
+                    cl = None
+                    if kind == 1:
+                        cl = geom.clover_from_qphix(to_dev(ref.pack(config, 2, DIMS, clov, cb), ft), lay, ft, cb)
+                    if kind == 3:
+                        cl = geom.full_clover_from_qphix(to_dev(ref.pack(config, 3, DIMS, clov, cb), ft), lay,
+                                                         ft, cb)
+                    out_q = torch.zeros_like(psi_q)
+                    call(qamd, geom, kind, achi, recon, isign, cb, out_q, psi_q, chi_q, g, cl, lay=lay, acc=acc)
+                    got = out_q.cpu().numpy()
+                    want_lex, _ = ref.apply(config, DIMS, psi, U, isign, cb, recon=recon, mask=mask, **kw)
+                    want = ref.pack(config, 0, DIMS, want_lex, cb)
+                    if ft == 2:
+                        assert f16_within(got, want).all(), (kind, achi, isign, recon)
+                        same = float(np.mean(f16_ulps(got, want) == 0))
+                        assert same >= 0.99, (kind, achi, isign, recon, same)
+                        worst = max(worst, 1.0 - same)
+                    else:
+                        err = float(np.abs(got.astype(np.float64) - want).max() / np.abs(want).max())
+                        assert err < (1e-13 if ft == 0 else 1e-6), (kind, achi, isign, recon, err)
+                        worst = max(worst, err)
+    print(f"{config}: worst {'fraction of differing FP16 elements' if ft == 2 else 'relative error'} "
+          f"{worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("ft,V,S", [(0, 8, 4), (1, 16, 16), (2, 16, 8)], ids=["f64_s4", "f32_s16", "f16_s8"])
+def test_inplace_bit_identical_to_native(qamd, fields, ft, V, S):
+    """No mask, padded layouts: the in-place kernels give exactly the bits of
+    the native kernels on the same data (converted with the layout
+    converters), for every family and kernel, both isigns, 12/18 links."""
+    L = qamd.lattice
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    U, psi, chi, clt = (torch.from_numpy(fields[k]).to(DEV) if k != "clt" else fields[k]
+                        for k in ("U", "psi", "chi", "clt"))
+    lay = qamd.qphix_layout(ft, V, S, True, pad_xy=V // S, pad_xyz=2 * (V // S))
+    for kind in range(4):
+        for achi in (False, True):
+            for isign, cb in ((1, 0), (-1, 1)):
+                clov = epilogue(L, clt, kind, achi, isign)
+                for recon in (12, 18):
+                    g = geom.pack_gauge(U, ft, recon, cb)
+                    cl = None
+                    if kind == 1:
+                        cl = geom.pack_clover(torch.from_numpy(clov).to(DEV), ft, cb)
+                    if kind == 3:
+                        cl = geom.pack_full_clover(torch.from_numpy(clov).to(DEV), ft, cb)
+                    psi_n = geom.pack_spinor(psi, ft, 1 - cb)
+                    chi_n = geom.pack_spinor(chi, ft, cb)
+                    out_n = geom.new_spinor(ft, cb=cb)
+                    call(qamd, geom, kind, achi, recon, isign, cb, out_n.data, psi_n.data, chi_n.data, g, cl)
+                    psi_q = geom.spinor_to_qphix(psi_n, lay)
+                    chi_q = geom.spinor_to_qphix(chi_n, lay)
+                    out_q = torch.full_like(psi_q, float("nan"))
+                    call(qamd, geom, kind, achi, recon, isign, cb, out_q, psi_q, chi_q, g, cl, lay=lay)
+                    want = geom.spinor_to_qphix(out_n, lay, out=torch.full_like(psi_q, float("nan")))
+                    # padding slots are neither read nor written: NaN in both
+                    assert torch.equal(out_q.nan_to_num(7.0), want.nan_to_num(7.0)), (kind, achi, isign, recon)
+
+
+def test_inplace_argument_checks(qamd):
+    """Layouts the kernels cannot address and bad pointers are refused before
+    any launch."""
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    l = qamd.lib()
+    buf = torch.zeros(1 << 16, dtype=torch.float32, device=DEV)
+    p = _p(buf)
+    q = ctypes.c_void_p(buf.data_ptr() + 4096)
+    r = ctypes.c_void_p(buf.data_ptr() + 8192)
+    st = _stream()
+    ok = qamd.qphix_layout(1, 16, 16, True)
+    bf16 = qamd.qphix_layout(3, 16, 16, True)  # no QPhiX FT
+    assert l.qphix_amd_qphix_dslash(geom.ctx, ctypes.byref(bf16), 12, 1, 0, p, q, r, None, 1.0, 1.0, 1.0, st) == -1
+    g24 = qamd.Geometry((24, 8, 4, 4), t_boundary=-1.0)  # Lx/2 = 12: a valid layout, soalen 12 not 2^k
+    s12 = qamd.qphix_layout(1, 12, 12, True)
+    assert l.qphix_amd_qphix_dslash(g24.ctx, ctypes.byref(s12), 12, 1, 0, p, q, r, None, 1.0, 1.0, 1.0, st) == -1
+    assert l.qphix_amd_qphix_dslash(geom.ctx, ctypes.byref(ok), 12, 1, 0, p, p, r, None, 1.0, 1.0, 1.0, st) == -1
+    assert l.qphix_amd_qphix_dslash(geom.ctx, ctypes.byref(ok), 12, 2, 0, p, q, r, None, 1.0, 1.0, 1.0, st) == -1
+    assert l.qphix_amd_qphix_clov_dslash(geom.ctx, ctypes.byref(ok), 12, 1, 0, p, q, r, None, None, 1.0, 1.0,
+                                         1.0, st) == -1
+    odd = ctypes.c_void_p(buf.data_ptr() + 2)
+    assert l.qphix_amd_qphix_dslash(geom.ctx, ctypes.byref(ok), 12, 1, 0, odd, q, r, None, 1.0, 1.0, 1.0,
+                                    st) == -6
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(buf) == 0
