@@ -226,6 +226,33 @@ def tm_kernel(q, geom, s, dev, n):
             "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4), "flop_per_site": 1392}
 
 
+def inplace_kernels(q, geom, s, n):
+    """clov_dslash FP32, 12-real links, reading and writing QPhiX
+    FourSpinorBlock arrays in place (qphix_amd_qphix_clov_dslash; the links
+    and clover blocks are the native launch's) for the reference's AVX-512
+    FP32 layouts, veclen 16 with soalen 16, 8 and 4: HIP-event average and
+    fraction of 8 TB/s by the same algorithmic bytes as the native kernel."""
+    import torch
+    out = {}
+    op = q.ClovDslash(geom)
+    b = algorithmic_bytes(12, 72, False, BYTES_F32) * geom.vcb
+    for S in (16, 8, 4):
+        lay = q.qphix_layout(q.F32, 16, S, True)
+        psi_q = q.QPhiXSpinor(geom.spinor_to_qphix(s["psi1"], lay), lay)
+        res_q = q.QPhiXSpinor(torch.empty_like(psi_q.data), lay)
+
+        def k(i):
+            op.dslash(res_q, psi_q, s["g0"], s["inv0"], 1, 0)
+
+        for i in range(3):
+            k(i)
+        t = time_kernel(k, n)
+        out[f"soalen{S}"] = {"us": round(t * 1e6, 2), "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}
+        del psi_q, res_q
+    torch.cuda.empty_cache()
+    return out
+
+
 def _err(out, ref):
     d = (out - ref).abs().max().item()
     return {"max_abs": d, "max_abs_rel": d / ref.abs().max().item()}
@@ -509,9 +536,9 @@ def main():
                     help="clover32: the 32^3x64 clover operator (the headline line, with the "
                          "48^3x96 block); tmclover48: the 48^3x96 twisted-mass clover block alone")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="launch clov_dslash, clov_dslash_achimbdpsi and Wilson dslash 10 times "
-                         "each plus a 1 GiB device copy (for rocprofv3 --pmc runs and their "
-                         "copy-kernel calibration), print nothing")
+                    help="launch clov_dslash, clov_dslash_achimbdpsi, Wilson dslash and clov_dslash "
+                         "on QPhiX arrays in place 10 times each plus a 1 GiB device copy (for "
+                         "rocprofv3 --pmc runs and their copy-kernel calibration), print nothing")
     ap.add_argument("--lattice", default=None,
                     help="with --kernels-only: Lx,Ly,Lz,Lt (default 32,32,32,64)")
     ap.add_argument("--traversal", type=int, default=-1,
@@ -579,6 +606,13 @@ def main():
         for i in range(10):
             s = sets[i & 1]
             wil.dslash(s["t0"], s["psi1"], s["g0"], 1, 0)
+        # clov_dslash on QPhiX arrays in place (soalen = veclen = 16)
+        lay = q.qphix_layout(q.F32, 16, 16, True)
+        psi_q = q.QPhiXSpinor(geom.spinor_to_qphix(sets[0]["psi1"], lay), lay)
+        res_q = q.QPhiXSpinor(torch.empty_like(psi_q.data), lay)
+        for i in range(10):
+            op.dslash(res_q, psi_q, sets[0]["g0"], sets[0]["inv0"], 1, 0)
+        del psi_q, res_q
         del sets
         a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
@@ -604,6 +638,7 @@ def main():
     t_d = time_kernel(k_dslash, args.steps)
     t_a = time_kernel(k_achi, args.steps)
     tm = tm_kernel(q, geom, sets[0], dev, args.steps)
+    inplace = inplace_kernels(q, geom, sets[0], args.steps)
     err = errors_vs_f64(q, geom, dev)
     # naive count (every neighbour spinor read from memory), as often quoted
     b_naive = (8 * 24 + 8 * recon + 72 + 24) * BYTES_F32 * vcb
@@ -639,6 +674,7 @@ def main():
                                 "clov_dslash_achimbdpsi": round(b_a / t_a / 1e9 / PEAK_HBM_GBS, 4)},
             "parallelism": "single GPU",
             "tm_dslash": tm,
+            "qphix_inplace_clov_dslash": inplace,
             "errors_vs_f64": err,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,

@@ -5,7 +5,7 @@ binding, operator classes mirroring QPhiX's, and synthetic lattices."""
 from ._lib import (BF16, F16, F32, F64, FIELD_CLOVER, FIELD_FULLCLOVER, FIELD_GAUGE,  # noqa: F401
                    FIELD_SPINOR, LIB_PATH, QPhiXAMDError, QPhiXLayout, lib)
 from .dslash import (PREC_NAME, TORCH_DTYPE, ClovDslash, Dslash, Field, Geometry,  # noqa: F401
-                     TMClovDslash, TMDslash, face_finish, face_pack, qphix_layout)
+                     QPhiXSpinor, TMClovDslash, TMDslash, face_finish, face_pack, qphix_layout)
 from . import lattice  # noqa: F401
 from .solver import (EvenOddCloverOperator, InvCG, clover_force, clover_from_links, clover_invert,  # noqa: F401
                      reunitarize, tmclover_blocks)

@@ -146,6 +146,10 @@ __device__ __forceinline__ void store_reals(S *__restrict__ base, int vcb,
 }
 
 // ---- complex arithmetic ---------------------------------------------------
+// A fused multiply-add in the arithmetic type itself (never a wider one).
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R> struct cx {
   R re, im;
 };
@@ -204,17 +208,21 @@ __device__ __forceinline__ cx<R> cdot3(cx<R> u0, cx<R> u1, cx<R> u2, cx<R> h0, c
   r.im = fma(u0.im, h0.re, r.im);
   return cmac(cmac(r, u1, h1), u2, h2);
 }
-// conj(p q - r s) (the third row of an SU(3) matrix from its first two)
+// conj(p q - r s) (the third row of an SU(3) matrix from its first two),
+// with the reference's rounding sequence (codegen/dslash_common.cc:
+// 701-718, Conj_CrossProd): the real part starts from p.re q.re, the
+// imaginary part from r.re s.im, one product rounded and three fused
+// multiply-adds each.
 template <typename R> __device__ __forceinline__ cx<R> conj_diff(cx<R> p, cx<R> q, cx<R> r, cx<R> s, R nz)
 {
   R re = fma(p.re, q.re, nz);
   re = fma(-p.im, q.im, re);
   re = fma(-r.re, s.re, re);
   re = fma(r.im, s.im, re);
-  R im = fma(-p.re, q.im, nz);
-  im = fma(-p.im, q.re, im);
-  im = fma(r.re, s.im, im);
+  R im = fma(r.re, s.im, nz);
   im = fma(r.im, s.re, im);
+  im = fma(-p.re, q.im, im);
+  im = fma(-p.im, q.re, im);
   return {re, im};
 }
 template <typename R> __device__ __forceinline__ cx<R> conj_diff(cx<R> p, cx<R> q, cx<R> r, cx<R> s)
@@ -223,11 +231,21 @@ template <typename R> __device__ __forceinline__ cx<R> conj_diff(cx<R> p, cx<R> 
   re = fma(-p.im, q.im, re);
   re = fma(-r.re, s.re, re);
   re = fma(r.im, s.im, re);
-  R im = -p.re * q.im;
-  im = fma(-p.im, q.re, im);
-  im = fma(r.re, s.im, im);
+  R im = r.re * s.im;
   im = fma(r.im, s.re, im);
+  im = fma(-p.re, q.im, im);
+  im = fma(-p.im, q.re, im);
   return {re, im};
+}
+// a b with one rounded product and one fused multiply-add per part, the
+// order the reference's complex multiplies use (codegen/dslash_common.cc:
+// 1188-1232, 1491-1524): re = fma(-a.im, b.im, a.re b.re),
+// im = fma(a.im, b.re, a.re b.im).
+template <typename R> __device__ __forceinline__ cx<R> cmul_ord(cx<R> a, cx<R> b)
+{
+  R re = a.re * b.re;
+  R im = a.re * b.im;
+  return {fma(-a.im, b.im, re), fma(a.im, b.re, im)};
 }
 
 // ---- checkerboard geometry ------------------------------------------------

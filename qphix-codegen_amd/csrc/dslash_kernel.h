@@ -406,7 +406,7 @@ __device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const NbrSpinor<S, R>
 #pragma unroll
     for (int k = 0; k < 2; k++)
 #pragma unroll
-      for (int i = 0; i < 3; i++) w[k][i] = phase * w[k][i];
+      for (int i = 0; i < 3; i++) w[k][i] = cmul_ord(phase, w[k][i]);
   }
   reconstruct<MU, SGN>(acc, w, beta);
 }
@@ -502,13 +502,32 @@ __device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
 #pragma unroll
     for (int r = 0; r < 6; r++) {
       cx<R> a0 = {blk[(r * 6) * 2], blk[(r * 6) * 2 + 1]};
-      cx<R> res = a0 * v[0];
+      cx<R> res = cmul_ord(a0, v[0]);
 #pragma unroll
       for (int c = 1; c < 6; c++) {
         cx<R> a = {blk[(r * 6 + c) * 2], blk[(r * 6 + c) * 2 + 1]};
         res = cmac(res, a, v[c]);
       }
       out[2 * b + r / 3][r % 3] = res;
+    }
+  }
+}
+
+// a (1 + i m g5) in, m real (g5 = +1 on spins 0,1, -1 on 2,3), in the
+// reference's order: (1 + i m g5) in by one fused multiply-add per part,
+// then the product by a (codegen/dslash_common.cc:1234-1489).  The
+// twisted-mass dslash applies muinv (1 - isign i mu g5), its achimbdpsi form
+// alpha (1 + isign i mu g5).
+template <typename R>
+__device__ __forceinline__ void twist_ord(const cx<R> in[4][3], cx<R> out[4][3], R a, R m)
+{
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const R ms = s < 2 ? m : -m;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const cx<R> v = in[s][c];
+      out[s][c] = {a * fma(-ms, v.im, v.re), a * fma(ms, v.re, v.im)};
     }
   }
 }
@@ -635,10 +654,64 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   }
 
   cx<R> acc[4][3];
+  // a chi - b D psi in the reference's order (codegen/dslash_common.cc:
+  // 1830-1932): the output starts as the chi term, and every direction's hop
+  // is then subtracted by its own fused multiply-adds (mbeta tables,
+  // codegen/dslash_common.cc:784-890): its coefficient is negated.  The chi
+  // term is formed just before the first hop (after the LDS barrier), so
+  // that it is not held across the staging loop.
+  auto start_acc = [&]() {
+  if constexpr (ACHI) {
+    R chr[24];
+    if constexpr (SPL)
+      load_q_reals<S, R, true>(static_cast<const S *>(p.chi), q0, p.q_sh, chr);
+    else
+      load_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
+    cx<R> chi[4][3];
 #pragma unroll
-  for (int s = 0; s < 4; s++)
+    for (int s = 0; s < 4; s++)
 #pragma unroll
-    for (int k = 0; k < 3; k++) acc[s][k] = {R(0), R(0)};
+      for (int k = 0; k < 3; k++) chi[s][k] = {chr[(s * 3 + k) * 2], chr[(s * 3 + k) * 2 + 1]};
+    if (OP == OP_WILSON) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc[s][k] = p.alpha * chi[s][k];
+    } else if (OP == OP_TM) {  // alpha (1 + isign i mu g5) chi (codegen/dslash_common.cc:1364-1489)
+      twist_ord<R>(chi, acc, p.alpha, (R)ISIGN * p.mu);
+    } else {
+      if (OP == OP_CLOVER)
+        clover_packed<S>(static_cast<const S *>(p.clov), vcb, idx, chi, acc);
+      else
+        clover_full<S>(static_cast<const S *>(p.clov), vcb, idx, chi, acc);
+      // + isign i rho g5 chi, one fused multiply-add per part
+      // (codegen/dslash_common.cc:1873-1906)
+      const R rho = (R)ISIGN * p.rho;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const R f = s < 2 ? rho : -rho;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+          acc[s][k] = {fmx(-f, chi[s][k].im, acc[s][k].re), fmx(f, chi[s][k].re, acc[s][k].im)};
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) acc[s][k] = {R(0), R(0)};
+  }
+  };
+  if constexpr (ACHI) {
+    bxb = -bxb;
+    bxf = -bxf;
+    byb = -byb;
+    byf = -byf;
+    bzb = -bzb;
+    bzf = -bzf;
+    btb = -btb;
+    btf = -btf;
+  }
 
   const int tm = p.tbc_mask;
   const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
@@ -648,6 +721,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     const int zb = c.z == 0 ? g.Lz - 1 : c.z - 1, zf = c.z == g.Lz - 1 ? 0 : c.z + 1;
     const int tb = c.t == 0 ? g.Lt - 1 : c.t - 1, tf = c.t == g.Lt - 1 ? 0 : c.t + 1;
     const int sh = p.q_sh;
+    start_acc();
     hop_mu<S, RECON, 0, ISIGN, 1>(acc, in, gauge, vcb, idx, qphix_site(p, xb >> 1, c.y, c.z, c.t),
                                   qphix_site(p, xf >> 1, c.y, c.z, c.t), bxb, bxf, tm & 1, ph0, p.one, sh);
     hop_mu<S, RECON, 1, ISIGN, 1>(acc, in, gauge, vcb, idx, qphix_site(p, c.xh, yb, c.z, c.t),
@@ -672,8 +746,9 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     r1 = (int)(fdiv(base + kDslashBlock - 1, g.fxh) - r0);
     y0 = (int)frem(r0, g.fy);
   }
-  if (p.lds && y0 + r1 < g.Ly) {
-    __shared__ Vec16 lds[(24 * sizeof(S) / 16) * kLdsSites];
+  const bool use_lds = p.lds && y0 + r1 < g.Ly;  // uniform per work-group
+  __shared__ Vec16 lds[(24 * sizeof(S) / 16) * kLdsSites];
+  if (use_lds) {
     const int plane = row - c.y * g.Lxh;
     const int nst = (r1 + 3) * g.Lxh;
     const Vec16 *vin = reinterpret_cast<const Vec16 *>(in);
@@ -688,6 +763,13 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       for (int ch = 0; ch < NC; ch++) lds[ch * kLdsSites + j] = vin[(unsigned)(ch * vcb + src)];
     }
     __syncthreads();
+  }
+#else
+  const bool use_lds = false;
+#endif
+  start_acc();
+#if QAMD_LDS_XY
+  if (use_lds) {
     const int rl = c.y - y0 + 1;  // staged row of this site
     hop_mu_lds<S, RECON, 0, ISIGN>(acc, lds, gauge, vcb, idx, rl * g.Lxh + (xb >> 1),
                                    rl * g.Lxh + (xf >> 1), bxb, bxf, tm & 1, ph0, p.one);
@@ -704,7 +786,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   }
 
   cx<R> res[4][3];
-  if (!ACHI) {
+  if constexpr (!ACHI) {
     if (OP == OP_WILSON) {
 #pragma unroll
       for (int s = 0; s < 4; s++)
@@ -715,44 +797,13 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
     } else if (OP == OP_TMCLOVER) {
       clover_full<S>(static_cast<const S *>(p.clov), vcb, idx, acc, res);
     } else {  // muinv (1 - isign i mu g5) (codegen/dslash_common.cc:1234-1359)
-      twist<R>(acc, res, p.muinv, -(R)ISIGN * p.mu * p.muinv);
+      twist_ord<R>(acc, res, p.muinv, -(R)ISIGN * p.mu);
     }
   } else {
-    R chr[24];
-    if constexpr (SPL)
-      load_q_reals<S, R, true>(static_cast<const S *>(p.chi), q0, p.q_sh, chr);
-    else
-      load_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
-    cx<R> chi[4][3];
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int k = 0; k < 3; k++) chi[s][k] = {chr[(s * 3 + k) * 2], chr[(s * 3 + k) * 2 + 1]};
-    if (OP == OP_WILSON) {
-#pragma unroll
-      for (int s = 0; s < 4; s++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) res[s][k] = p.alpha * chi[s][k];
-    } else if (OP == OP_TM) {  // alpha (1 + isign i mu g5) chi
-      twist<R>(chi, res, p.alpha, (R)ISIGN * p.mu * p.alpha);
-    } else {
-      if (OP == OP_CLOVER)
-        clover_packed<S>(static_cast<const S *>(p.clov), vcb, idx, chi, res);
-      else
-        clover_full<S>(static_cast<const S *>(p.clov), vcb, idx, chi, res);
-      // + isign i rho g5 chi (codegen/dslash_common.cc:1873-1906)
-      const R rho = (R)ISIGN * p.rho;
-#pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const R f = s < 2 ? rho : -rho;
-#pragma unroll
-        for (int k = 0; k < 3; k++) res[s][k] = res[s][k] + f * timesI(chi[s][k]);
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 4; s++)
-#pragma unroll
-      for (int k = 0; k < 3; k++) res[s][k] = res[s][k] - acc[s][k];
+      for (int k = 0; k < 3; k++) res[s][k] = acc[s][k];
   }
 
   R o[24];

@@ -272,7 +272,39 @@ def _layout_str(l):
             f"compress12={l.compress12}, pad_xy={l.pad_xy}, pad_xyz={l.pad_xyz})")
 
 
-def _check_args(res, psi, u, isign, cb):
+class QPhiXSpinor:
+    """A QPhiX FourSpinorBlock array used in place: a flat device tensor of
+    the layout's FT (float64 / float32 / float16) in the block order of the
+    layout (qphix_layout).  The operator methods below accept it wherever
+    they take a native spinor (res, psi and chi all QPhiX arrays of one
+    layout, the links and clover native fields of the same precision), and
+    then call the qphix_amd_qphix_* entry points, which read and write the
+    arrays where they are.  accumulate: optional uint8 device tensor, one
+    byte per site slot of the array (block * soalen + lane), bit d = 2 mu +
+    fb enabling direction d at that output site -- the reference's
+    accumulate[8] lane masks (codegen/dslash.cc:248,285-290); None enables
+    every direction."""
+
+    def __init__(self, data, layout, accumulate=None, cb=None):
+        if data.dtype != TORCH_DTYPE.get(layout.ft) or not data.is_cuda or not data.is_contiguous():
+            raise QPhiXAMDError("a QPhiX array is a contiguous device tensor of the layout's FT")
+        if accumulate is not None and (accumulate.dtype != torch.uint8 or not accumulate.is_cuda):
+            raise QPhiXAMDError("accumulate is a uint8 device tensor")
+        self.data, self.layout, self.accumulate, self.cb = data, layout, accumulate, cb
+
+    @property
+    def prec(self):
+        return self.layout.ft
+
+    def ptr(self):
+        return ctypes.c_void_p(self.data.data_ptr())
+
+
+def _same_layout(a, b):
+    return all(getattr(a, f) == getattr(b, f) for f, _ in QPhiXLayout._fields_)
+
+
+def _check_args(res, psi, u, isign, cb, geom=None, chi=None):
     if isign not in (1, -1) or cb not in (0, 1):
         raise QPhiXAMDError("isign must be +-1 and cb 0 or 1")
     if not (res.prec == psi.prec == u.prec):
@@ -281,7 +313,35 @@ def _check_args(res, psi, u, isign, cb):
         raise QPhiXAMDError("u must be a packed gauge field")
     if u.cb is not None and u.cb != cb:
         raise QPhiXAMDError(f"gauge packed for cb {u.cb}, operator applied on cb {cb}")
+    spinors = [res, psi] + ([chi] if chi is not None else [])
+    if isinstance(res, QPhiXSpinor):
+        if not all(isinstance(f, QPhiXSpinor) and _same_layout(f.layout, res.layout) for f in spinors):
+            raise QPhiXAMDError("res, psi and chi must be QPhiX arrays of one layout")
+        n = geom.qphix_field_bytes(res.layout, FIELD_SPINOR) // res.data.element_size()
+        if any(f.data.numel() < n for f in spinors):
+            raise QPhiXAMDError("QPhiX array smaller than the layout requires")
+        if res.accumulate is not None and res.accumulate.numel() < n // 24:
+            raise QPhiXAMDError("accumulate needs one byte per site slot")
+    elif any(isinstance(f, QPhiXSpinor) for f in spinors):
+        raise QPhiXAMDError("res, psi and chi must be all native or all QPhiX arrays")
     res.cb = cb
+
+
+def _run(geom, name, res, psi, chi, u, clov, isign, cb, scalars):
+    """One operator call: the native entry qphix_amd_<name>, or, for QPhiX
+    arrays, qphix_amd_qphix_<name> on them in place."""
+    _check_args(res, psi, u, isign, cb, geom, chi)
+    q = isinstance(res, QPhiXSpinor)
+    args = [geom.ctx, ctypes.byref(res.layout) if q else res.prec, u.recon, isign, cb, res.ptr(), psi.ptr()]
+    if chi is not None:
+        args.append(chi.ptr())
+    args.append(u.ptr())
+    if clov is not None:
+        args.append(clov.ptr())
+    if q:
+        args.append(_ptr(res.accumulate))
+    fn = ("qphix_amd_qphix_" if q else "qphix_amd_") + name
+    check(getattr(lib(), fn)(*args, *[float(v) for v in scalars], _stream()), fn)
 
 
 class Dslash:
@@ -293,40 +353,26 @@ class Dslash:
         self.aniso_t = float(aniso_coeff_t)
 
     def dslash(self, res, psi, u, isign, cb):
-        _check_args(res, psi, u, isign, cb)
-        check(lib().qphix_amd_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(), psi.ptr(),
-                                     u.ptr(), self.aniso_s, self.aniso_t, self.aniso_t, _stream()),
-              "qphix_amd_dslash")
+        _run(self.geom, "dslash", res, psi, None, u, None, isign, cb,
+             (self.aniso_s, self.aniso_t, self.aniso_t))
 
     def dslashAChiMinusBDPsi(self, res, psi, chi, u, alpha, beta, isign, cb):
-        _check_args(res, psi, u, isign, cb)
         b = float(beta)
-        check(lib().qphix_amd_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
-                                                psi.ptr(), chi.ptr(), u.ptr(), float(alpha),
-                                                b * self.aniso_s, b * self.aniso_t, b * self.aniso_t,
-                                                _stream()),
-              "qphix_amd_dslash_achimbdpsi")
+        _run(self.geom, "dslash_achimbdpsi", res, psi, chi, u, None, isign, cb,
+             (alpha, b * self.aniso_s, b * self.aniso_t, b * self.aniso_t))
 
 
 class ClovDslash(Dslash):
     """Wilson-clover (QPhiX ClovDslash): A^-1 applied after the hop."""
 
     def dslash(self, res, psi, u, invclov, isign, cb):
-        _check_args(res, psi, u, isign, cb)
-        check(lib().qphix_amd_clov_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
-                                          psi.ptr(), u.ptr(), invclov.ptr(), self.aniso_s,
-                                          self.aniso_t, self.aniso_t, _stream()),
-              "qphix_amd_clov_dslash")
+        _run(self.geom, "clov_dslash", res, psi, None, u, invclov, isign, cb,
+             (self.aniso_s, self.aniso_t, self.aniso_t))
 
     def dslashAChiMinusBDPsi(self, res, psi, chi, u, clov, beta, isign, cb, prec_mass_rho=0.0):
-        _check_args(res, psi, u, isign, cb)
         b = float(beta)
-        check(lib().qphix_amd_clov_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb,
-                                                     res.ptr(), psi.ptr(), chi.ptr(), u.ptr(),
-                                                     clov.ptr(), b * self.aniso_s, b * self.aniso_t,
-                                                     b * self.aniso_t, float(prec_mass_rho),
-                                                     _stream()),
-              "qphix_amd_clov_dslash_achimbdpsi")
+        _run(self.geom, "clov_dslash_achimbdpsi", res, psi, chi, u, clov, isign, cb,
+             (b * self.aniso_s, b * self.aniso_t, b * self.aniso_t, prec_mass_rho))
 
 
 class TMDslash(Dslash):
@@ -342,39 +388,23 @@ class TMDslash(Dslash):
         self.muinv = self.alpha / (self.alpha ** 2 + self.mu_phys ** 2)
 
     def dslash(self, res, psi, u, isign, cb):
-        _check_args(res, psi, u, isign, cb)
-        check(lib().qphix_amd_tm_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
-                                        psi.ptr(), u.ptr(), self.aniso_s, self.aniso_t, self.aniso_t,
-                                        self.mu, self.muinv, _stream()),
-              "qphix_amd_tm_dslash")
+        _run(self.geom, "tm_dslash", res, psi, None, u, None, isign, cb,
+             (self.aniso_s, self.aniso_t, self.aniso_t, self.mu, self.muinv))
 
     def dslashAChiMinusBDPsi(self, res, psi, chi, u, alpha, beta, isign, cb):
-        _check_args(res, psi, u, isign, cb)
         b = float(beta)
-        check(lib().qphix_amd_tm_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb,
-                                                   res.ptr(), psi.ptr(), chi.ptr(), u.ptr(),
-                                                   float(alpha), b * self.aniso_s, b * self.aniso_t,
-                                                   b * self.aniso_t, self.mu_phys / float(alpha),
-                                                   _stream()),
-              "qphix_amd_tm_dslash_achimbdpsi")
+        _run(self.geom, "tm_dslash_achimbdpsi", res, psi, chi, u, None, isign, cb,
+             (alpha, b * self.aniso_s, b * self.aniso_t, b * self.aniso_t, self.mu_phys / float(alpha)))
 
 
 class TMClovDslash(Dslash):
     """Twisted-mass clover (QPhiX TMClovDslash): full complex 6x6 blocks."""
 
     def dslash(self, res, psi, u, invclov, isign, cb):
-        _check_args(res, psi, u, isign, cb)
-        check(lib().qphix_amd_tm_clov_dslash(self.geom.ctx, res.prec, u.recon, isign, cb, res.ptr(),
-                                             psi.ptr(), u.ptr(), invclov.ptr(), self.aniso_s,
-                                             self.aniso_t, self.aniso_t, _stream()),
-              "qphix_amd_tm_clov_dslash")
+        _run(self.geom, "tm_clov_dslash", res, psi, None, u, invclov, isign, cb,
+             (self.aniso_s, self.aniso_t, self.aniso_t))
 
     def dslashAChiMinusBDPsi(self, res, psi, chi, u, clov, beta, isign, cb, prec_mass_rho=0.0):
-        _check_args(res, psi, u, isign, cb)
         b = float(beta)
-        check(lib().qphix_amd_tm_clov_dslash_achimbdpsi(self.geom.ctx, res.prec, u.recon, isign, cb,
-                                                        res.ptr(), psi.ptr(), chi.ptr(), u.ptr(),
-                                                        clov.ptr(), b * self.aniso_s,
-                                                        b * self.aniso_t, b * self.aniso_t,
-                                                        float(prec_mass_rho), _stream()),
-              "qphix_amd_tm_clov_dslash_achimbdpsi")
+        _run(self.geom, "tm_clov_dslash_achimbdpsi", res, psi, chi, u, clov, isign, cb,
+             (b * self.aniso_s, b * self.aniso_t, b * self.aniso_t, prec_mass_rho))

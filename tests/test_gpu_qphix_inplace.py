@@ -10,9 +10,8 @@ FourSpinorBlock array.
   non-trivial accumulate masks (x lane by lane, y too when soalen < veclen,
   z and t per vector, the patterns its bodies express): every family x
   {dslash, achimbdpsi} x isign x 12/18-real links, soalen == veclen and the
-  xy-packed soalen = veclen / 2, FP64 (1e-13 relative) and FP32 (1e-6);
-  FP16 storage against the reference's low-precision build (within one
-  FP16 unit in the last place, and >= 99% of the elements bit-identical);
+  xy-packed soalen = veclen / 2, FP64 and FP32, and FP16 storage against
+  the reference's low-precision build: the same bits in every element;
 * without a mask, bit-identical to the native-layout kernels on the same
   data (the in-place path changes addressing only).
 """
@@ -23,7 +22,7 @@ import pytest
 import torch
 
 import qphix_blocks as qb
-from _util import accumulate_mask, f16_ulps, f16_within
+from _util import accumulate_mask, f16_ulps
 
 pytestmark = pytest.mark.gpu
 
@@ -147,14 +146,12 @@ def test_inplace_equals_reference_kernels_with_masks(qamd, fields, config):
                     want_lex, _ = ref.apply(config, DIMS, psi, U, isign, cb, recon=recon, mask=mask, **kw)
                     want = ref.pack(config, 0, DIMS, want_lex, cb)
                     if ft == 2:
-                        assert f16_within(got, want).all(), (kind, achi, isign, recon)
                         same = float(np.mean(f16_ulps(got, want) == 0))
-                        assert same >= 0.99, (kind, achi, isign, recon, same)
                         worst = max(worst, 1.0 - same)
                     else:
                         err = float(np.abs(got.astype(np.float64) - want).max() / np.abs(want).max())
-                        assert err < (1e-13 if ft == 0 else 1e-6), (kind, achi, isign, recon, err)
                         worst = max(worst, err)
+                    assert np.array_equal(got, want), (kind, achi, isign, recon, worst)
     print(f"{config}: worst {'fraction of differing FP16 elements' if ft == 2 else 'relative error'} "
           f"{worst:.3e}", flush=True)
 
@@ -218,3 +215,49 @@ def test_inplace_argument_checks(qamd):
                                     st) == -6
     torch.cuda.synchronize()
     assert torch.count_nonzero(buf) == 0
+
+
+def test_python_classes_take_qphix_arrays(qamd, fields):
+    """The QPhiX-shaped classes (Dslash, ClovDslash, TMDslash, TMClovDslash)
+    take QPhiXSpinor arrays where they take native spinors and give the
+    native results' bits, with a mask exactly the C ABI's in-place result."""
+    L = qamd.lattice
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    U = torch.from_numpy(fields["U"]).to(DEV)
+    psi = torch.from_numpy(fields["psi"]).to(DEV)
+    chi = torch.from_numpy(fields["chi"]).to(DEV)
+    clt = fields["clt"]
+    lay = qamd.qphix_layout(1, 16, 8, True)
+    cb = 1
+    g = geom.pack_gauge(U, 1, 12, cb)
+    inv = geom.pack_clover(L.invert_clover(clt).to(DEV), 1, cb)
+    clov = geom.pack_clover(clt.to(DEV), 1, cb)
+    psi_n, chi_n = geom.pack_spinor(psi, 1, 1 - cb), geom.pack_spinor(chi, 1, cb)
+    psi_q = qamd.QPhiXSpinor(geom.spinor_to_qphix(psi_n, lay), lay)
+    chi_q = qamd.QPhiXSpinor(geom.spinor_to_qphix(chi_n, lay), lay)
+    ops = [(qamd.Dslash(geom, 0.875, 1.25), lambda op, r, p, c: op.dslash(r, p, g, -1, cb)),
+           (qamd.ClovDslash(geom), lambda op, r, p, c: op.dslashAChiMinusBDPsi(r, p, c, g, clov, 0.5, -1, cb, 0.125)),
+           (qamd.ClovDslash(geom), lambda op, r, p, c: op.dslash(r, p, g, inv, 1, cb)),
+           (qamd.TMDslash(geom, -0.25, 0.1), lambda op, r, p, c: op.dslashAChiMinusBDPsi(r, p, c, g, 3.75, 0.25, 1, cb))]
+    for op, run in ops:
+        out_n = geom.new_spinor(1, cb=cb)
+        run(op, out_n, psi_n, chi_n)
+        out_q = qamd.QPhiXSpinor(torch.zeros_like(psi_q.data), lay)
+        run(op, out_q, psi_q, chi_q)
+        assert torch.equal(out_q.data, geom.spinor_to_qphix(out_n, lay))
+    # with a mask: the class call equals the direct C ABI call
+    lay_q = qb.Layout(1, 16, 8)
+    mask = accumulate_mask(DIMS, cb, 8, 16, 5)
+    acc = torch.from_numpy(qphix_mask(lay_q, DIMS, cb, mask)).to(DEV)
+    out_q = qamd.QPhiXSpinor(torch.zeros_like(psi_q.data), lay, accumulate=acc)
+    qamd.ClovDslash(geom, 0.875, 1.25).dslash(out_q, psi_q, g, inv, 1, cb)
+    direct = torch.zeros_like(psi_q.data)
+    assert qamd.lib().qphix_amd_qphix_clov_dslash(geom.ctx, ctypes.byref(lay), 12, 1, cb, _p(direct), _p(psi_q.data),
+                                                   g.ptr(), inv.ptr(), _p(acc), 0.875, 1.25, 1.25, _stream()) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(out_q.data, direct)
+    full = torch.zeros_like(psi_q.data)
+    qamd.ClovDslash(geom, 0.875, 1.25).dslash(qamd.QPhiXSpinor(full, lay), psi_q, g, inv, 1, cb)
+    assert not torch.equal(full, direct)  # the mask drops hops
+    with pytest.raises(qamd.QPhiXAMDError):
+        qamd.Dslash(geom).dslash(out_q, psi_n, g, -1, cb)  # mixed native / QPhiX spinors
