@@ -11,7 +11,8 @@
 #      configure_file would (its only variable is @CMAKE_BINARY_DIR@);
 #   3. the generator is run; it writes the kernel bodies;
 #   4. oracle/ref_driver.cpp (our QPhiX-style block driver) #includes the
-#      bodies it needs inside functions with the parameter names of
+#      bodies it needs (the operator bodies and the face pack / finish
+#      bodies) inside functions with the parameter names of
 #      codegen/jinja/*_decl.h.j2, and is compiled into a shared library.
 #
 # Configurations (ISA, precision, VECLEN = SOALEN, codegen/jinja/isa.js):
@@ -108,7 +109,10 @@ $(OUT)/%/generated.stamp: $(OUT)/%/codegen
 $(OUT)/%/bodies.inc: $(OUT)/%/generated.stamp ref_wrappers.py
 	python3 ref_wrappers.py $@ $(FT_$*) $(VL_$*) $(call soa,$*)
 
-$(OUT)/libref_%.so: $(OUT)/%/bodies.inc ref_driver.cpp
+$(OUT)/%/faces.inc: $(OUT)/%/generated.stamp ref_wrappers.py
+	python3 ref_wrappers.py --faces $@ $(FT_$*) $(VL_$*) $(call soa,$*)
+
+$(OUT)/libref_%.so: $(OUT)/%/bodies.inc $(OUT)/%/faces.inc ref_driver.cpp
 	$(CXX) -std=c++14 -O3 -fopenmp -fPIC -shared -w $(MARCH_$*) $(DEFS_$*) \
 	  -DREF_CONFIG_NAME='"$*"' -I$(OUT)/$* -I$(OUT)/$*/generated/$(ARCH_$*) -I$(REF)/include \
 	  ref_driver.cpp -o $@

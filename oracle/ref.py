@@ -38,6 +38,10 @@ def lib(config):
         l.ref_pack_field.argtypes = [i, i, i, i, i, i, i, _dp, ctypes.c_void_p]
         l.ref_pack_field.restype = ctypes.c_long
         l.ref_soalen.restype = i
+        l.ref_face_pack.argtypes = [i, i, i, i, i, i, i, _dp, _dp]
+        l.ref_face_pack.restype = i
+        l.ref_face_finish.argtypes = [i, i, i, i, i, i, i, i, i, _dp, _dp, _dp, _dp, d, d, d, _dp]
+        l.ref_face_finish.restype = i
         l.ref_stream_triad.argtypes = [ctypes.c_long, i, i]
         l.ref_stream_triad.restype = d
         _libs[config] = l
@@ -115,3 +119,40 @@ def pack(config, field, dims, src, cb, recon=18):
     out = np.empty(n, dtype=dt)
     l.ref_pack_field(field, recon, cb, *dims, _p(src), out.ctypes.data_as(ctypes.c_void_p))
     return out
+
+
+def _phases(tbc_phases):
+    if tbc_phases is None:
+        return None
+    return np.array([v for z in tbc_phases for v in (complex(z).real, complex(z).imag)], dtype=np.float64)
+
+
+def face_pack(config, dims, psi, isign, dir, pcb):
+    """The reference's face pack body (face_proj_dir_{plus,minus}) over face
+    dir = 2 mu + fb of checkerboard pcb: returns hs (V, 12), the projected
+    half-spinor ([spin 0|1][colour][re|im]) at each face site, zero
+    elsewhere."""
+    V = int(np.prod(dims))
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    hs = np.zeros((V, 12))
+    rc = lib(config).ref_face_pack(isign, dir, pcb, *dims, _p(psi), _p(hs))
+    if rc != 0:
+        raise ValueError(f"reference face pack rejected the call (config {config}, dims {dims})")
+    return hs
+
+
+def face_finish(config, dims, kind, isign, recon, dir, cb, hs, U, out, beta, clov=None, mu=0.0, muinv=0.0,
+                tbc_phases=None):
+    """The reference's face finish body of family kind (0 face_finish_dir,
+    1 face_clov_finish_dir, 2 tm_face_finish_dir, 3 tm_clov_face_finish_dir)
+    on face dir of checkerboard cb: the face sites receive hs (V, 12, read
+    at those sites) and update a copy of out (V, 4, 3, 2), which is
+    returned."""
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    hs, U, clov = c(hs), c(U), c(clov)
+    res = np.array(out, dtype=np.float64, copy=True, order="C")
+    rc = lib(config).ref_face_finish(kind, isign, recon, dir, cb, *dims, _p(hs), _p(U), _p(clov), _p(res),
+                                     beta, mu, muinv, _p(_phases(tbc_phases)))
+    if rc != 0:
+        raise ValueError(f"reference face finish rejected the call (config {config}, dims {dims})")
+    return res

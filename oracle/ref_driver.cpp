@@ -102,9 +102,39 @@ typedef void (*ref_body_fn)(const RefArgs &);
   const FT prec_mass_rho = A.prec_mass_rho;                                          \
   const FT(*tbc_phases)[2] = A.tbc_phases;
 
+// The face bodies' parameters (codegen/jinja/*_dslash_decl.h.j2:90-115).
+struct FaceArgs {
+  const void *xyBase, *gBase, *clBase;
+  void *oBase;
+  const FT *inbuf;
+  FT *outbuf;
+  int offs[VECLEN], gOffs[VECLEN];
+  unsigned int mask;
+  int dir;
+  FT beta, mu, muinv;
+  FT tbc_phases[4][2];
+};
+
+typedef void (*ref_face_fn)(const FaceArgs &);
+
+#define REF_BIND_FACE                                                                \
+  const Ty::FourSpinorBlock *xyBase = (const Ty::FourSpinorBlock *)F.xyBase;         \
+  Ty::FourSpinorBlock *oBase = (Ty::FourSpinorBlock *)F.oBase;                       \
+  const Ty::SU3MatrixBlock *gBase = (const Ty::SU3MatrixBlock *)F.gBase;             \
+  const FT *inbuf = F.inbuf;                                                         \
+  FT *outbuf = F.outbuf;                                                             \
+  const int *offs = F.offs, *gOffs = F.gOffs;                                        \
+  const int si_prefdist = 0, hsprefdist = 0, gprefdist = 0, soprefdist = 0;          \
+  const int clprefdist = 0;                                                          \
+  const unsigned int mask = F.mask;                                                  \
+  const int dir = F.dir;                                                             \
+  const FT beta = F.beta, mu = F.mu, muinv = F.muinv;                                \
+  const FT(*tbc_phases)[2] = F.tbc_phases;
+
 using namespace QPhiX;
 
 #include "bodies.inc"
+#include "faces.inc"
 
 namespace {
 
@@ -465,6 +495,197 @@ int ref_apply(int kind, int achi, int isign, int recon, int cb, int Lx, int Ly, 
   free(gb);
   free(cl);
   return 0;
+}
+
+// ---- faces ----------------------------------------------------------------
+// The reference's face bodies driven the way the reference's face
+// declarations imply (codegen/jinja/dslash_decl.h.j2:90-115): one call per
+// gauge vector of the boundary slice, the vector's spinor blocks addressed
+// through xyBase / oBase + offs[row * SOALEN] as in the body calls, and a
+// half-spinor buffer per call of nActive lanes per real, [spin 0|1][colour]
+// [re|im][nActive] (codegen/data_types.cc:508-543, 587-626):
+//   X: the boundary lane of each row, lane 0 (back face, x = 0) or lane
+//      SOALEN - 1 (forward face, x = Lx - 1), on the rows whose parity puts
+//      a site there, (VECLEN / SOALEN + 1) / 2 of them, compress-stored /
+//      expand-loaded under the runtime lane mask;
+//   Y: the vector's first row (back, y = 0) or last row (forward,
+//      y = Ly - 1), SOALEN lanes (all VECLEN when SOALEN = VECLEN);
+//   Z, T: the whole vector.
+// The X face loads use the x-direction split loads of the body (lane 0
+// from offs[row * SOALEN], lanes 1.. from offs[row * SOALEN + 1] for the
+// back face; lanes ..S-2 from offs[row * SOALEN], lane S-1 from
+// offs[row * SOALEN + S - 1] for the forward face), pointed here at the
+// row's own sites.  Host arrays are lexicographic FP64; a half-spinor array
+// hs holds 12 reals ([spin 0|1][colour][re|im]) per lexicographic site and
+// is read / written only at the face sites.
+
+namespace {
+
+struct FaceLanes {
+  int lane[VECLEN];
+  long site[VECLEN];
+  int n;
+  unsigned int mask;
+};
+
+// active lanes of gauge vector (t, z, yg, xb) for face dir on checkerboard
+// cb, in lane order (the compress / expand order); n = 0 off the slice
+FaceLanes face_lanes(const Lat &g, int dir, int cb, int t, int z, int yg, int xb)
+{
+  FaceLanes f;
+  f.n = 0;
+  f.mask = 0;
+  const int dim = dir / 2, fw = dir % 2;
+  const int Lyg = g.L[1] / NGY;
+  const bool on = dim == 0 ? xb == (fw ? g.nxb - 1 : 0)
+                  : dim == 1 ? yg == (fw ? Lyg - 1 : 0)
+                  : dim == 2 ? z == (fw ? g.L[2] - 1 : 0)
+                             : t == (fw ? g.L[3] - 1 : 0);
+  if (!on) return f;
+  for (int row = 0; row < NGY; row++) {
+    const int y = yg * NGY + row;
+    if (dim == 1 && NGY > 1 && row != (fw ? NGY - 1 : 0)) continue;
+    for (int i = 0; i < S; i++) {
+      const int x = g.xof(xb * S + i, y, z, t, cb);
+      if (dim == 0 && x != (fw ? g.L[0] - 1 : 0)) continue;
+      f.lane[f.n] = row * S + i;
+      f.site[f.n] = g.lex(x, y, z, t);
+      f.mask |= 1u << (row * S + i);
+      f.n++;
+    }
+  }
+  return f;
+}
+
+int face_active(int dim)
+{
+  return dim == 0 ? (NGY + 1) / 2 : dim == 1 ? S : VECLEN;
+}
+
+// offs of the vector whose first row segment is block b0, per row
+void face_offs(const Lat &g, int dir, int t, int z, int yg, int xb, long b0, int *offs)
+{
+  for (int row = 0; row < NGY; row++) {
+    const int L0 = row * S;
+    const int ro = (int)((g.block(t, z, yg * NGY + row, xb) - b0) * BLK);
+    offs[L0] = ro;
+    if (dir == 0 && S > 1) offs[L0 + 1] = ro + 1;
+    if (dir == 1 && S > 1) offs[L0 + S - 1] = ro + S - 1;
+  }
+}
+
+}  // namespace
+
+// Face pack through face_proj_dir_{plus,minus}: the projected half-spinors
+// of the sites of checkerboard pcb on face dir = 2 mu + fb (fb 0: x_mu = 0,
+// sent to the back neighbour; 1: x_mu = L_mu - 1, sent forward) into hs.
+// Returns 0, or -1 for an unsupported geometry / argument.
+int ref_face_pack(int isign, int dir, int pcb, int Lx, int Ly, int Lz, int Lt, const double *psi,
+                  double *hs)
+{
+  Lat g;
+  if (!make_lat(g, Lx, Ly, Lz, Lt) || dir < 0 || dir > 7 || (pcb != 0 && pcb != 1) || S < 2) return -1;
+  const long nblk = g.nblk();
+  FT *in = amalloc<FT>(nblk * BLK);
+  FT *buf = amalloc<FT>(12 * VECLEN);
+  pack_spinor(in, psi, g, pcb);
+  const ref_face_fn fn = ref_face_pack_table[isign > 0 ? 0 : 1];
+  const int na = face_active(dir / 2), Lyg = Ly / NGY;
+  int rc = 0;
+  for (int t = 0; t < Lt && !rc; t++)
+    for (int z = 0; z < Lz; z++)
+      for (int yg = 0; yg < Lyg; yg++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          const FaceLanes fl = face_lanes(g, dir, pcb, t, z, yg, xb);
+          if (!fl.n) continue;
+          if (fl.n != na) rc = -1;
+          if (rc) break;
+          const long b0 = g.block(t, z, yg * NGY, xb);
+          FaceArgs F;
+          memset(&F, 0, sizeof(F));
+          face_offs(g, dir, t, z, yg, xb, b0, F.offs);
+          F.xyBase = in + b0 * BLK;
+          F.outbuf = buf;
+          F.mask = fl.mask;
+          F.dir = dir;
+          memset(buf, 0, 12 * VECLEN * sizeof(FT));
+          fn(F);
+          for (int k = 0; k < na; k++)
+            for (int c = 0; c < 12; c++) hs[fl.site[k] * 12 + c] = from_ft(buf[c * na + k]);
+        }
+  free(in);
+  free(buf);
+  return rc;
+}
+
+// Face finish through the family's face finish body (kind 0 face_finish_dir,
+// 1 face_clov_finish_dir with clov the packed clover, 2 tm_face_finish_dir,
+// 3 tm_clov_face_finish_dir with clov the full clover blocks): the sites of
+// checkerboard cb on face dir receive the half-spinors hs (read at those
+// sites) through the direction-dir link; out (lexicographic, cb sites) is
+// read and written in place.  tbc_phases (8 doubles) selects the twisted-
+// boundary bodies.  Returns 0, or -1 for an unsupported geometry / argument.
+int ref_face_finish(int kind, int isign, int recon, int dir, int cb, int Lx, int Ly, int Lz, int Lt,
+                    const double *hs, const double *U, const double *clov, double *out, double beta,
+                    double mu, double muinv, const double *tbc_phases)
+{
+  Lat g;
+  if (!make_lat(g, Lx, Ly, Lz, Lt) || kind < 0 || kind > 3 || dir < 0 || dir > 7 ||
+      (recon != 12 && recon != 18) || (cb != 0 && cb != 1) || S < 2)
+    return -1;
+  const long nblk = g.nblk(), ngblk = nblk / NGY;
+  const int nrow = recon == 12 ? 2 : 3;
+  const long gblk = 8L * nrow * 3 * 2 * VECLEN;
+  const long cblk = (kind == 3 ? 144L : 72L) * VECLEN;
+  FT *o = amalloc<FT>(nblk * BLK);
+  FT *gb = amalloc<FT>(ngblk * gblk);
+  FT *cl = (kind == 1 || kind == 3) ? amalloc<FT>(ngblk * cblk) : nullptr;
+  FT *buf = amalloc<FT>(12 * VECLEN);
+  pack_spinor(o, out, g, cb);
+  pack_gauge(gb, U, g, cb, nrow);
+  if (kind == 1) pack_clover(cl, clov, g, cb);
+  if (kind == 3) pack_full_clover(cl, clov, g, cb);
+  const ref_face_fn fn = (tbc_phases ? ref_face_table_tbc : ref_face_table)[kind][isign > 0 ? 0 : 1]
+                                                                           [recon == 12 ? 0 : 1];
+  const int na = face_active(dir / 2), Lyg = Ly / NGY;
+  int rc = 0;
+  for (int t = 0; t < Lt && !rc; t++)
+    for (int z = 0; z < Lz; z++)
+      for (int yg = 0; yg < Lyg; yg++)
+        for (int xb = 0; xb < g.nxb; xb++) {
+          const FaceLanes fl = face_lanes(g, dir, cb, t, z, yg, xb);
+          if (!fl.n) continue;
+          if (fl.n != na) rc = -1;
+          if (rc) break;
+          const long b0 = g.block(t, z, yg * NGY, xb);
+          const long gi = g.gblock(t, z, yg * NGY, xb);
+          FaceArgs F;
+          memset(&F, 0, sizeof(F));
+          face_offs(g, -1, t, z, yg, xb, b0, F.offs);
+          for (int k = 0; k < na; k++)
+            for (int c = 0; c < 12; c++) buf[c * na + k] = to_ft(hs[fl.site[k] * 12 + c]);
+          F.oBase = o + b0 * BLK;
+          F.gBase = gb + gi * gblk;
+          F.clBase = cl ? cl + gi * cblk : nullptr;
+          F.inbuf = buf;
+          F.mask = fl.mask;
+          F.dir = dir;
+          F.beta = to_ft(beta);
+          F.mu = to_ft(mu);
+          F.muinv = to_ft(muinv);
+          if (tbc_phases)
+            for (int m = 0; m < 4; m++) {
+              F.tbc_phases[m][0] = to_ft(tbc_phases[2 * m]);
+              F.tbc_phases[m][1] = to_ft(tbc_phases[2 * m + 1]);
+            }
+          fn(F);
+        }
+  if (!rc) unpack_spinor(out, o, g, cb);
+  free(o);
+  free(gb);
+  free(cl);
+  free(buf);
+  return rc;
 }
 
 }  // extern "C"

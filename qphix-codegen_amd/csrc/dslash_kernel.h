@@ -449,8 +449,9 @@ __device__ __forceinline__ void hop_mu_lds(cx<R> acc[4][3], const Vec16 *lds, co
 }
 
 // out = A in, A = two packed Hermitian 6x6 blocks
-// (codegen/dslash_common.cc:1098-1186).
-template <typename S, typename R>
+// (codegen/dslash_common.cc:1098-1186); ACC: out += A in, each row's chain
+// starting from out (the face form, codegen/dslash_common.cc:1125-1131).
+template <typename S, bool ACC = false, typename R>
 __device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
                                               const cx<R> in[4][3], cx<R> out[4][3])
 {
@@ -464,7 +465,13 @@ __device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
     for (int k = 0; k < 6; k++) v[k] = in[2 * b + k / 3][k % 3];
 #pragma unroll
     for (int r = 0; r < 6; r++) {
-      cx<R> res = blk[r] * v[r];
+      cx<R> res;
+      if constexpr (ACC) {
+        const cx<R> o = out[2 * b + r / 3][r % 3];
+        res = {fmx(blk[r], v[r].re, o.re), fmx(blk[r], v[r].im, o.im)};
+      } else {
+        res = blk[r] * v[r];
+      }
 #pragma unroll
       for (int c = 0; c < 6; c++) {
         if (c < r) {
@@ -483,8 +490,9 @@ __device__ __forceinline__ void clover_packed(const S *clov, int vcb, int site,
 }
 
 // out = B in, B = two full complex 6x6 blocks, row-major
-// (codegen/dslash_common.cc:1188-1232).
-template <typename S, typename R>
+// (codegen/dslash_common.cc:1188-1232); ACC: out += B in, each row's chain
+// starting from out (the face form).
+template <typename S, bool ACC = false, typename R>
 __device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
                                             const cx<R> in[4][3], cx<R> out[4][3])
 {
@@ -502,7 +510,11 @@ __device__ __forceinline__ void clover_full(const S *clov, int vcb, int site,
 #pragma unroll
     for (int r = 0; r < 6; r++) {
       cx<R> a0 = {blk[(r * 6) * 2], blk[(r * 6) * 2 + 1]};
-      cx<R> res = cmul_ord(a0, v[0]);
+      cx<R> res;
+      if constexpr (ACC)
+        res = cmac(out[2 * b + r / 3][r % 3], a0, v[0]);
+      else
+        res = cmul_ord(a0, v[0]);
 #pragma unroll
       for (int c = 1; c < 6; c++) {
         cx<R> a = {blk[(r * 6 + c) * 2], blk[(r * 6 + c) * 2 + 1]};
@@ -528,21 +540,6 @@ __device__ __forceinline__ void twist_ord(const cx<R> in[4][3], cx<R> out[4][3],
     for (int c = 0; c < 3; c++) {
       const cx<R> v = in[s][c];
       out[s][c] = {a * fma(-ms, v.im, v.re), a * fma(ms, v.re, v.im)};
-    }
-  }
-}
-
-// (a + i*f*g5) applied to in, f real: g5 = +1 on spins 0,1, -1 on 2,3.
-template <typename R>
-__device__ __forceinline__ void twist(const cx<R> in[4][3], cx<R> out[4][3], R a, R f)
-{
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const R fs = s < 2 ? f : -f;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      cx<R> v = in[s][c];
-      out[s][c] = {a * v.re - fs * v.im, a * v.im + fs * v.re};
     }
   }
 }

@@ -92,18 +92,17 @@ __device__ __forceinline__ void project_rt(int mu, int sgn, const R *psi, cx<R> 
 }
 
 template <typename R>
-__device__ __forceinline__ void reconstruct_rt(int mu, int sgn, cx<R> acc[4][3], const cx<R> w[2][3])
+__device__ __forceinline__ void reconstruct_rt(int mu, int sgn, cx<R> acc[4][3], const cx<R> w[2][3], R b)
 {
-  const R one = R(1);
   switch (mu * 2 + (sgn > 0)) {
-    case 0: reconstruct<0, -1>(acc, w, one); break;
-    case 1: reconstruct<0, 1>(acc, w, one); break;
-    case 2: reconstruct<1, -1>(acc, w, one); break;
-    case 3: reconstruct<1, 1>(acc, w, one); break;
-    case 4: reconstruct<2, -1>(acc, w, one); break;
-    case 5: reconstruct<2, 1>(acc, w, one); break;
-    case 6: reconstruct<3, -1>(acc, w, one); break;
-    default: reconstruct<3, 1>(acc, w, one); break;
+    case 0: reconstruct<0, -1>(acc, w, b); break;
+    case 1: reconstruct<0, 1>(acc, w, b); break;
+    case 2: reconstruct<1, -1>(acc, w, b); break;
+    case 3: reconstruct<1, 1>(acc, w, b); break;
+    case 4: reconstruct<2, -1>(acc, w, b); break;
+    case 5: reconstruct<2, 1>(acc, w, b); break;
+    case 6: reconstruct<3, -1>(acc, w, b); break;
+    default: reconstruct<3, 1>(acc, w, b); break;
   }
 }
 
@@ -133,13 +132,21 @@ __global__ __launch_bounds__(kFaceBlock) void face_pack_k(S *buf, const S *in, G
 
 template <typename R> struct FaceArgs {
   int cb, mu, side, isign, nface, tbc;
-  R beta, ph_re, ph_im, tm_mu, tm_muinv;
+  R beta, ph_re, ph_im, tm_mu, tm_muinv, one;
 };
 
-// out(site f) += E [beta (U h)(f) reconstructed], E the family's epilogue
-// (codegen/dslash.cc:463-553): -1 for Wilson, -muinv (1 - isign i mu g5) for
-// twisted mass (the reference's mbeta recons), A^-1 for clover and B for
-// twisted-mass clover (pbeta recons into a zeroed dout, then dout applied).
+// out(site f) += the face's hop, in the reference's face finish order
+// (codegen/dslash.cc:463-553, codegen/dslash_common.cc:1098-1186,1188-1232,
+// 1234-1300):
+//   w = U h (the hop's cdot3 chains; U_mu(x - mu)^dagger from the back
+//   neighbour, U_mu(x) from the front), times the twisted-boundary phase
+//   (conjugated from the back), then
+//   Wilson:  out -= beta R(w), one fused multiply-add per real into out;
+//   clover:  d = beta R(w), out += A d, each row's chain starting from out;
+//   TM:      d = -beta R(w), out += muinv (1 - isign i mu g5) d, each real
+//            one fused multiply-add of muinv into out;
+//   TM clover: d = beta R(w), out += B d, each row's chain starting from out;
+// R the reconstruction of the half-spinor to the full spinor.
 template <typename S, int RECON, int OP>
 __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf, const S *gauge,
                                                             const S *clov, Geom g,
@@ -158,54 +165,70 @@ __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf
   for (int s = 0; s < 2; s++)
 #pragma unroll
     for (int k = 0; k < 3; k++) h[s][k] = {hr[(s * 3 + k) * 2], hr[(s * 3 + k) * 2 + 1]};
+  const R nz = -R(0) * a.one;  // opaque -0.0, as in the operator kernels
   cx<R> ub[3][3], uf[3][3];
   switch (a.mu) {
-    case 0: load_link_pair<S, RECON, 0>(gauge, vcb, idx, ub, uf); break;
-    case 1: load_link_pair<S, RECON, 1>(gauge, vcb, idx, ub, uf); break;
-    case 2: load_link_pair<S, RECON, 2>(gauge, vcb, idx, ub, uf); break;
-    default: load_link_pair<S, RECON, 3>(gauge, vcb, idx, ub, uf); break;
+    case 0: load_link_pair<S, RECON, 0>(gauge, vcb, idx, ub, uf, nz); break;
+    case 1: load_link_pair<S, RECON, 1>(gauge, vcb, idx, ub, uf, nz); break;
+    case 2: load_link_pair<S, RECON, 2>(gauge, vcb, idx, ub, uf, nz); break;
+    default: load_link_pair<S, RECON, 3>(gauge, vcb, idx, ub, uf, nz); break;
   }
-  // from the back neighbour: U_mu(x - mu)^dagger, conj phase; from the front: U_mu(x)
-  cx<R> fac = {a.beta, R(0)};
-  if (a.tbc) fac = {a.beta * a.ph_re, a.side ? a.beta * a.ph_im : -a.beta * a.ph_im};
+  const cx<R> ph = {a.ph_re, a.side ? a.ph_im : -a.ph_im};
   cx<R> w[2][3];
 #pragma unroll
   for (int k = 0; k < 2; k++)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      const cx<R> ui0 = a.side ? uf[i][0] : ub[i][0];
-      const cx<R> ui1 = a.side ? uf[i][1] : ub[i][1];
-      const cx<R> ui2 = a.side ? uf[i][2] : ub[i][2];
-      w[k][i] = fac * (ui0 * h[k][0] + ui1 * h[k][1] + ui2 * h[k][2]);
+      const cx<R> u0 = a.side ? uf[i][0] : ub[i][0];
+      const cx<R> u1 = a.side ? uf[i][1] : ub[i][1];
+      const cx<R> u2 = a.side ? uf[i][2] : ub[i][2];
+      if constexpr (std::is_same<S, __half>::value)
+        w[k][i] = cdot3(u0, u1, u2, h[k][0], h[k][1], h[k][2], nz);
+      else
+        w[k][i] = cdot3(u0, u1, u2, h[k][0], h[k][1], h[k][2]);
+      if (a.tbc) w[k][i] = cmul_ord(ph, w[k][i]);
     }
-  cx<R> d[4][3];
+  const int sgn = a.side ? -a.isign : a.isign;
+  R o[24];
+  load_reals<S, spinor_w<S>(), 0, 24>(out, vcb, idx, o);
+  cx<R> acc[4][3];
 #pragma unroll
   for (int s = 0; s < 4; s++)
 #pragma unroll
-    for (int k = 0; k < 3; k++) d[s][k] = {R(0), R(0)};
-  reconstruct_rt(a.mu, a.side ? -a.isign : a.isign, d, w);
-
-  cx<R> e[4][3];
-  if (OP == OP_CLOVER) {
-    clover_packed<S>(clov, vcb, idx, d, e);
-  } else if (OP == OP_TMCLOVER) {
-    clover_full<S>(clov, vcb, idx, d, e);
-  } else if (OP == OP_TM) {  // -muinv (1 - isign i mu g5) d
-    twist<R>(d, e, -a.tm_muinv, (R)a.isign * a.tm_mu * a.tm_muinv);
+    for (int k = 0; k < 3; k++) acc[s][k] = {o[(s * 3 + k) * 2], o[(s * 3 + k) * 2 + 1]};
+  if (OP == OP_WILSON) {
+    reconstruct_rt(a.mu, sgn, acc, w, -a.beta);
   } else {
+    cx<R> d[4][3];
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int k = 0; k < 3; k++) e[s][k] = {-d[s][k].re, -d[s][k].im};
+      for (int k = 0; k < 3; k++) d[s][k] = {R(0), R(0)};
+    reconstruct_rt(a.mu, sgn, d, w, OP == OP_TM ? -a.beta : a.beta);
+    if (OP == OP_CLOVER) {
+      clover_packed<S, true>(clov, vcb, idx, d, acc);
+    } else if (OP == OP_TMCLOVER) {
+      clover_full<S, true>(clov, vcb, idx, d, acc);
+    } else {  // (1 - isign i mu g5) d by one fused multiply-add per real, then muinv
+      const R m = -(R)a.isign * a.tm_mu;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const R ms = s < 2 ? m : -m;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const cx<R> v = d[s][k];
+          acc[s][k] = {fmx(a.tm_muinv, fmx(-ms, v.im, v.re), acc[s][k].re),
+                       fmx(a.tm_muinv, fmx(ms, v.re, v.im), acc[s][k].im)};
+        }
+      }
+    }
   }
-  R o[24];
-  load_reals<S, spinor_w<S>(), 0, 24>(out, vcb, idx, o);
 #pragma unroll
   for (int s = 0; s < 4; s++)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      o[(s * 3 + k) * 2] += e[s][k].re;
-      o[(s * 3 + k) * 2 + 1] += e[s][k].im;
+      o[(s * 3 + k) * 2] = acc[s][k].re;
+      o[(s * 3 + k) * 2 + 1] = acc[s][k].im;
     }
   store_reals<S, spinor_w<S>(), 0, 24>(out, vcb, idx, o);
 }
@@ -266,6 +289,7 @@ int finish(qphix_amd_ctx *ctx, int op, int prec, int recon, int isign, int cb, i
     a.ph_im = (R)ctx->tbc_im[dim];
     a.tm_mu = (R)mu;
     a.tm_muinv = (R)muinv;
+    a.one = R(1);
     S *o = static_cast<S *>(out);
     const S *b = static_cast<const S *>(buf), *u = static_cast<const S *>(gauge);
     const S *cl = static_cast<const S *>(clov);

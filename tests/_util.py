@@ -135,3 +135,51 @@ def masked_oracle(orc, dims, psi, U, isign, cb, mask, **kw):
         fwd, bwd = (h0 - 1j * h1) / 2, (h0 + 1j * h1) / 2
         total += bits[:, 2 * mu, None, None] * bwd + bits[:, 2 * mu + 1, None, None] * fwd
     return np.stack([total.real, total.imag], axis=-1)
+
+
+def face_sites(dims, mu, side, cb):
+    """Lexicographic sites of checkerboard cb on the slice x_mu = 0 (side 0)
+    or x_mu = L_mu - 1 (side 1), in the order of the face index f of the
+    library's face buffers: the slice's other coordinates, the first of them
+    halved on the checkerboard (x, or y when mu = x), fastest first."""
+    Lx, Ly, Lz, Lt = dims
+    if mu == 0:
+        t, z, yh = np.meshgrid(np.arange(Lt), np.arange(Lz), np.arange(Ly // 2), indexing="ij")
+        t, z, yh = t.ravel(), z.ravel(), yh.ravel()
+        x = np.full_like(t, Lx - 1 if side else 0)
+        y = 2 * yh + ((cb + x + z + t) & 1)
+    else:
+        if mu == 1:
+            t, z, xh = (a.ravel() for a in np.meshgrid(np.arange(Lt), np.arange(Lz), np.arange(Lx // 2),
+                                                       indexing="ij"))
+            y = np.full_like(t, Ly - 1 if side else 0)
+        elif mu == 2:
+            t, y, xh = (a.ravel() for a in np.meshgrid(np.arange(Lt), np.arange(Ly), np.arange(Lx // 2),
+                                                       indexing="ij"))
+            z = np.full_like(t, Lz - 1 if side else 0)
+        else:
+            z, y, xh = (a.ravel() for a in np.meshgrid(np.arange(Lz), np.arange(Ly), np.arange(Lx // 2),
+                                                       indexing="ij"))
+            t = np.full_like(z, Lt - 1 if side else 0)
+        x = 2 * xh + ((cb + y + z + t) & 1)
+    return ((t * Lz + z) * Ly + y) * Lx + x
+
+
+def face_width(itemsize):
+    """Reals per vector unit of a face buffer (12 reals per face site)."""
+    w = 16 // itemsize
+    return w if 12 % w == 0 else 8 // itemsize
+
+
+def face_from_buffer(buf, nface, itemsize):
+    """Face buffer (flat: unit-major, then face site, then the unit's reals)
+    -> (nface, 12) in [spin 0|1][colour][re|im] order."""
+    w = face_width(itemsize)
+    return np.asarray(buf).reshape(12 // w, nface, w).transpose(1, 0, 2).reshape(nface, 12)
+
+
+def face_to_buffer(hs, itemsize):
+    """(nface, 12) -> the flat face buffer layout."""
+    w = face_width(itemsize)
+    n = hs.shape[0]
+    return np.ascontiguousarray(np.asarray(hs).reshape(n, 12 // w, w).transpose(1, 0, 2)).ravel()
