@@ -417,8 +417,9 @@ FORCE_REPEATS = 3
 
 def _force(q, op, U, x, csw):
     """The whole pseudofermion force at x (hopping + clover terms, FP64 force
-    kernels): one call that sizes the operator's workspaces, then the median
-    wall time of FORCE_REPEATS calls (each synchronises on return)."""
+    kernels; the hopping term alone with csw None): one call that sizes the
+    operator's workspaces, then the median wall time of FORCE_REPEATS calls
+    (each synchronises on return)."""
     import torch
     F = op.force(torch.zeros_like(U), x, U, project=True, csw=csw)
     times = []
@@ -444,8 +445,9 @@ def cfg4(q, dev, cg=True):
         CG_REPEATS solves), each solution's true residual recomputed in FP64
         outside the solver, and the two solutions' difference;
       * the pseudofermion force at each solution (hopping and clover terms,
-        median of FORCE_REPEATS calls), and the force from the mixed
-        solution against the force from the FP64 solution."""
+        and the hopping term alone; median of FORCE_REPEATS calls), and the
+        force from the mixed solution against the force from the FP64
+        solution."""
     import torch
     L = q.lattice
     dims = (48, 48, 48, 96)
@@ -510,6 +512,7 @@ def cfg4(q, dev, cg=True):
                              "rsd_true_recomputed": true_residual(q, geom, op, x, b)}
         forces[name], ft = _force(q, op, U, x, csw)
         out[f"cg_{name}"]["force_seconds"] = round(ft, 4)
+        out[f"cg_{name}"]["hopping_force_seconds"] = round(_force(q, op, U, x, None)[1], 4)
         sols[name] = x.data
         op.close()
         del x

@@ -551,14 +551,14 @@ int qphix_amd_clover_force(qphix_amd_ctx *ctx, double *force_lex,
                            double scale_odd, int project, void *stream);
 
 /* Scratch-free qphix_amd_clover_force: the same arguments and result
-   (the same kernels), with its workspace -- the clover weights, 108 doubles
+   (the same kernels), with its workspace -- the clover weights, 48 doubles
    per lattice site -- in a static device array of
    QPHIX_AMD_NOALLOC_SCRATCH_BYTES (64 MiB) per GPU instead of a per-call
    hipMalloc, so the call allocates nothing and does not synchronise.  No
    counterpart in the reference; it is for callers that cannot allocate
    device memory (code under HIP graph capture, tight HMC loops, harnesses
-   that forbid hipMalloc).  Lattices of more than 77,672 sites (2 * vcb *
-   108 * 8 bytes > 64 MiB; e.g. 16^4 fits, 16^3x32 does not) return
+   that forbid hipMalloc).  Lattices of more than 174,762 sites (2 * vcb *
+   48 * 8 bytes > 64 MiB; e.g. 16^3x32 fits, 16^3x48 does not) return
    QPHIX_AMD_ESCRATCH without launching anything.  Every call on one GPU
    shares that GPU's array: calls that overlap on one GPU race.           */
 #define QPHIX_AMD_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)

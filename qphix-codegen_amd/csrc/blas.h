@@ -1,6 +1,7 @@
 // blas.h -- vector kernels and device-resident scalars of the solvers.
 #pragma once
 
+#include "force_hop.h"
 #include "internal.h"
 
 namespace qamd {
@@ -46,11 +47,32 @@ hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const d
                                  double alpha, double csw, hipStream_t st);
 hipError_t run_clover_blocks(const qphix_amd_ctx *ctx, double *out, const double *clov, int mode,
                              double mu, int sign, hipStream_t st);
+// Reals per plane and site of the clover force's weights K (anti-Hermitian
+// traceless 3x3: 8 independent reals), and per site (6 planes).
+constexpr int kKReals = 8;
+constexpr int kKSiteReals = 6 * kKReals;
+
 // Accumulates scale * link gradient of Re sum_x w(x) chi^dag A psi (lex FP64
-// fields; w = w_even / w_odd by parity); K: workspace of 6*18*V doubles
-// (component-major, K[(plane * 18 + e) * V + site]).
+// fields; w = w_even / w_odd by parity); K: workspace of kKSiteReals * V
+// doubles (component-major, K[(plane * kKReals + e) * V + site]).
 hipError_t run_clover_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *chi,
                             const double *psi, const double *U, double csw, double w_even, double w_odd,
                             double scale, int project, hipStream_t st);
+
+// The same with chi and psi given as native FP64 spinors per checkerboard,
+// and the hopping-term force of nhop terms (FP64 native spinors, unit
+// coefficients, each weighted by its w) added to each link's G before the
+// projection: the whole clover-operator force in one gather.
+hipError_t run_clover_hop_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *const chi_cb[2],
+                                const double *const psi_cb[2], const double *U, double csw, double w_even,
+                                double w_odd, double scale, int project, const HopTerm *hop, int nhop,
+                                hipStream_t st);
+
+// The hopping-term force of qphix_amd_dslash_force (FP64 native spinors,
+// unit coefficients) of one or two terms, summed before the projection.
+hipError_t run_hop_force(const qphix_amd_ctx *ctx, double *force_lex, const double *u_lex, const HopTerm *terms,
+                         int nterm, double scale, int project, hipStream_t st);
+// unit hop coefficients with the context's time boundary and twisted phases
+HopCoef hop_unit_coef(const qphix_amd_ctx *ctx);
 
 }  // namespace qamd

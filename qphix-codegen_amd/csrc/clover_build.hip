@@ -19,14 +19,22 @@ struct M3 {
   cx<double> m[3][3];
 };
 
-__device__ __forceinline__ M3 ld_link(const double *U, long s, int mu)
+// Link U_mu(s) of the lexicographic [V][4][18] link field; DAG: its adjoint.
+template <bool DAG = false>
+__device__ __forceinline__ M3 ldl(const double *U, long s, int mu)
 {
   M3 r;
   const double *p = U + (s * 4 + mu) * 18;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) r.m[i][j] = {p[(i * 3 + j) * 2], p[(i * 3 + j) * 2 + 1]};
+    for (int j = 0; j < 3; j++) {
+      const double re = p[(i * 3 + j) * 2], im = p[(i * 3 + j) * 2 + 1];
+      if (DAG)
+        r.m[j][i] = {re, -im};
+      else
+        r.m[i][j] = {re, im};
+    }
   return r;
 }
 
@@ -128,13 +136,13 @@ __global__ __launch_bounds__(128) void clover_from_links_k(double *clov, const d
       const long xmmpn = shifted(g, c, mu, -1, nu, 1), xmmmn = shifted(g, c, mu, -1, nu, -1);
       const long xpmmn = shifted(g, c, mu, 1, nu, -1);
       // four leaves, each starting and ending at x
-      M3 P1 = mul_nd(mul(ld_link(U, s, mu), ld_link(U, xpm, nu)), mul(ld_link(U, s, nu), ld_link(U, xpn, mu)));
-      M3 P2 = mul(mul_nd(ld_link(U, s, nu), ld_link(U, xmmpn, mu)),
-                  mul_dn(ld_link(U, xmm, nu), ld_link(U, xmm, mu)));
-      M3 P3 = mul(mul_dn(mul(ld_link(U, xmmmn, nu), ld_link(U, xmm, mu)), ld_link(U, xmmmn, mu)),
-                  ld_link(U, xmn, nu));
-      M3 P4 = mul(mul_dn(ld_link(U, xmn, nu), ld_link(U, xmn, mu)),
-                  mul_nd(ld_link(U, xpmmn, nu), ld_link(U, s, mu)));
+      M3 P1 = mul_nd(mul(ldl(U, s, mu), ldl(U, xpm, nu)), mul(ldl(U, s, nu), ldl(U, xpn, mu)));
+      M3 P2 = mul(mul_nd(ldl(U, s, nu), ldl(U, xmmpn, mu)),
+                  mul_dn(ldl(U, xmm, nu), ldl(U, xmm, mu)));
+      M3 P3 = mul(mul_dn(mul(ldl(U, xmmmn, nu), ldl(U, xmm, mu)), ldl(U, xmmmn, mu)),
+                  ldl(U, xmn, nu));
+      M3 P4 = mul(mul_dn(ldl(U, xmn, nu), ldl(U, xmn, mu)),
+                  mul_nd(ldl(U, xpmmn, nu), ldl(U, s, mu)));
       cx<double> F[3][3];
 #pragma unroll
       for (int i = 0; i < 3; i++)
@@ -340,9 +348,15 @@ SigmaTable make_sigma()
 // part of (Q - Q^dag)/(8i)).  Each link then collects, from the 24 leaves it
 // lies on (2 squares in each of its 3 planes, every square a leaf of its 4
 // corners), G with dL = Re tr(dU G).
-__global__ __launch_bounds__(128) void clover_weights_k(double *K, const double *chi, const double *psi,
-                                                        Geom g, double csw, double w_even, double w_odd,
-                                                        SigmaTable sig)
+// Spinor sources of the weights: lexicographic FP64 fields [V][24], or (the
+// operator's force) native FP64 spinors per checkerboard.
+struct WeightSrc {
+  const double *chi, *psi;          // lexicographic, or null
+  const double *chi_cb[2], *psi_cb[2];  // native, by checkerboard
+};
+
+__global__ __launch_bounds__(128) void clover_weights_k(double *K, WeightSrc src, Geom g, double csw, double w_even,
+                                                        double w_odd, SigmaTable sig)
 {
   const long V = (long)g.vcb * 2;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,14 +368,27 @@ __global__ __launch_bounds__(128) void clover_weights_k(double *K, const double 
   r /= g.Ly;
   const int z = (int)(r % g.Lz);
   const int t = (int)(r / g.Lz);
-  const double w = ((x + y + z + t) & 1) ? w_odd : w_even;
+  const int par = (x + y + z + t) & 1;
+  const double w = par ? w_odd : w_even;
+  double ar[24], br[24];
+  if (src.chi) {
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+      ar[k] = src.chi[s * 24 + k];
+      br[k] = src.psi[s * 24 + k];
+    }
+  } else {
+    const int cbi = (int)(s >> 1);  // (((t Lz + z) Ly + y) Lx + x) / 2, Lx even
+    load_reals<double, spinor_w<double>(), 0, 24>(src.chi_cb[par], g.vcb, cbi, ar);
+    load_reals<double, spinor_w<double>(), 0, 24>(src.psi_cb[par], g.vcb, cbi, br);
+  }
   cx<double> a[4][3], b[4][3];  // chi, psi
 #pragma unroll
   for (int sp = 0; sp < 4; sp++)
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      a[sp][c] = {chi[s * 24 + (sp * 3 + c) * 2], chi[s * 24 + (sp * 3 + c) * 2 + 1]};
-      b[sp][c] = {psi[s * 24 + (sp * 3 + c) * 2], psi[s * 24 + (sp * 3 + c) * 2 + 1]};
+      a[sp][c] = {ar[(sp * 3 + c) * 2], ar[(sp * 3 + c) * 2 + 1]};
+      b[sp][c] = {br[(sp * 3 + c) * 2], br[(sp * 3 + c) * 2 + 1]};
     }
   for (int p = 0; p < 6; p++) {
     cx<double> L[3][3];
@@ -383,53 +410,57 @@ __global__ __launch_bounds__(128) void clover_weights_k(double *K, const double 
 #pragma unroll
         for (int i = 0; i < 3; i++) L[j][i] = cmac_conj(L[j][i], a[sa][i], u[j]);
     }
-    const cx<double> tr = {(L[0][0].re + L[1][1].re + L[2][2].re) / 3.0,
-                           (L[0][0].im + L[1][1].im + L[2][2].im) / 3.0};
-    double *o = K + (long)p * 18 * V + s;  // component e at o[e * V]
+    // H = L' + L'^dag, K = f H / i = -i f H: anti-Hermitian and traceless,
+    // stored as its 8 independent reals (see ld_k)
+    const double trr = (L[0][0].re + L[1][1].re + L[2][2].re) / 3.0;
+    double *o = K + (long)p * kKReals * V + s;  // component e at o[e * V]
     const double f = 0.5 * csw * w / 8.0;
+    o[0] = -f * (2.0 * L[0][0].re - 2.0 * trr);
+    o[V] = -f * (2.0 * L[1][1].re - 2.0 * trr);
+    const int rr[3] = {0, 0, 1}, cc[3] = {1, 2, 2};
 #pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        // H = L' + L'^dag, K = f H / i = -i f H
-        cx<double> h = L[j][i] + conj(L[i][j]);
-        if (i == j) h = {h.re - 2.0 * tr.re, 0.0};
-        o[((j * 3 + i) * 2) * V] = f * h.im;
-        o[((j * 3 + i) * 2 + 1) * V] = -f * h.re;
-      }
+    for (int k = 0; k < 3; k++) {
+      const cx<double> h = L[rr[k]][cc[k]] + conj(L[cc[k]][rr[k]]);
+      o[(2 + 2 * k) * V] = f * h.im;
+      o[(3 + 2 * k) * V] = -f * h.re;
+    }
   }
 }
 
-// K is stored component-major, K[(p * 18 + e) * V + s] (e = 0..17 the
-// reals of the 3x3 matrix of plane p at site s), so a wave's loads of one
-// component are contiguous.
+// K is stored component-major, K[(p * 8 + e) * V + s], so that a wave's
+// loads of one component are contiguous; the 8 reals of the anti-Hermitian
+// traceless K of plane p at site s: Im K00, Im K11, K01, K02, K12 (re, im);
+// K22 = -i (Im K00 + Im K11), K_ji = -conj(K_ij).
 __device__ __forceinline__ M3 ld_k(const double *K, long V, long s, int p)
 {
+  const double *q = K + (long)p * kKReals * V + s;
+  const double d0 = q[0], d1 = q[V];
+  const cx<double> k01 = {q[2 * V], q[3 * V]}, k02 = {q[4 * V], q[5 * V]}, k12 = {q[6 * V], q[7 * V]};
   M3 r;
-  const double *q = K + (long)p * 18 * V + s;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) r.m[i][j] = {q[((i * 3 + j) * 2) * V], q[((i * 3 + j) * 2 + 1) * V]};
+  r.m[0][0] = {0.0, d0};
+  r.m[1][1] = {0.0, d1};
+  r.m[2][2] = {0.0, -(d0 + d1)};
+  r.m[0][1] = k01;
+  r.m[1][0] = {-k01.re, k01.im};
+  r.m[0][2] = k02;
+  r.m[2][0] = {-k02.re, k02.im};
+  r.m[1][2] = k12;
+  r.m[2][1] = {-k12.re, k12.im};
   return r;
 }
 
-__device__ __forceinline__ M3 dagm(const M3 &a)
-{
-  M3 r;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) r.m[i][j] = conj(a.m[j][i]);
-  return r;
-}
-
-__device__ __forceinline__ void acc(M3 &G, const M3 &a)
+// acc += a b
+__device__ __forceinline__ void mac(M3 &acc, const M3 &a, const M3 &b)
 {
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) G.m[i][j] = G.m[i][j] + a.m[i][j];
+    for (int j = 0; j < 3; j++) {
+      cx<double> v = acc.m[i][j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) v = cmac(v, a.m[i][k], b.m[k][j]);
+      acc.m[i][j] = v;
+    }
 }
 
 __device__ __forceinline__ void acc_dag(M3 &G, const M3 &a)
@@ -445,12 +476,33 @@ __device__ __forceinline__ int plane_of(int a, int b)  // a < b
   return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
 }
 
+// One thread per link (y, mu): G = the derivative of
+// sum_x sum_{planes} Re tr(K_p(x) Q_p(x)) with respect to U_mu(y), summed over
+// the two squares of each plane (mu, nu) that contain the link, each square
+// with its four corner insertions; squares are U_a(z) U_b(z+a) U_a(z+b)^dag
+// U_b(z)^dag, a < b, with K of plane (a, b) inserted before the factor that
+// starts at each corner.  Each square's four terms are evaluated as nested
+// products (8 matrix products per square) so that few 3x3 matrices are live
+// at once.  Output: scale * TA(U G) (project) or scale * G^dagger, added to
+// the force field of the same layout as U.
+// Hopping terms fused into the gather (their G added before the
+// projection).
+struct LinkHop {
+  HopTerm term[2];
+  int n;
+  HopCoef k;
+};
+
+template <int HOP>
 __global__ __launch_bounds__(128) void clover_links_k(double *force, const double *K, const double *U, Geom g,
-                                                      double scale, int project)
+                                                      double scale, int project, LinkHop hop)
 {
   const long V = (long)g.vcb * 2;
-  const long y = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= V) return;
+  // the four links of a chunk of sites as the fastest work items, the
+  // lattice swept t-slice by t-slice by all XCDs (neighbour reuse in cache)
+  int mu;
+  const long y = sweep_site(g, 4, mu);
+  if (y < 0) return;
   int c[4];
   {
     long r = y;
@@ -461,88 +513,111 @@ __global__ __launch_bounds__(128) void clover_links_k(double *force, const doubl
     c[2] = (int)(r % g.Lz);
     c[3] = (int)(r / g.Lz);
   }
+  M3 G;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) G.m[i][j] = {0.0, 0.0};
+  // six squares: (nu, upper / lower) for the three nu != mu, one per
+  // iteration (no loads hoisted across squares)
 #pragma unroll 1
-  for (int mu = 0; mu < 4; mu++) {
-    M3 G;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) G.m[i][j] = {0.0, 0.0};
+  for (int q = 0; q < 6; q++) {
+    int nu = q >> 1;
+    if (nu >= mu) nu++;
+    const int a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
+    const int p = plane_of(a, b);
+    const int kind = (mu == a ? 0 : 2) + (q & 1);
+    if (kind == 0) {  // square based at y, the link its first factor:
+      // G += U2 (U3 (U4 K0 + K3 U4) + K2 U3 U4) + K1 U2 U3 U4
+      const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
+      const long yb = shifted(g, c, a, 0, b, 1);
+      const M3 U4 = ldl<true>(U, y, b);
+      M3 T = mul(U4, ld_k(K, V, y, p));
+      mac(T, ld_k(K, V, yb, p), U4);
+      const M3 U3 = ldl<true>(U, yb, a);
+      const M3 U34 = mul(U3, U4);
+      M3 S = mul(U3, T);
+      mac(S, ld_k(K, V, yab, p), U34);
+      const M3 U2 = ldl(U, ya, b);
+      mac(G, U2, S);
+      mac(G, ld_k(K, V, ya, p), mul(U2, U34));
+    } else if (kind == 1) {  // square based at y - b, the link its third factor (daggered):
+      // M = V4 ((K0 V1 + V1 K1) V2 + V1 V2 K2) + K3 V4 V1 V2, G += M^dag
+      const long ymb = shifted(g, c, a, 0, b, -1), ymba = shifted(g, c, a, 1, b, -1);
+      const long ya = shifted(g, c, a, 1, b, 0);
+      const M3 V1 = ldl(U, ymb, a);
+      M3 T = mul(ld_k(K, V, ymb, p), V1);
+      mac(T, V1, ld_k(K, V, ymba, p));
+      const M3 V2 = ldl(U, ymba, b);
+      const M3 V12 = mul(V1, V2);
+      M3 S = mul(T, V2);
+      mac(S, V12, ld_k(K, V, ya, p));
+      const M3 V4 = ldl<true>(U, ymb, b);
+      M3 M = mul(V4, S);
+      mac(M, ld_k(K, V, y, p), mul(V4, V12));
+      acc_dag(G, M);
+    } else if (kind == 2) {  // square based at y - a, the link its second factor:
+      // G += W3 (W4 (K0 W1 + W1 K1) + K3 W4 W1) + K2 W3 W4 W1
+      const long yma = shifted(g, c, a, -1, b, 0), ymab = shifted(g, c, a, -1, b, 1);
+      const long yb = shifted(g, c, a, 0, b, 1);
+      const M3 W1 = ldl(U, yma, a);
+      M3 X = mul(ld_k(K, V, yma, p), W1);
+      mac(X, W1, ld_k(K, V, y, p));
+      const M3 W4 = ldl<true>(U, yma, b);
+      M3 S = mul(W4, X);
+      const M3 W41 = mul(W4, W1);
+      mac(S, ld_k(K, V, ymab, p), W41);
+      const M3 W3 = ldl<true>(U, ymab, a);
+      mac(G, W3, S);
+      mac(G, ld_k(K, V, yb, p), mul(W3, W41));
+    } else {  // square based at y, the link its fourth factor (daggered):
+      // M = U1 (U2 (K2 U3 + U3 K3) + K1 U2 U3) + K0 U1 U2 U3, G += M^dag
+      const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
+      const long yb = shifted(g, c, a, 0, b, 1);
+      const M3 U3 = ldl<true>(U, yb, a);
+      M3 X = mul(ld_k(K, V, yab, p), U3);
+      mac(X, U3, ld_k(K, V, yb, p));
+      const M3 U2 = ldl(U, ya, b);
+      M3 S = mul(U2, X);
+      const M3 U23 = mul(U2, U3);
+      mac(S, ld_k(K, V, ya, p), U23);
+      const M3 U1 = ldl(U, y, a);
+      M3 M = mul(U1, S);
+      mac(M, ld_k(K, V, y, p), mul(U1, U23));
+      acc_dag(G, M);
+    }
+  }
+  if constexpr (HOP) {
+    const int cbi_y = (((c[3] * g.Lz + c[2]) * g.Ly + c[1]) * g.Lxh) + (c[0] >> 1);
 #pragma unroll 1
-    for (int nu = 0; nu < 4; nu++) {
-      if (nu == mu) continue;
-      const int a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
-      const int p = plane_of(a, b);
-      // loops run a then b: square(z) = U_a(z) U_b(z+a) U_a(z+b)^dag U_b(z)^dag,
-      // a leaf of each of its corners z, z+a, z+a+b, z+b (cyclic rotations)
-      if (mu == a) {
-        {  // square based at y: the link is its first factor
-          const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
-          const long yb = shifted(g, c, a, 0, b, 1);
-          const M3 U2 = ld_link(U, ya, b), U3 = dagm(ld_link(U, yb, a)), U4 = dagm(ld_link(U, y, b));
-          const M3 U34 = mul(U3, U4), U23 = mul(U2, U3), U234 = mul(U2, U34);
-          acc(G, mul(U234, ld_k(K, V, y, p)));
-          acc(G, mul(ld_k(K, V, ya, p), U234));
-          acc(G, mul(mul(U2, ld_k(K, V, yab, p)), U34));
-          acc(G, mul(mul(U23, ld_k(K, V, yb, p)), U4));
-        }
-        {  // square based at y - b: the link is its third factor, daggered
-          const long ymb = shifted(g, c, a, 0, b, -1), ymba = shifted(g, c, a, 1, b, -1);
-          const long ya = shifted(g, c, a, 1, b, 0);
-          const M3 V1 = ld_link(U, ymb, a), V2 = ld_link(U, ymba, b), V4 = dagm(ld_link(U, ymb, b));
-          const M3 V41 = mul(V4, V1), V12 = mul(V1, V2), V412 = mul(V41, V2);
-          M3 M = mul(mul(V4, ld_k(K, V, ymb, p)), V12);
-          acc(M, mul(mul(V41, ld_k(K, V, ymba, p)), V2));
-          acc(M, mul(V412, ld_k(K, V, ya, p)));
-          acc(M, mul(ld_k(K, V, y, p), V412));
-          acc_dag(G, M);
-        }
-      } else {
-        {  // square based at y - a: the link is its second factor
-          const long yma = shifted(g, c, a, -1, b, 0), ymab = shifted(g, c, a, -1, b, 1);
-          const long yb = shifted(g, c, a, 0, b, 1);
-          const M3 W1 = ld_link(U, yma, a), W3 = dagm(ld_link(U, ymab, a)), W4 = dagm(ld_link(U, yma, b));
-          const M3 W34 = mul(W3, W4), W41 = mul(W4, W1), W341 = mul(W34, W1);
-          acc(G, mul(mul(W34, ld_k(K, V, yma, p)), W1));
-          acc(G, mul(W341, ld_k(K, V, y, p)));
-          acc(G, mul(ld_k(K, V, yb, p), W341));
-          acc(G, mul(mul(W3, ld_k(K, V, ymab, p)), W41));
-        }
-        {  // square based at y: the link is its fourth factor, daggered
-          const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
-          const long yb = shifted(g, c, a, 0, b, 1);
-          const M3 U1 = ld_link(U, y, a), U2 = ld_link(U, ya, b), U3 = dagm(ld_link(U, yb, a));
-          const M3 U12 = mul(U1, U2), U23 = mul(U2, U3), U123 = mul(U12, U3);
-          M3 M = mul(ld_k(K, V, y, p), U123);
-          acc(M, mul(mul(U1, ld_k(K, V, ya, p)), U23));
-          acc(M, mul(mul(U12, ld_k(K, V, yab, p)), U3));
-          acc(M, mul(U123, ld_k(K, V, yb, p)));
-          acc_dag(G, M);
-        }
-      }
-    }
-    double *out = force + (y * 4 + mu) * 18;
-    if (!project) {  // G^dagger
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          out[(i * 3 + j) * 2] += scale * G.m[j][i].re;
-          out[(i * 3 + j) * 2 + 1] -= scale * G.m[j][i].im;
-        }
-      continue;
-    }
-    // F = TA(U G)
-    const M3 W = mul(ld_link(U, y, mu), G);
-    const double trim = (W.m[0][0].im + W.m[1][1].im + W.m[2][2].im) / 3.0;
+    for (int t = 0; t < hop.n; t++) hop_g_add<double>(G.m, hop.term[t], hop.k, g, c, cbi_y, mu);
+  }
+  if (!project) {  // G^dagger
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        out[(i * 3 + j) * 2] += scale * 0.5 * (W.m[i][j].re - W.m[j][i].re);
-        out[(i * 3 + j) * 2 + 1] += scale * (0.5 * (W.m[i][j].im + W.m[j][i].im) - (i == j ? trim : 0.0));
+        const int e = (i * 3 + j) * 2;
+        double *o = force + (y * 4 + mu) * 18 + e;
+        double *oi = force + (y * 4 + mu) * 18 + e + 1;
+        *o += scale * G.m[j][i].re;
+        *oi -= scale * G.m[j][i].im;
       }
+    return;
   }
+  // F = TA(U G)
+  const M3 W = mul(ldl(U, y, mu), G);
+  const double trim = (W.m[0][0].im + W.m[1][1].im + W.m[2][2].im) / 3.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int e = (i * 3 + j) * 2;
+      double *o = force + (y * 4 + mu) * 18 + e;
+      double *oi = force + (y * 4 + mu) * 18 + e + 1;
+      *o += scale * 0.5 * (W.m[i][j].re - W.m[j][i].re);
+      *oi += scale * (0.5 * (W.m[i][j].im + W.m[j][i].im) - (i == j ? trim : 0.0));
+    }
 }
 
 }  // namespace
@@ -563,12 +638,41 @@ hipError_t run_clover_force(const qphix_amd_ctx *ctx, double *force, double *K, 
 {
   const long V = (long)ctx->g.vcb * 2;
   const int bs = 128;
-  hipLaunchKernelGGL(clover_weights_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, K, chi, psi, ctx->g, csw,
-                     w_even, w_odd, make_sigma());
+  WeightSrc src{};
+  src.chi = chi;
+  src.psi = psi;
+  hipLaunchKernelGGL(clover_weights_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, K, src, ctx->g, csw, w_even,
+                     w_odd, make_sigma());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(clover_links_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, force, K, U, ctx->g, scale,
-                     project);
+  hipLaunchKernelGGL((clover_links_k<0>), dim3(sweep_grid(ctx->g, 4, bs)), dim3(bs), 0, st, force, K, U, ctx->g,
+                     scale, project, LinkHop{});
+  return hipGetLastError();
+}
+
+hipError_t run_clover_hop_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *const chi_cb[2],
+                                const double *const psi_cb[2], const double *U, double csw, double w_even,
+                                double w_odd, double scale, int project, const HopTerm *hop, int nhop,
+                                hipStream_t st)
+{
+  if (nhop < 0 || nhop > 2 || (ctx->g.Lx & 1)) return hipErrorInvalidValue;
+  const long V = (long)ctx->g.vcb * 2;
+  const int bs = 128;
+  WeightSrc src{};
+  for (int c = 0; c < 2; c++) {
+    src.chi_cb[c] = chi_cb[c];
+    src.psi_cb[c] = psi_cb[c];
+  }
+  hipLaunchKernelGGL(clover_weights_k, dim3(blocks_for(V, bs)), dim3(bs), 0, st, K, src, ctx->g, csw, w_even,
+                     w_odd, make_sigma());
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  LinkHop lh{};
+  for (int i = 0; i < nhop; i++) lh.term[i] = hop[i];
+  lh.n = nhop;
+  lh.k = hop_unit_coef(ctx);
+  hipLaunchKernelGGL((clover_links_k<1>), dim3(sweep_grid(ctx->g, 4, bs)), dim3(bs), 0, st, force, K, U, ctx->g,
+                     scale, project, lh);
   return hipGetLastError();
 }
 

@@ -299,4 +299,48 @@ __device__ __forceinline__ int xcd_block()
 
 inline int xcd_grid(long nblocks) { return (int)((nblocks + kNumXcd - 1) / kNumXcd * kNumXcd); }
 
+// Stencil kernels on the lexicographic site order, with `sub` work items
+// (fastest) per chunk of blockDim.x consecutive sites.  When Lz is a
+// multiple of 8 the XCDs sweep t together: work-group b runs on XCD
+// x = b % 8 (the dispatch order), which owns the z-planes [x zs, x zs + zs),
+// zs = Lz / 8, and walks them t-slice by t-slice, so a t-neighbour was
+// loaded one step earlier by some XCD (one t-slice of data stays in the
+// shared last-level cache) and a z-neighbour by the same XCD (its L2).
+// Otherwise each XCD sweeps one contiguous range.  Returns the thread's
+// lexicographic site, or -1 past the end; item: the sub item.
+// bs sites per work-group (blockDim.x by default), this thread's site the
+// local-th of them.
+__device__ __forceinline__ long sweep_site(const Geom &g, int sub, int &item, int bs, int local)
+{
+  const long lxy = (long)g.Lx * g.Ly, V = lxy * g.Lz * g.Lt;
+  if (g.Lz % kNumXcd) {
+    const long b = xcd_block();
+    item = (int)(b % sub);
+    const long y = (b / sub) * bs + local;
+    return y < V ? y : -1;
+  }
+  const int x = (int)(blockIdx.x % kNumXcd);
+  long j = blockIdx.x / kNumXcd;
+  item = (int)(j % sub);
+  j /= sub;
+  const long slab = (long)(g.Lz / kNumXcd) * lxy;  // sites of one XCD per t
+  const long nch = (slab + bs - 1) / bs;
+  const long t = j / nch, off = (j % nch) * bs + local;
+  if (t >= g.Lt || off >= slab) return -1;
+  return (t * g.Lz + (long)x * (g.Lz / kNumXcd)) * lxy + off;
+}
+
+__device__ __forceinline__ long sweep_site(const Geom &g, int sub, int &item)
+{
+  return sweep_site(g, sub, item, blockDim.x, threadIdx.x);
+}
+
+inline int sweep_grid(const Geom &g, int sub, int bs)
+{
+  const long lxy = (long)g.Lx * g.Ly, V = lxy * g.Lz * g.Lt;
+  if (g.Lz % kNumXcd) return xcd_grid((long)sub * ((V + bs - 1) / bs));
+  const long slab = (long)(g.Lz / kNumXcd) * lxy;
+  return (int)((long)kNumXcd * g.Lt * ((slab + bs - 1) / bs) * sub);
+}
+
 }  // namespace qamd

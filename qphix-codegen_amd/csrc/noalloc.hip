@@ -50,7 +50,7 @@ extern "C" int qphix_amd_clover_force_noalloc(qphix_amd_ctx *ctx, double *force_
 {
   if (!ctx || !force_lex || !chi_lex || !psi_lex || !u_lex) return QPHIX_AMD_EINVAL;
   if (ctx->comm_mask) return QPHIX_AMD_EINVAL;  // needs the whole lattice
-  const size_t need = (size_t)2 * ctx->g.vcb * 108 * sizeof(double);
+  const size_t need = (size_t)2 * ctx->g.vcb * qamd::kKSiteReals * sizeof(double);
   if (need > QPHIX_AMD_NOALLOC_SCRATCH_BYTES) return QPHIX_AMD_ESCRATCH;
   void *K = nullptr;
   if (scratch_address(&K) != hipSuccess) return QPHIX_AMD_EHIP;

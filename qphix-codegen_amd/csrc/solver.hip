@@ -86,7 +86,7 @@ struct qphix_amd_eo_op {
   // clover force's lexicographic fields and K, the CG's sloppy vectors,
   // scalars and partial sums, and a pinned block for the scalars.  Calls on
   // one operator do not overlap, so they share these.
-  DevBuf w64[3], wchi, wpsi, wk, wrs, wp, wap, wxs, wscb, wpart, wpnorm;
+  DevBuf w64[3], wk, wrs, wp, wap, wxs, wscb, wpart, wpnorm;
   qamd::CGScalars *hsc = nullptr;
   ~qphix_amd_eo_op()
   {
@@ -102,42 +102,8 @@ int level_recon(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? 12 : op-
 // norm: when set, per work-group partials of |out|^2 (vcb / 128 of them);
 // step: when set too (dagger only), out holds a residual r, which receives
 // r - (*step) M^dag in, and norm gets the partials of its |r|^2
-hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
-                    hipStream_t st, double *norm = nullptr, const double *step = nullptr)
-{
-  const int isign = dagger ? -1 : 1;
-  DslashCall c{};
-  c.prec = level_prec(op, lvl);
-  c.recon = level_recon(op, lvl);
-  c.op = op->tm ? OP_TMCLOVER : OP_CLOVER;
-  c.isign = isign;
-  c.coeff_s = c.coeff_tf = c.coeff_tb = 1.0;
-  // t = F_ee^-1 D_eo in
-  c.achi = false;
-  c.cb = 0;
-  c.out = op->tmp[lvl].p;
-  c.in = in;
-  c.gauge = op->gauge[lvl][0].p;
-  c.clov = op->dsl[lvl][dagger].p;
-  hipError_t e = run_dslash_any(op->ctx, c, st);
-  if (e != hipSuccess) return e;
-  // out = F_oo in - 1/4 D_oe t
-  c.achi = true;
-  c.cb = 1;
-  c.out = out;
-  c.in = op->tmp[lvl].p;
-  c.chi = in;
-  c.gauge = op->gauge[lvl][1].p;
-  c.clov = op->achi[lvl][dagger].p;
-  c.coeff_s = c.coeff_tf = c.coeff_tb = 0.25;
-  c.rho = 0.0;
-  c.norm = norm;
-  c.step = step;
-  return run_dslash_any(op->ctx, c, st);
-}
-
-// first stage of eo_apply only: out (cb 0) = F_ee^-1 D_eo in (dagger: the
-// dagger blocks and isign -1)
+// first stage of M: out (cb 0) = F_ee^-1 D_eo in (dagger: the dagger blocks
+// and isign -1)
 hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
                      hipStream_t st)
 {
@@ -153,6 +119,38 @@ hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
   c.gauge = op->gauge[lvl][0].p;
   c.clov = op->dsl[lvl][dagger].p;
   return run_dslash_any(op->ctx, c, st);
+}
+
+// second stage: out (cb 1) = F_oo in - 1/4 D_oe t, t the first stage's result
+hipError_t eo_stage2(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *t, const void *in,
+                     hipStream_t st, double *norm = nullptr, const double *step = nullptr)
+{
+  DslashCall c{};
+  c.prec = level_prec(op, lvl);
+  c.recon = level_recon(op, lvl);
+  c.op = op->tm ? OP_TMCLOVER : OP_CLOVER;
+  c.isign = dagger ? -1 : 1;
+  c.achi = true;
+  c.cb = 1;
+  c.out = out;
+  c.in = t;
+  c.chi = in;
+  c.gauge = op->gauge[lvl][1].p;
+  c.clov = op->achi[lvl][dagger].p;
+  c.coeff_s = c.coeff_tf = c.coeff_tb = 0.25;
+  c.rho = 0.0;
+  c.norm = norm;
+  c.step = step;
+  return run_dslash_any(op->ctx, c, st);
+}
+
+// out = M in (dagger: M^dag), through the operator's cb-0 temporary
+hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
+                    hipStream_t st, double *norm = nullptr, const double *step = nullptr)
+{
+  hipError_t e = eo_stage1(op, lvl, dagger, op->tmp[lvl].p, in, st);
+  if (e != hipSuccess) return e;
+  return eo_stage2(op, lvl, dagger, out, op->tmp[lvl].p, in, st, norm, step);
 }
 
 // out = M^dag M in, via the cb-1 temporary mid = M in
@@ -288,6 +286,16 @@ int qphix_amd_eo_op_apply(qphix_amd_eo_op *op, int prec, int dagger, void *out, 
                                                                                        : QPHIX_AMD_EHIP;
 }
 
+namespace {
+// the two terms of dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>] (FP64 native
+// spinors; Y, W on checkerboard 1, 0)
+void hopping_terms(HopTerm t[2], const void *y, const void *z, const void *w, const void *x, double scale)
+{
+  t[0] = HopTerm{y, z, 1, 1, 0.5 * scale};
+  t[1] = HopTerm{w, x, 0, 1, 0.5 * scale};
+}
+}  // namespace
+
 int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
                           const double *u_lex, double scale, int project, void *stream)
 {
@@ -298,19 +306,17 @@ int qphix_amd_eo_op_force(qphix_amd_eo_op *op, double *force_lex, const void *x,
   DevBuf &y = op->w64[0], &z = op->w64[1], &w = op->w64[2];
   if (y.reserve(n * 8) != hipSuccess || z.reserve(n * 8) != hipSuccess || w.reserve(n * 8) != hipSuccess)
     return QPHIX_AMD_ENOMEM;
-  // Y = M X, Z = F_ee^-1 D_eo X, W = (F_ee^-1)^dag D_oe^dag Y
-  hipError_t e = eo_apply(op, 0, 0, y.p, x, st);
-  if (e == hipSuccess) e = eo_stage1(op, 0, 0, z.p, x, st);
+  // Z = F_ee^-1 D_eo X, Y = M X (from Z), W = (F_ee^-1)^dag D_oe^dag Y
+  hipError_t e = eo_stage1(op, 0, 0, z.p, x, st);
+  if (e == hipSuccess) e = eo_stage2(op, 0, 0, y.p, z.p, x, st);
   if (e == hipSuccess) e = eo_stage1(op, 0, 1, w.p, y.p, st);
   if (e != hipSuccess) return QPHIX_AMD_EHIP;
-  // dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>]
-  int rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 1, force_lex, y.p, z.p, u_lex, 1.0, 1.0,
-                                  1.0, 0.5 * scale, project, stream);
-  if (rc == QPHIX_AMD_OK)
-    rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 0, force_lex, w.p, x, u_lex, 1.0, 1.0,
-                                1.0, 0.5 * scale, project, stream);
-  if (rc == QPHIX_AMD_OK && hipStreamSynchronize(st) != hipSuccess) rc = QPHIX_AMD_EHIP;
-  return rc;
+  // dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>], both terms in one kernel
+  HopTerm terms[2];
+  hopping_terms(terms, y.p, z.p, w.p, x, scale);
+  e = run_hop_force(op->ctx, force_lex, u_lex, terms, 2, 1.0, project, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP;
 }
 
 int qphix_amd_clover_force(qphix_amd_ctx *ctx, double *force_lex, const double *chi_lex,
@@ -322,7 +328,7 @@ int qphix_amd_clover_force(qphix_amd_ctx *ctx, double *force_lex, const double *
   hipStream_t st = (hipStream_t)stream;
   const long V = 2L * ctx->g.vcb;
   DevBuf K;
-  if (K.alloc(V * 108 * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
+  if (K.alloc(V * kKSiteReals * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
   hipError_t e = run_clover_force(ctx, force_lex, (double *)K.p, chi_lex, psi_lex, u_lex, csw, scale_even,
                                   scale_odd, 1.0, project, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);  // K is released on return
@@ -338,32 +344,27 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
   hipStream_t st = (hipStream_t)stream;
   const long n = 24L * op->ctx->g.vcb;
   DevBuf &y = op->w64[0], &z = op->w64[1], &w = op->w64[2];
-  DevBuf &chi = op->wchi, &psi = op->wpsi, &K = op->wk;
+  DevBuf &K = op->wk;
+  const long V = 2L * op->ctx->g.vcb;
   if (y.reserve(n * 8) != hipSuccess || z.reserve(n * 8) != hipSuccess || w.reserve(n * 8) != hipSuccess ||
-      chi.reserve(2 * n * 8) != hipSuccess || psi.reserve(2 * n * 8) != hipSuccess ||
-      K.reserve(2L * op->ctx->g.vcb * 108 * 8) != hipSuccess)
+      K.reserve(V * kKSiteReals * 8) != hipSuccess)
     return QPHIX_AMD_ENOMEM;
-  // Y = M X, Z = F_ee^-1 D_eo X, W = (F_ee^-1)^dag D_oe^dag Y
-  hipError_t e = eo_apply(op, 0, 0, y.p, x, st);
-  if (e == hipSuccess) e = eo_stage1(op, 0, 0, z.p, x, st);
+  // Z = F_ee^-1 D_eo X, Y = M X (from Z), W = (F_ee^-1)^dag D_oe^dag Y
+  hipError_t e = eo_stage1(op, 0, 0, z.p, x, st);
+  if (e == hipSuccess) e = eo_stage2(op, 0, 0, y.p, z.p, x, st);
   if (e == hipSuccess) e = eo_stage1(op, 0, 1, w.p, y.p, st);
   if (e != hipSuccess) return QPHIX_AMD_EHIP;
-  // hopping term: dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>]
-  int rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 1, force_lex, y.p, z.p, u_lex, 1.0, 1.0,
-                                  1.0, 0.5 * scale, project, stream);
-  if (rc == QPHIX_AMD_OK)
-    rc = qphix_amd_dslash_force(op->ctx, QPHIX_AMD_F64, 1, 0, force_lex, w.p, x, u_lex, 1.0, 1.0,
-                                1.0, 0.5 * scale, project, stream);
-  if (rc != QPHIX_AMD_OK) return rc;
-  // clover term: dS = -2 Re <Y, dA_oo X> - 1/2 Re <W, dA_ee Z>
-  // (dF = dA: the twisted mass does not depend on the links)
-  e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)psi.p, x, 1, st);
-  if (e == hipSuccess) e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)psi.p, z.p, 0, st);
-  if (e == hipSuccess) e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)chi.p, y.p, 1, st);
-  if (e == hipSuccess) e = run_unpack_spinor(op->ctx, QPHIX_AMD_F64, (double *)chi.p, w.p, 0, st);
-  if (e == hipSuccess)
-    e = run_clover_force(op->ctx, force_lex, (double *)K.p, (const double *)chi.p, (const double *)psi.p,
-                         u_lex, csw, -0.5 * scale, -2.0 * scale, 1.0, project, st);
+  // hopping term: dS = 1/2 Re[<Y, dD_oe Z> + <W, dD_eo X>]; clover term:
+  // dS = -2 Re <Y, dA_oo X> - 1/2 Re <W, dA_ee Z> (dF = dA: the twisted mass
+  // does not depend on the links).  One gather forms both terms' G per link
+  // and adds the projected force to force_lex; chi = (W, Y) and psi = (Z, X)
+  // on checkerboards (0, 1).
+  const double *chi_cb[2] = {(const double *)w.p, (const double *)y.p};
+  const double *psi_cb[2] = {(const double *)z.p, (const double *)x};
+  HopTerm terms[2];
+  hopping_terms(terms, y.p, z.p, w.p, x, scale);
+  e = run_clover_hop_force(op->ctx, force_lex, (double *)K.p, chi_cb, psi_cb, u_lex, csw, -0.5 * scale,
+                           -2.0 * scale, 1.0, project, terms, 2, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP;
 }

@@ -64,9 +64,9 @@ def test_clover_force_noalloc_matches_allocating_entry(qamd):
 
 
 def test_clover_force_noalloc_refuses_large_lattice(qamd):
-    """16^3x32 needs 131,072 sites x 864 B = 113 MB of weights > 64 MiB."""
+    """16^3x48 needs 196,608 sites x 384 B = 75.5 MB of weights > 64 MiB."""
     lib = qamd.lib()
-    geom = qamd.Geometry((16, 16, 16, 32), t_boundary=-1.0)
+    geom = qamd.Geometry((16, 16, 16, 48), t_boundary=-1.0)
     dummy = torch.zeros(16, dtype=torch.float64, device=DEV)
     assert lib.qphix_amd_clover_force_noalloc(geom.ctx, _p(dummy), _p(dummy), _p(dummy), _p(dummy), 1.0, 1.0,
                                               1.0, 1, _stream()) == ESCRATCH
