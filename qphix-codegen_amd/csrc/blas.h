@@ -13,6 +13,15 @@ struct CGScalars {
   double pAp;      // <p, A p>
   double alpha, beta;
   double tmp;      // result of a plain reduction
+  // The reliable-update trigger, evaluated on the device after each
+  // iteration's |r|^2 (finalize, kind 1): with the iterated residual in true
+  // units rest = sqrt(rr) sigma, the iterations stop (stop = 1) when
+  // rest < delta rmax or rest^2 < target2 (mixed; plain CG: rest^2 <
+  // target2) or rr is not finite, else rmax = max(rmax, rest).
+  double sigma, rmax, delta, target2;
+  int mixed;
+  int stop;   // set: the iteration kernels still queued return at once
+  int iters;  // iterations completed since the host last set the scalars
 };
 
 // work-groups of a vector kernel (and partial sums of a reduction)
@@ -22,26 +31,33 @@ struct CGScalars {
 constexpr int kMaxPartials = QAMD_MAX_PARTIALS;
 
 int blas_grid(long nsites);
+// The kernels of a CG iteration take `stop` (&sc->stop there, else null):
+// when it points to a set flag they return without doing anything.
+
 // Re<a, b> over n = 24 * vcb reals (whole sites); kind: 0 -> alpha = rr / result,
 // 2 -> sc->tmp = result.
 void blas_dot(int prec, const void *a, const void *b, long n, double *partial, CGScalars *sc,
-              int kind, hipStream_t st);
+              int kind, hipStream_t st, const int *stop = nullptr);
 // y = a x + b y (types may differ); with sc != null, sc->tmp = |y|^2.
 void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double b, long n,
                 double *partial, CGScalars *sc, hipStream_t st);
 // Sum n partials in a fixed order into sc (kind as for blas_dot); above
 // 2048 partials in two stages through scratch (n / 2048 doubles, at most
 // kMaxPartials).
-void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, double *scratch, hipStream_t st);
+void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, double *scratch, hipStream_t st,
+                   const int *stop = nullptr);
 // r -= alpha Ap, x += alpha p, then beta = |r|^2 / rr, rr = |r|^2.
 void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const void *Ap, long n,
-                    double *partial, CGScalars *sc, hipStream_t st);
+                    double *partial, CGScalars *sc, hipStream_t st, const int *stop = nullptr);
 // p = r + beta p
-void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st);
+void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st,
+                  const int *stop = nullptr);
 // x += alpha p, then (update_p) p = r + beta p: the vector half of a CG step
 // whose r -= alpha Ap and |r|^2 the operator's last kernel did
+// *sc = v in stream order, without a host synchronisation
+void blas_set_scalars(CGScalars *sc, const CGScalars &v, hipStream_t st);
 void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long n, CGScalars *sc,
-                    bool update_p, hipStream_t st);
+                    bool update_p, hipStream_t st, const int *stop = nullptr);
 
 hipError_t run_clover_from_links(const qphix_amd_ctx *ctx, double *clov, const double *U,
                                  double alpha, double csw, hipStream_t st);

@@ -133,8 +133,10 @@ __global__ __launch_bounds__(kBlasBlock) void axpby_k(const SX *__restrict__ x, 
 template <typename S, typename SXS>
 __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *__restrict__ r, SXS *__restrict__ x,
                                                           const S *__restrict__ p, const S *__restrict__ Ap,
-                                                          const CGScalars *sc, int vcb, double *partial)
+                                                          const CGScalars *sc, int vcb, double *partial,
+                                                          const int *stop)
 {
+  if (stop && *stop) return;
   using R = typename Traits<S>::R;
   using RX = typename Traits<SXS>::R;
   const R alpha = (R)sc->alpha;
@@ -171,8 +173,9 @@ __global__ __launch_bounds__(kBlasBlock) void cg_update_k(S *__restrict__ r, SXS
 // p = r + beta p, beta = sc->beta
 template <typename S>
 __global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *__restrict__ p, const S *__restrict__ r,
-                                                        const CGScalars *sc, int vcb)
+                                                        const CGScalars *sc, int vcb, const int *stop)
 {
+  if (stop && *stop) return;
   using R = typename Traits<S>::R;
   const R beta = (R)sc->beta;
   QAMD_SITE_LOOP(site, vcb)
@@ -195,8 +198,9 @@ __global__ __launch_bounds__(kBlasBlock) void cg_xpay_k(S *__restrict__ p, const
 template <typename S, typename SXS, bool P>
 __global__ __launch_bounds__(kBlasBlock) void cg_xpay_x_k(S *__restrict__ p, SXS *__restrict__ x,
                                                           const S *__restrict__ r, const CGScalars *sc,
-                                                          int vcb)
+                                                          int vcb, const int *stop)
 {
+  if (stop && *stop) return;
   using R = typename Traits<S>::R;
   using RX = typename Traits<SXS>::R;
   const R alpha = (R)sc->alpha, beta = (R)sc->beta;
@@ -242,11 +246,13 @@ __global__ __launch_bounds__(kBlasBlock) void reduce_partials_k(const double *__
 
 // Sum the partials in a fixed order and update the scalars:
 //   kind 0: pAp  -> alpha = rr / pAp
-//   kind 1: rr_new -> beta = rr_new / rr, rr = rr_new
+//   kind 1: rr_new -> beta = rr_new / rr, rr = rr_new, one more iteration
+//           done, and the reliable-update trigger (see CGScalars)
 //   kind 2: plain sum into sc->tmp
 __global__ __launch_bounds__(kBlasBlock) void finalize_k(const double *partial, int n, CGScalars *sc,
-                                                         int kind)
+                                                         int kind, const int *stop)
 {
+  if (stop && *stop) return;
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += kBlasBlock) v += partial[i];
   const double t = block_sum(v);
@@ -258,11 +264,22 @@ __global__ __launch_bounds__(kBlasBlock) void finalize_k(const double *partial, 
       sc->beta = t / sc->rr;
       sc->rr_prev = sc->rr;
       sc->rr = t;
+      sc->iters += 1;
+      const double rest = sqrt(t) * sc->sigma;
+      const bool reliable = sc->mixed ? (rest < sc->delta * sc->rmax || rest * rest < sc->target2)
+                                      : rest * rest < sc->target2;
+      if (!isfinite(t) || reliable)
+        sc->stop = 1;
+      else if (rest > sc->rmax)
+        sc->rmax = rest;
     } else {
       sc->tmp = t;
     }
   }
 }
+
+// *sc = v (one thread; the values travel as the launch's argument)
+__global__ void set_scalars_k(CGScalars *sc, CGScalars v) { *sc = v; }
 
 template <typename S> const S *cp(const void *p) { return static_cast<const S *>(p); }
 template <typename S> S *mp(void *p) { return static_cast<S *>(p); }
@@ -284,7 +301,7 @@ int blas_grid(long nsites)
   }
 
 void blas_dot(int prec, const void *a, const void *b, long n, double *partial, CGScalars *sc,
-              int kind, hipStream_t st)
+              int kind, hipStream_t st, const int *stop)
 {
   const int ng = (int)(n / 24);  // sites
   const int grid = blas_grid(ng);
@@ -292,25 +309,27 @@ void blas_dot(int prec, const void *a, const void *b, long n, double *partial, C
   hipLaunchKernelGGL(dot_k<T>, dim3(grid), dim3(kBlasBlock), 0, st, cp<T>(a), cp<T>(b), ng, partial)
   QAMD_PREC_SWITCH(prec, QAMD_DOT)
 #undef QAMD_DOT
-  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, kind);
+  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, kind, stop);
 }
 
-void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, double *scratch, hipStream_t st)
+void blas_finalize(const double *partial, long n, CGScalars *sc, int kind, double *scratch, hipStream_t st,
+                   const int *stop)
 {
   if (n > kReduceChunk) {  // one work-group per 2048 partials first
     const int g = (int)((n + kReduceChunk - 1) / kReduceChunk);
     hipLaunchKernelGGL(reduce_partials_k, dim3(g), dim3(kBlasBlock), 0, st, partial, (int)n, scratch);
-    hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, scratch, g, sc, kind);
+    hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, scratch, g, sc, kind, stop);
     return;
   }
-  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, (int)n, sc, kind);
+  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, (int)n, sc, kind, stop);
 }
 
 void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double b, long n,
                 double *partial, CGScalars *sc, hipStream_t st)
 {
   const int ng = (int)(n / 24);  // sites
-  const int grid = blas_grid(ng);
+  // one site per thread unless there are partial sums (at most kMaxPartials)
+  const int grid = sc ? blas_grid(ng) : blocks_for(ng, kBlasBlock);
   double *pp = sc ? partial : nullptr;
 #define QAMD_AXPBY_Y(TY)                                                                    \
   switch (prec_x) {                                                                         \
@@ -333,34 +352,34 @@ void blas_axpby(int prec_x, const void *x, int prec_y, void *y, double a, double
   }
   QAMD_PREC_SWITCH(prec_y, QAMD_AXPBY_Y)
 #undef QAMD_AXPBY_Y
-  if (sc) hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, 2);
+  if (sc) hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, 2, nullptr);
 }
 
 void blas_cg_update(int prec, int prec_x, void *r, void *x, const void *p, const void *Ap, long n,
-                    double *partial, CGScalars *sc, hipStream_t st)
+                    double *partial, CGScalars *sc, hipStream_t st, const int *stop)
 {
   const int ng = (int)(n / 24);  // sites
   const int grid = blas_grid(ng);
 #define QAMD_CGU(T)                                                                          \
   if (prec_x == QPHIX_AMD_F64)                                                               \
     hipLaunchKernelGGL((cg_update_k<T, double>), dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(r), \
-                       mp<double>(x), cp<T>(p), cp<T>(Ap), sc, ng, partial);                 \
+                       mp<double>(x), cp<T>(p), cp<T>(Ap), sc, ng, partial, stop);           \
   else                                                                                       \
     hipLaunchKernelGGL((cg_update_k<T, float>), dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(r), \
-                       mp<float>(x), cp<T>(p), cp<T>(Ap), sc, ng, partial);
+                       mp<float>(x), cp<T>(p), cp<T>(Ap), sc, ng, partial, stop);
   QAMD_PREC_SWITCH(prec, QAMD_CGU)
 #undef QAMD_CGU
-  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, 1);
+  hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kBlasBlock), 0, st, partial, grid, sc, 1, stop);
 }
 
 void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long n, CGScalars *sc,
-                    bool update_p, hipStream_t st)
+                    bool update_p, hipStream_t st, const int *stop)
 {
   const int ng = (int)(n / 24);  // sites
-  const int grid = blas_grid(ng);
+  const int grid = blocks_for(ng, kBlasBlock);  // no partials: one site per thread
 #define QAMD_XPX_P(T, TX, P)                                                                    \
   hipLaunchKernelGGL((cg_xpay_x_k<T, TX, P>), dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(p), \
-                     mp<TX>(x), cp<T>(r), sc, ng)
+                     mp<TX>(x), cp<T>(r), sc, ng, stop)
 #define QAMD_XPX_X(T, TX)        \
   if (update_p)                  \
     QAMD_XPX_P(T, TX, true);     \
@@ -378,14 +397,19 @@ void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long 
 #undef QAMD_XPX_P
 }
 
-void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st)
+void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st, const int *stop)
 {
   const int ng = (int)(n / 24);  // sites
-  const int grid = blas_grid(ng);
+  const int grid = blocks_for(ng, kBlasBlock);  // no partials: one site per thread
 #define QAMD_XPAY(T) \
-  hipLaunchKernelGGL(cg_xpay_k<T>, dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(p), cp<T>(r), sc, ng)
+  hipLaunchKernelGGL(cg_xpay_k<T>, dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(p), cp<T>(r), sc, ng, stop)
   QAMD_PREC_SWITCH(prec, QAMD_XPAY)
 #undef QAMD_XPAY
+}
+
+void blas_set_scalars(CGScalars *sc, const CGScalars &v, hipStream_t st)
+{
+  hipLaunchKernelGGL(set_scalars_k, dim3(1), dim3(1), 0, st, sc, v);
 }
 
 }  // namespace qamd

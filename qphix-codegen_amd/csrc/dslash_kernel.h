@@ -48,6 +48,9 @@ template <typename R> struct DslashParams {
   // when set (with norm): out holds a CG residual r, and the launch stores
   // r - (*step) * result in its place; norm then sums |r|^2
   const double *step;
+  // when set and *stop != 0 the launch does nothing (CG iterations queued
+  // past a reliable update)
+  const int *stop;
   // QPhiX FourSpinorBlock arrays (kernel template SPL = 1): soalen 1 << q_sh,
   // block strides q_nvecs (y), q_pxy (z), q_pxyz (t)
   int q_sh, q_nvecs, q_pxy, q_pxyz;
@@ -567,7 +570,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
 #else
   const int blk = (int)blockIdx.x;
 #endif
-  if (blk >= p.nblocks) return;
+  if (blk >= p.nblocks || (p.stop && *p.stop)) return;
   const Geom &g = p.g;
   int idx;
   if (p.order == 1) {

@@ -28,6 +28,9 @@ struct DslashCall {
   // optional, with norm, for the clover achimbdpsi kernels with isign -1:
   // out holds a residual r and receives r - (*step) * result
   const double *step;
+  // optional (CG iterations): when it points to a set flag the launch does
+  // nothing
+  const int *stop;
   // spl = 1: in, chi and out are QPhiX FourSpinorBlock arrays of the
   // geometry q (converted from a qphix_amd_qphix_layout); acc, when set, is
   // the per-site direction mask in their site order (block * soalen + lane)

@@ -105,9 +105,10 @@ int level_recon(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? 12 : op-
 // first stage of M: out (cb 0) = F_ee^-1 D_eo in (dagger: the dagger blocks
 // and isign -1)
 hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
-                     hipStream_t st)
+                     hipStream_t st, const int *stop = nullptr)
 {
   DslashCall c{};
+  c.stop = stop;
   c.prec = level_prec(op, lvl);
   c.recon = level_recon(op, lvl);
   c.op = op->tm ? OP_TMCLOVER : OP_CLOVER;
@@ -123,9 +124,11 @@ hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
 
 // second stage: out (cb 1) = F_oo in - 1/4 D_oe t, t the first stage's result
 hipError_t eo_stage2(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *t, const void *in,
-                     hipStream_t st, double *norm = nullptr, const double *step = nullptr)
+                     hipStream_t st, double *norm = nullptr, const double *step = nullptr,
+                     const int *stop = nullptr)
 {
   DslashCall c{};
+  c.stop = stop;
   c.prec = level_prec(op, lvl);
   c.recon = level_recon(op, lvl);
   c.op = op->tm ? OP_TMCLOVER : OP_CLOVER;
@@ -146,20 +149,21 @@ hipError_t eo_stage2(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
 
 // out = M in (dagger: M^dag), through the operator's cb-0 temporary
 hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
-                    hipStream_t st, double *norm = nullptr, const double *step = nullptr)
+                    hipStream_t st, double *norm = nullptr, const double *step = nullptr,
+                    const int *stop = nullptr)
 {
-  hipError_t e = eo_stage1(op, lvl, dagger, op->tmp[lvl].p, in, st);
+  hipError_t e = eo_stage1(op, lvl, dagger, op->tmp[lvl].p, in, st, stop);
   if (e != hipSuccess) return e;
-  return eo_stage2(op, lvl, dagger, out, op->tmp[lvl].p, in, st, norm, step);
+  return eo_stage2(op, lvl, dagger, out, op->tmp[lvl].p, in, st, norm, step, stop);
 }
 
 // out = M^dag M in, via the cb-1 temporary mid = M in
 hipError_t eo_normal(qphix_amd_eo_op *op, int lvl, void *out, void *mid, const void *in,
-                     hipStream_t st)
+                     hipStream_t st, const int *stop = nullptr)
 {
-  hipError_t e = eo_apply(op, lvl, 0, mid, in, st);
+  hipError_t e = eo_apply(op, lvl, 0, mid, in, st, nullptr, nullptr, stop);
   if (e != hipSuccess) return e;
-  return eo_apply(op, lvl, 1, out, mid, st);
+  return eo_apply(op, lvl, 1, out, mid, st, nullptr, nullptr, stop);
 }
 
 }  // namespace
@@ -369,14 +373,12 @@ int qphix_amd_eo_op_force_clover(qphix_amd_eo_op *op, double *force_lex, const v
   return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP;
 }
 
-// CG iterations enqueued between two host reads of the scalars: BF16
-// storage (7-bit mantissa) wants its reliable updates on time, the others
-// tolerate a few iterations of delay
+// CG iterations enqueued between two host reads of the scalars.  The
+// reliable-update trigger is evaluated on the device after every iteration,
+// and the iterations queued after it fire return at once, so the batch
+// length costs only those empty launches, never a late update.
 #ifndef QAMD_CG_CHECK
-#define QAMD_CG_CHECK 4
-#endif
-#ifndef QAMD_CG_CHECK_BF16
-#define QAMD_CG_CHECK_BF16 1
+#define QAMD_CG_CHECK 8
 #endif
 // <p, M^dag M p> = |M p|^2 summed by the kernel that stores M p, instead of
 // a dot product of p and M^dag M p (which reads both again); and the
@@ -403,10 +405,12 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   const int lvl = mixed ? 1 : 0;
   // Default reliable-update threshold by sloppy storage: the FP16 (10-bit
   // mantissa) and FP32 iterations stay accurate across a larger residual drop
-  // than BF16 (7-bit); values from a sweep at 32^3x64, kappa 0.135, csw 1.
+  // than BF16 (7-bit); values from a sweep of the time to solution at
+  // 32^3x64, kappa 0.135, csw 1 (FP16: 0.02 -> 123 iterations, 6 updates;
+  // BF16: 0.15 -> 131 iterations, 12 updates).
   const double delta = prm->delta > 0.0 ? prm->delta
-                       : ps == QPHIX_AMD_BF16 ? 0.1
-                       : ps == QPHIX_AMD_F16  ? 0.05
+                       : ps == QPHIX_AMD_BF16 ? 0.15
+                       : ps == QPHIX_AMD_F16  ? 0.02
                                               : 0.01;
 
   // fused |M p|^2 needs every work-group of the operator kernels full
@@ -435,14 +439,20 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e;
   };
-  auto set_scalars = [&](double rr, double beta) -> hipError_t {
+  double target2 = 0.0;  // |r|^2 to reach (true units), set from |b|^2 below
+  auto set_scalars = [&](double rr, double beta, double sig, double rmx) -> hipError_t {
     CGScalars h{};
     h.rr = rr;
     h.beta = beta;
-    hipError_t e = hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);  // h is on the stack
-    return e;
+    h.sigma = sig;
+    h.rmax = rmx;
+    h.delta = delta;
+    h.mixed = mixed ? 1 : 0;
+    h.target2 = target2;
+    blas_set_scalars(sc, h, st);
+    return hipGetLastError();
   };
+
   // r64 = b - M^dag M x64, returns |r64|^2 in *r2
   auto true_residual = [&](double *r2) -> hipError_t {
     hipError_t e = eo_normal(op, 0, ax64.p, mid.p, x, st);
@@ -464,91 +474,95 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   if (e == hipSuccess) e = fetch();
   if (e != hipSuccess) return QPHIX_AMD_EHIP;
   const double b2 = hsc->tmp;
-  const double target2 = prm->rsd_target * prm->rsd_target * b2;
+  target2 = prm->rsd_target * prm->rsd_target * b2;
   double r2 = b2;
   // sloppy units: r_s = r / sigma
   double sigma = mixed ? std::sqrt(r2 / (double)n) : 1.0;
   if (!(sigma > 0.0)) sigma = 1.0;
   blas_axpby(QPHIX_AMD_F64, r64.p, ps, rs.p, 1.0 / sigma, 0.0, n, partial, nullptr, st);
   blas_axpby(ps, rs.p, ps, p.p, 1.0, 0.0, n, partial, nullptr, st);
-  e = set_scalars(r2 / (sigma * sigma), 0.0);
   double rmax = std::sqrt(r2);  // max true-unit residual since the last update
+  e = set_scalars(r2 / (sigma * sigma), 0.0, sigma, rmax);
   bool converged = b2 == 0.0;
   int rc = QPHIX_AMD_OK;
+  int base = 0;         // iterations before the scalars were last set
+  bool restart = true;  // the next iteration starts from a fresh p
+  int last_len = 0;     // iterations between the last two reliable updates
+  const int *stop = &sc->stop;
   while (e == hipSuccess && !converged && iters < prm->max_iters) {
-    // a batch of iterations is enqueued back to back (alpha, beta and the
-    // norms never leave the device); the host reads the scalars once per
-    // batch and only then decides between the plain p update and a reliable
-    // update, which therefore comes at most batch - 1 iterations later than a
-    // per-iteration check would place it.
-    const int batch = std::min(ps == QPHIX_AMD_BF16 ? QAMD_CG_CHECK_BF16 : QAMD_CG_CHECK,
-                               prm->max_iters - iters);
+    // A batch of iterations is enqueued back to back (alpha, beta, the norms
+    // and the reliable-update trigger never leave the device); an iteration
+    // begins with the previous one's x += alpha p, p = r + beta p.  Once the
+    // trigger fires, the rest of the batch does nothing; the host reads the
+    // scalars once per batch.
+    // batches end near the iteration where the trigger last fired (counted
+    // from the update before), so that few launches are queued past it
+    int batch = QAMD_CG_CHECK;
+    if (last_len > 0) batch = std::max(2, std::min(batch, last_len - (iters - base) + 1));
+    batch = std::min(batch, prm->max_iters - iters);
     for (int k = 0; k < batch && e == hipSuccess; k++) {
+      const bool step_p = !(restart && k == 0);
       if (fuse) {
-        // (x += alpha p, p = r + beta p); mid = M p with the partials of
-        // |M p|^2 -> alpha; r -= alpha M^dag mid with those of |r|^2 -> beta
-        if (k > 0) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, true, st);
+        // mid = M p with the partials of |M p|^2 -> alpha; r -= alpha M^dag
+        // mid with those of |r|^2 -> beta, and the trigger
+        if (step_p) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, true, st, stop);
         double *pn = (double *)pnorm.p;
-        e = eo_apply(op, lvl, 0, mid.p, p.p, st, pn);
+        e = eo_apply(op, lvl, 0, mid.p, p.p, st, pn, nullptr, stop);
         if (e != hipSuccess) break;
-        blas_finalize(pn, nwg, sc, 0, partial, st);
-        e = eo_apply(op, lvl, 1, rs.p, mid.p, st, pn, &sc->alpha);
+        blas_finalize(pn, nwg, sc, 0, partial, st, stop);
+        e = eo_apply(op, lvl, 1, rs.p, mid.p, st, pn, &sc->alpha, stop);
         if (e != hipSuccess) break;
-        blas_finalize(pn, nwg, sc, 1, partial, st);
+        blas_finalize(pn, nwg, sc, 1, partial, st, stop);
       } else {
-        if (k > 0) blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
-        e = eo_normal(op, lvl, ap.p, mid.p, p.p, st);
+        if (step_p) blas_cg_xpay(ps, p.p, rs.p, n, sc, st, stop);
+        e = eo_normal(op, lvl, ap.p, mid.p, p.p, st, stop);
         if (e != hipSuccess) break;
-        blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st);
-        blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st);
+        blas_dot(ps, p.p, ap.p, n, partial, sc, 0, st, stop);
+        blas_cg_update(ps, px, rs.p, xs.p, p.p, ap.p, n, partial, sc, st, stop);
       }
-      mv += 2;
-      iters++;
     }
     if (e != hipSuccess) break;
+    restart = false;
     e = fetch();
     if (e != hipSuccess) break;
-    const double rr = hsc->rr;              // |r_s|^2
-    const double rest = std::sqrt(rr) * sigma;  // iterated residual, true units
+    iters = base + hsc->iters;
+    if (!hsc->stop) continue;  // the batch ran through
+    const double rr = hsc->rr;  // |r_s|^2
     if (!std::isfinite(rr)) {
       rc = QPHIX_AMD_ENOCONV;
       break;
     }
-    const bool reliable = mixed ? (rest < delta * rmax || rest * rest < target2)
-                                : (rest * rest < target2);
-    if (reliable) {
-      // fold the increment into x, recompute the true residual in FP64
-      if (fuse) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, false, st);
-      blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
-      e = hipMemsetAsync(xs.p, 0, n * sbytes(px), st);
-      if (e != hipSuccess) break;
-      e = true_residual(&r2);
-      if (e != hipSuccess) break;
-      updates++;
-      mv += 2;
-      if (r2 < target2) {
-        converged = true;
-        break;
-      }
-      const double sig_new = mixed ? std::sqrt(r2 / (double)n) : 1.0;
-      // beta from the true residual against the previous iterated one
-      const double beta = r2 / (hsc->rr_prev * sigma * sigma);
-      blas_axpby(QPHIX_AMD_F64, r64.p, ps, rs.p, 1.0 / sig_new, 0.0, n, partial, nullptr, st);
-      e = set_scalars(r2 / (sig_new * sig_new), beta * sigma / sig_new);
-      if (e != hipSuccess) break;
-      blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
-      sigma = sig_new;
-      rmax = std::sqrt(r2);
-    } else {
-      if (fuse)
-        blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, true, st);
-      else
-        blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
-      if (rest > rmax) rmax = rest;
+    // reliable update: fold the increment into x, recompute the true
+    // residual in FP64
+    if (fuse) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, false, st);
+    blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
+    e = hipMemsetAsync(xs.p, 0, n * sbytes(px), st);
+    if (e != hipSuccess) break;
+    e = true_residual(&r2);
+    if (e != hipSuccess) break;
+    updates++;
+    if (r2 < target2) {
+      converged = true;
+      break;
     }
+    const double sig_new = mixed ? std::sqrt(r2 / (double)n) : 1.0;
+    // beta from the true residual against the previous iterated one
+    const double beta = r2 / (hsc->rr_prev * sigma * sigma);
+    blas_axpby(QPHIX_AMD_F64, r64.p, ps, rs.p, 1.0 / sig_new, 0.0, n, partial, nullptr, st);
+    rmax = std::sqrt(r2);
+    e = set_scalars(r2 / (sig_new * sig_new), beta * sigma / sig_new, sig_new, rmax);
+    if (e != hipSuccess) break;
+    blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+    sigma = sig_new;
+    last_len = iters - base;
+    base = iters;
+    restart = true;
   }
+  mv = 2L * iters + 2L * updates;
   if (e == hipSuccess && !converged && rc == QPHIX_AMD_OK) {
-    // report the true residual of the final iterate
+    // report the true residual of the final iterate (its last x += alpha p
+    // first, when an iteration is pending)
+    if (fuse && !restart) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, false, st);
     blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
     e = true_residual(&r2);
     converged = r2 < target2;
