@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -248,6 +249,154 @@ int gpu_mode(const char *in_path, const char *out_path)
   return 0;
 }
 
+// A lattice split in two along t, emulated on one GPU through the C++
+// classes: each half has its own Geometry (t marked split with set_comm),
+// its links handed over in a QPhiX layout with the back links of its
+// boundary sites from across the cut (gauge halo), runs every family's
+// operator (the interior part), packs both t faces, and completes them from
+// the other half's faces through the family's face_finish_* wrapper.
+// Input: int dims[4] of a half, int n (doubles of a half's QPhiX gauge
+// array: FP64, veclen = soalen = 1, 18 reals), then per half the gauge array
+// and lexicographic FP64 psi, chi, A^-1, A, B (the (A + i mu g5)^-1 blocks)
+// and F (the A + i mu g5 blocks), isign +1, output checkerboard 0.
+// Output: per (family, dslash | achimbdpsi), per half, the lexicographic
+// result (checkerboard 0 sites set).
+int split_mode(const char *in_path, const char *out_path)
+{
+  using namespace qphix_amd;
+  FILE *f = std::fopen(in_path, "rb");
+  if (!f) return fail("cannot open input");
+  int dims[4], ng = 0;
+  if (std::fread(dims, sizeof(int), 4, f) != 4 || std::fread(&ng, sizeof(int), 1, f) != 1)
+    return fail("short input");
+  const long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+  const long nS = V * 24, nC = V * 72, nF = V * 144;
+  const long per = ng + 2 * nS + 2 * nC + 2 * nF;
+  std::vector<double> h(2 * per);
+  if (std::fread(h.data(), sizeof(double), h.size(), f) != h.size()) return fail("short input");
+  std::fclose(f);
+
+  const double aniso_s = 0.9, aniso_t = 1.1, alpha = 4.2, beta = 0.25, rho = 0.05, mass = 0.2, tmu = 0.1;
+  const double t_boundary = -1.0;
+  const int prec = QPHIX_AMD_F64, recon = 18, isign = 1, cb = 0, T = 3;
+  const qphix_amd_qphix_layout lay = {QPHIX_AMD_F64, 1, 1, 0, 0, 0};
+  const bool comm[4] = {false, false, false, true};
+  struct Half {
+    Geometry geom;
+    Dev g, psi, chi, Ai, A, B, F, res, outlex, face0, face1;
+    Half(const int *d, long nS)
+        : geom(d[0], d[1], d[2], d[3], -1.0),
+          g(geom.field_bytes(QPHIX_AMD_FIELD_GAUGE, QPHIX_AMD_F64, 18)),
+          psi(geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, QPHIX_AMD_F64)),
+          chi(geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, QPHIX_AMD_F64)),
+          Ai(geom.field_bytes(QPHIX_AMD_FIELD_CLOVER, QPHIX_AMD_F64)),
+          A(geom.field_bytes(QPHIX_AMD_FIELD_CLOVER, QPHIX_AMD_F64)),
+          B(geom.field_bytes(QPHIX_AMD_FIELD_FULLCLOVER, QPHIX_AMD_F64)),
+          F(geom.field_bytes(QPHIX_AMD_FIELD_FULLCLOVER, QPHIX_AMD_F64)),
+          res(geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, QPHIX_AMD_F64)),
+          outlex(nS * 8),
+          face0(geom.face_sites(3) * 12 * 8),
+          face1(geom.face_sites(3) * 12 * 8)
+    {
+    }
+    void *face(int fb) const { return fb ? face1.p : face0.p; }
+  };
+  std::vector<std::unique_ptr<Half>> half;
+  for (int k = 0; k < 2; k++) {
+    half.emplace_back(new Half(dims, nS));
+    Half &H = *half[k];
+    H.geom.set_comm(comm);
+    const double *src = h.data() + k * per;
+    Dev qg(ng * 8), lex(nF * 8);
+    HIPCHECK(hipMemcpy(qg.p, src, ng * 8, hipMemcpyHostToDevice));
+    if (H.geom.qphix_field_bytes(lay, QPHIX_AMD_FIELD_GAUGE) != (size_t)ng * 8) return fail("gauge array size");
+    H.geom.gauge_from_qphix(lay, H.g.p, prec, recon, qg.p, cb);
+    src += ng;
+    struct {
+      Dev *dst;
+      long n;
+      int kind;  // 0 spinor cb 1, 1 spinor cb 0, 2 clover, 3 full clover
+    } fields[] = {{&H.psi, nS, 0}, {&H.chi, nS, 1}, {&H.Ai, nC, 2}, {&H.A, nC, 2}, {&H.B, nF, 3}, {&H.F, nF, 3}};
+    for (const auto &fd : fields) {
+      HIPCHECK(hipMemcpy(lex.p, src, fd.n * 8, hipMemcpyHostToDevice));
+      if (fd.kind == 0) H.geom.pack_spinor(fd.dst->p, prec, lex.d(), 1 - cb);
+      if (fd.kind == 1) H.geom.pack_spinor(fd.dst->p, prec, lex.d(), cb);
+      if (fd.kind == 2) H.geom.pack_clover(fd.dst->p, prec, lex.d(), cb);
+      if (fd.kind == 3) H.geom.pack_full_clover(fd.dst->p, prec, lex.d(), cb);
+      src += fd.n;
+    }
+  }
+  FILE *o = std::fopen(out_path, "wb");
+  if (!o) return fail("cannot open output");
+  std::vector<double> host(nS);
+  for (int kind = 0; kind < 4; kind++)
+    for (int achi = 0; achi < 2; achi++) {
+      for (int k = 0; k < 2; k++) {
+        Half &H = *half[k];
+        Dslash W(H.geom, prec, recon, aniso_s, aniso_t);
+        ClovDslash C(H.geom, prec, recon, aniso_s, aniso_t);
+        TMDslash Tm(H.geom, prec, recon, mass, tmu, aniso_s, aniso_t);
+        TMClovDslash TC(H.geom, prec, recon, aniso_s, aniso_t);
+        switch (kind) {  // the interior part
+          case 0:
+            if (achi) W.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, alpha, beta, isign, cb);
+            else W.dslash(H.res.p, H.psi.p, H.g.p, isign, cb);
+            break;
+          case 1:
+            if (achi) C.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, H.A.p, beta, isign, cb, rho);
+            else C.dslash(H.res.p, H.psi.p, H.g.p, H.Ai.p, isign, cb);
+            break;
+          case 2:
+            if (achi) Tm.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, alpha, beta, isign, cb);
+            else Tm.dslash(H.res.p, H.psi.p, H.g.p, isign, cb);
+            break;
+          default:
+            if (achi) TC.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, H.F.p, beta, isign, cb, rho);
+            else TC.dslash(H.res.p, H.psi.p, H.g.p, H.B.p, isign, cb);
+        }
+        for (int fb = 0; fb < 2; fb++) H.geom.face_pack(prec, isign, cb, T, fb, H.face(fb), H.psi.p);
+      }
+      for (int k = 0; k < 2; k++) {
+        Half &H = *half[k];
+        Dslash W(H.geom, prec, recon, aniso_s, aniso_t);
+        ClovDslash C(H.geom, prec, recon, aniso_s, aniso_t);
+        TMDslash Tm(H.geom, prec, recon, mass, tmu, aniso_s, aniso_t);
+        TMClovDslash TC(H.geom, prec, recon, aniso_s, aniso_t);
+        for (int fb = 0; fb < 2; fb++) {
+          // fb 0 completes t = 0 from the half behind (its forward face),
+          // fb 1 t = Lt - 1 from the half ahead (its backward face)
+          const Half &nb = *half[1 - k];
+          const void *buf = nb.face(1 - fb);
+          const bool edge = fb == 0 ? k == 0 : k == 1;  // the global time boundary
+          const double coeff = aniso_t * (edge ? t_boundary : 1.0);
+          if (achi) {
+            const double b = beta * coeff;
+            switch (kind) {
+              case 0: W.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb); break;
+              case 1: C.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb); break;
+              case 2: Tm.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb); break;
+              default: TC.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb);
+            }
+          } else {
+            switch (kind) {
+              case 0: W.face_finish_dslash(H.res.p, buf, H.g.p, -coeff, isign, cb, T, fb); break;
+              case 1: C.face_finish_dslash(H.res.p, buf, H.g.p, H.Ai.p, coeff, isign, cb, T, fb); break;
+              case 2: Tm.face_finish_dslash(H.res.p, buf, H.g.p, -coeff, isign, cb, T, fb); break;
+              default: TC.face_finish_dslash(H.res.p, buf, H.g.p, H.B.p, coeff, isign, cb, T, fb);
+            }
+          }
+        }
+        HIPCHECK(hipMemset(H.outlex.p, 0, nS * 8));
+        H.geom.unpack_spinor(H.outlex.d(), H.res.p, prec, cb);
+        HIPCHECK(hipMemcpy(host.data(), H.outlex.p, nS * 8, hipMemcpyDeviceToHost));
+        std::fwrite(host.data(), sizeof(double), nS, o);
+      }
+    }
+  std::fclose(o);
+  std::printf("split ok\n");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -255,9 +404,10 @@ int main(int argc, char **argv)
   try {
     if (argc >= 2 && std::strcmp(argv[1], "cpu") == 0) return cpu_mode();
     if (argc >= 4 && std::strcmp(argv[1], "gpu") == 0) return gpu_mode(argv[2], argv[3]);
+    if (argc >= 4 && std::strcmp(argv[1], "split") == 0) return split_mode(argv[2], argv[3]);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "host_api_check: %s\n", e.what());
     return 1;
   }
-  return fail("usage: host_api_check cpu | gpu <in.bin> <out.bin>");
+  return fail("usage: host_api_check cpu | gpu <in.bin> <out.bin> | split <in.bin> <out.bin>");
 }

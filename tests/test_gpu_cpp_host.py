@@ -96,3 +96,70 @@ def test_cpp_classes_against_oracle(qamd, orc, tmp_path):
         err = np.abs(fr[k] - F).max() / np.abs(F).max()
         assert err < 1e-13, (k, err)
     op.close()
+
+
+def test_cpp_face_wrappers_split_t(qamd, orc, tmp_path):
+    """The C++ face wrappers (Geometry::face_pack, Dslash / ClovDslash /
+    TMDslash / TMClovDslash::face_finish_dslash and face_finish_achimbdpsi)
+    on a 16x8x8x8 lattice split in two along t (host_api_check split): each
+    half runs the interior operator, packs its t faces, and completes its
+    boundary sites from the other half's faces; the halves assembled must
+    equal the whole-lattice FP64 oracle (1e-13), for every family, dslash and
+    achimbdpsi (isign +1, output checkerboard 0, antiperiodic time)."""
+    import qphix_blocks as qb
+    assert os.path.exists(EXE), "tests/cpp/host_api_check is not built (build() builds it)"
+    G = (16, 8, 8, 8)
+    Lloc = (16, 8, 8, 4)
+    L = qamd.lattice
+    U = L.random_gauge(G, 0.3, 3200).numpy()
+    psi = L.random_spinor(G, 3201).numpy()
+    chi = L.random_spinor(G, 3202).numpy()
+    clov_t = L.random_clover(G, 3203)
+    clov, inv = clov_t.numpy(), L.invert_clover(clov_t).numpy()
+    B = L.tmclover_blocks(clov_t, TMU, 1, True).numpy()
+    F = L.tmclover_blocks(clov_t, TMU, 1, False).numpy()
+    lay = qb.Layout(0, 1, 1, compress12=False)
+    halves = []
+    for k in range(2):
+        origin = (0, 0, 0, k * Lloc[3])
+        t, z, y, x = np.meshgrid(*(np.arange(n) for n in Lloc[::-1]), indexing="ij")
+        gx, gy, gz, gt = (c.ravel() + o for c, o in zip((x, y, z, t), origin))
+        gi = ((gt * G[2] + gz) * G[1] + gy) * G[0] + gx
+        garr = qb.pack_gauge(U, lay, Lloc, 0, origin=origin, gdims=G).astype(np.float64).ravel()
+        halves.append((gi, garr))
+    inp = tmp_path / "split.bin"
+    with open(inp, "wb") as f:
+        np.asarray(Lloc, dtype=np.int32).tofile(f)
+        np.asarray([halves[0][1].size], dtype=np.int32).tofile(f)
+        for gi, garr in halves:
+            garr.tofile(f)
+            for a in (psi, chi, inv, clov, B, F):
+                np.ascontiguousarray(a[gi], dtype=np.float64).tofile(f)
+    out = tmp_path / "split_out.bin"
+    print("running host_api_check split", G, "in two along t", flush=True)
+    r = subprocess.run([EXE, "split", str(inp), str(out)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    Vl = int(np.prod(Lloc))
+    res = np.fromfile(out, dtype=np.float64).reshape(4, 2, 2, Vl, 4, 3, 2)
+    m = cb_mask(G, 0)
+    ml = cb_mask(Lloc, 0)  # the halves start at even t, so local parity = global parity
+    for kind in range(4):
+        for achi in (0, 1):
+            c = BETA if achi else 1.0
+            kw = dict(kind=kind, achi=bool(achi), coeff_s=c * ANISO_S, coeff_t_f=c * ANISO_T,
+                      coeff_t_b=c * ANISO_T, t_boundary=-1.0, alpha=ALPHA, rho=RHO, chi=chi)
+            if kind == 1:
+                kw["clov"] = clov if achi else inv
+            elif kind == 2:
+                a_op = 4.0 + MASS
+                kw["mu"] = TMU / ALPHA if achi else TMU / a_op
+                kw["muinv"] = a_op / (a_op ** 2 + TMU ** 2)
+            elif kind == 3:
+                kw["clov"] = F if achi else B
+            want = orc.dslash(G, psi, U, 1, 0, **kw)
+            got = np.zeros_like(want)
+            for k, (gi, _) in enumerate(halves):
+                got[gi[ml]] = res[kind, achi, k][ml]
+            err = np.abs(got[m] - want[m]).max() / np.abs(want[m]).max()
+            print(f"  split t: kind {kind} achi {achi}: {err:.3g}", flush=True)
+            assert err < 1e-13, (kind, achi, err)
