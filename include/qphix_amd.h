@@ -78,7 +78,7 @@ int qphix_amd_set_tbc(qphix_amd_ctx *ctx, const int tbc[4],
    reference (its SIMD loop has no work-group schedule); this is an MI355X
    tuning knob for callers that know their workload better than the default.
      -1  automatic (the default): 1 when one t-slice holds more than 32768
-         sites per checkerboard or Lt <= Lz, else 0;
+         sites per checkerboard or Lt <= Lz, else 3;
       0  each of the 8 XCDs sweeps its own t-slab;
       1  the XCDs sweep t together, each owning a z-range (Lz % 8 == 0);
       2  t-slabs visited z-major (Lt % 8 == 0);

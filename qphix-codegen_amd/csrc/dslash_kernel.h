@@ -83,7 +83,8 @@ constexpr int kDslashBlock = QAMD_DSLASH_BLOCK;
 // Work-group traversal order (DslashParams::order) for this geometry: the
 // requested one (qphix_amd_set_traversal, else QAMD_BLOCK_ORDER) when the
 // geometry admits it, else 0.  Automatic choice: 1 when one t-slice holds
-// more than 32768 sites per checkerboard or Lt <= Lz, else 0.
+// more than 32768 sites per checkerboard or Lt <= Lz, else 3 (0 where the
+// geometry does not admit 3).
 // * With order 0 every XCD sweeps its own t-slab.  Between the first and the
 //   last read of an input t-slice the eight XCDs then stream two slices each
 //   of every field: ~230 MB at 32^3 slices (FP32 clover), within the 256 MB
@@ -91,15 +92,19 @@ constexpr int kDslashBlock = QAMD_DSLASH_BLOCK;
 //   to HBM.  Thin slabs (Lt/8 <= Lz/8 slices) also put most slices next to
 //   another XCD's slab.
 // * Order 1 has the XCDs sweep t together, each owning a z-range.
+// * Order 3 is order 0 with each XCD's slab swept one z-part at a time, so
+//   the three input t-slices that meet in its L2 are half as large.
 // Measured FP32 clov_dslash, order 0 -> 1: 48^3x96 915 -> 855 us, 32^3x32
 // 82.7 -> 82.0, 32^3x16 46.6 -> 42.7, 32^3x8 24.3 -> 22.6, 32x32x64x64
-// 340 -> 334; order 0 kept: 32^3x64 152 -> 166, 32^3x128 331 -> 334,
-// 16^3x32 and 64x32x32x64 equal.
+// 340 -> 334; not taken: 32^3x64 152 -> 166, 32^3x128 331 -> 334,
+// 16^3x32 and 64x32x32x64 equal.  At 32^3x64, order 0 -> 3 (rocprofv3):
+// clov_dslash 155.0 -> 152.8 us, its achimbdpsi 166.4 -> 165.1, Wilson
+// 108.6 -> 107.9, with 3-5% fewer L2 misses (2, 4 or 8 z-parts equal).
 inline int dslash_traversal_for(const Geom &g, int requested)
 {
   if (g.sz % kDslashBlock != 0) return 0;
   int ord = requested;
-  if (ord < 0) ord = ((long)g.sz * g.Lz > 32768 || g.Lt <= g.Lz) ? 1 : 0;
+  if (ord < 0) ord = ((long)g.sz * g.Lz > 32768 || g.Lt <= g.Lz) ? 1 : 3;
   if (ord == 1 && g.Lz % kNumXcd == 0) return 1;
   if (ord == 2 && g.Lt % kNumXcd == 0) return 2;
   if (ord == 3 && g.Lt % kNumXcd == 0 && g.Lz % QAMD_ZPARTS == 0) return 3;

@@ -99,7 +99,7 @@ class Geometry:
 
     def set_traversal(self, order):
         """Work-group traversal of the operator kernels: -1 automatic (1 for
-        t-slices above 32768 sites per checkerboard or Lt <= Lz), 0 a
+        t-slices above 32768 sites per checkerboard or Lt <= Lz, else 3), 0 a
         t-slab per XCD, 1 XCDs sweep t together owning z-ranges, 2 t-slabs
         z-major, 3 t-slabs in two z-halves (bit-identical results)."""
         check(lib().qphix_amd_set_traversal(self._ctx, int(order)), "qphix_amd_set_traversal")

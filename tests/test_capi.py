@@ -114,15 +114,16 @@ def test_face_entry_points_validate(qamd):
 
 def test_traversal_choice(qamd):
     """Automatic work-group traversal: order 1 (XCDs sweep t together) when a
-    t-slice holds more than 32768 sites per checkerboard or Lt <= Lz, else 0
-    (and Lz % 8 == 0); requested
-    orders the geometry does not admit run as 0; bad arguments rejected."""
+    t-slice holds more than 32768 sites per checkerboard or Lt <= Lz (and
+    Lz % 8 == 0), else 3 (t-slabs swept one z-part at a time; Lt % 8 == 0);
+    requested orders the geometry does not admit run as 0; bad arguments
+    rejected."""
     lib = qamd.lib()
     cases = {  # dims: (automatic, effective order for requests 0..3)
-        (32, 32, 32, 64): (0, [0, 1, 2, 3]),
+        (32, 32, 32, 64): (3, [0, 1, 2, 3]),
         (48, 48, 48, 96): (1, [0, 1, 2, 3]),
-        (64, 32, 32, 64): (0, [0, 1, 2, 3]),
-        (32, 32, 32, 128): (0, [0, 1, 2, 3]),
+        (64, 32, 32, 64): (3, [0, 1, 2, 3]),
+        (32, 32, 32, 128): (3, [0, 1, 2, 3]),
         (32, 32, 32, 32): (1, [0, 1, 2, 3]),   # Lt <= Lz
         (32, 32, 32, 8): (1, [0, 1, 2, 3]),
         (24, 24, 24, 48): (0, [0, 0, 0, 0]),   # 12 x 24 = 288 sites per plane
