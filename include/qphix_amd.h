@@ -495,6 +495,7 @@ typedef struct qphix_amd_eo_op qphix_amd_eo_op;
 int qphix_amd_eo_op_create(qphix_amd_eo_op **op, qphix_amd_ctx *ctx, const double *u_lex,
                            const double *clov_lex, double mu, int prec_sloppy,
                            int recon_sloppy, void *stream);
+/* Frees the operator; like hipFree it waits for all work on the device. */
 int qphix_amd_eo_op_destroy(qphix_amd_eo_op *op);
 /* out = M in (dagger = 0) or M^dagger in (dagger = 1); prec is
    QPHIX_AMD_F64 or the operator's sloppy precision; cb-1 native spinors. */

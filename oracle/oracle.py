@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SO = os.path.join(_HERE, "liboracle.so")
+# QAMD_ORACLE_SO: another build of the same source (the sanitizer build,
+# oracle/Makefile target liboracle_san.so)
+_SO = os.environ.get("QAMD_ORACLE_SO") or os.path.join(_HERE, "liboracle.so")
 
 WILSON, CLOVER, TM, TMCLOVER = 0, 1, 2, 3
 

@@ -476,15 +476,84 @@ __device__ __forceinline__ int plane_of(int a, int b)  // a < b
   return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
 }
 
-// One thread per link (y, mu): G = the derivative of
-// sum_x sum_{planes} Re tr(K_p(x) Q_p(x)) with respect to U_mu(y), summed over
+// The link U_mu(y) (coordinates c) collects G = the derivative of
+// sum_x sum_{planes} Re tr(K_p(x) Q_p(x)) with respect to it, summed over
 // the two squares of each plane (mu, nu) that contain the link, each square
 // with its four corner insertions; squares are U_a(z) U_b(z+a) U_a(z+b)^dag
 // U_b(z)^dag, a < b, with K of plane (a, b) inserted before the factor that
-// starts at each corner.  Each square's four terms are evaluated as nested
-// products (8 matrix products per square) so that few 3x3 matrices are live
-// at once.  Output: scale * TA(U G) (project) or scale * G^dagger, added to
-// the force field of the same layout as U.
+// starts at each corner.  square_g adds square q (0..5: nu the (q/2)-th
+// direction other than mu, q odd the square on the far side) to G.  Each
+// square's four terms are evaluated as nested products (8 matrix products
+// per square) so that few 3x3 matrices are live at once.
+__device__ __forceinline__ void square_g(M3 &G, const double *K, const double *U, const Geom &g, long V,
+                                         const int c[4], long y, int mu, int q)
+{
+  int nu = q >> 1;
+  if (nu >= mu) nu++;
+  const int a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
+  const int p = plane_of(a, b);
+  const int kind = (mu == a ? 0 : 2) + (q & 1);
+  if (kind == 0) {  // square based at y, the link its first factor:
+    // G += U2 (U3 (U4 K0 + K3 U4) + K2 U3 U4) + K1 U2 U3 U4
+    const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
+    const long yb = shifted(g, c, a, 0, b, 1);
+    const M3 U4 = ldl<true>(U, y, b);
+    M3 T = mul(U4, ld_k(K, V, y, p));
+    mac(T, ld_k(K, V, yb, p), U4);
+    const M3 U3 = ldl<true>(U, yb, a);
+    const M3 U34 = mul(U3, U4);
+    M3 S = mul(U3, T);
+    mac(S, ld_k(K, V, yab, p), U34);
+    const M3 U2 = ldl(U, ya, b);
+    mac(G, U2, S);
+    mac(G, ld_k(K, V, ya, p), mul(U2, U34));
+  } else if (kind == 1) {  // square based at y - b, the link its third factor (daggered):
+    // M = V4 ((K0 V1 + V1 K1) V2 + V1 V2 K2) + K3 V4 V1 V2, G += M^dag
+    const long ymb = shifted(g, c, a, 0, b, -1), ymba = shifted(g, c, a, 1, b, -1);
+    const long ya = shifted(g, c, a, 1, b, 0);
+    const M3 V1 = ldl(U, ymb, a);
+    M3 T = mul(ld_k(K, V, ymb, p), V1);
+    mac(T, V1, ld_k(K, V, ymba, p));
+    const M3 V2 = ldl(U, ymba, b);
+    const M3 V12 = mul(V1, V2);
+    M3 S = mul(T, V2);
+    mac(S, V12, ld_k(K, V, ya, p));
+    const M3 V4 = ldl<true>(U, ymb, b);
+    M3 M = mul(V4, S);
+    mac(M, ld_k(K, V, y, p), mul(V4, V12));
+    acc_dag(G, M);
+  } else if (kind == 2) {  // square based at y - a, the link its second factor:
+    // G += W3 (W4 (K0 W1 + W1 K1) + K3 W4 W1) + K2 W3 W4 W1
+    const long yma = shifted(g, c, a, -1, b, 0), ymab = shifted(g, c, a, -1, b, 1);
+    const long yb = shifted(g, c, a, 0, b, 1);
+    const M3 W1 = ldl(U, yma, a);
+    M3 X = mul(ld_k(K, V, yma, p), W1);
+    mac(X, W1, ld_k(K, V, y, p));
+    const M3 W4 = ldl<true>(U, yma, b);
+    M3 S = mul(W4, X);
+    const M3 W41 = mul(W4, W1);
+    mac(S, ld_k(K, V, ymab, p), W41);
+    const M3 W3 = ldl<true>(U, ymab, a);
+    mac(G, W3, S);
+    mac(G, ld_k(K, V, yb, p), mul(W3, W41));
+  } else {  // square based at y, the link its fourth factor (daggered):
+    // M = U1 (U2 (K2 U3 + U3 K3) + K1 U2 U3) + K0 U1 U2 U3, G += M^dag
+    const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
+    const long yb = shifted(g, c, a, 0, b, 1);
+    const M3 U3 = ldl<true>(U, yb, a);
+    M3 X = mul(ld_k(K, V, yab, p), U3);
+    mac(X, U3, ld_k(K, V, yb, p));
+    const M3 U2 = ldl(U, ya, b);
+    M3 S = mul(U2, X);
+    const M3 U23 = mul(U2, U3);
+    mac(S, ld_k(K, V, ya, p), U23);
+    const M3 U1 = ldl(U, y, a);
+    M3 M = mul(U1, S);
+    mac(M, ld_k(K, V, y, p), mul(U1, U23));
+    acc_dag(G, M);
+  }
+}
+
 // Hopping terms fused into the gather (their G added before the
 // projection).
 struct LinkHop {
@@ -493,115 +562,95 @@ struct LinkHop {
   HopCoef k;
 };
 
+// Order in which the gather visits the lattice.  A work-group is one chunk
+// of kLinkChunk sites of an (x, y) cross-section of Lx * yb sites (yb rows,
+// at most kLinkSection sites) and one direction mu.  With zparts = 8 (Lz a
+// multiple of 8) work-group b runs on XCD b % 8 (the dispatch order), which
+// owns the z-planes [k zs, (k + 1) zs); each XCD walks, for one y-block
+// after the other, its z-planes (fastest) and then t, so that a site's z-
+// and t-neighbours were read a few cross-sections earlier and are still in
+// that XCD's L2.  At 48^3 x 96 (960 B of K and links per site) a
+// cross-section is 8 rows, 369 KB, and a t-neighbour was read 2.2 MB of
+// data earlier, against 13 MB when each XCD swept whole t-slabs.
+constexpr int kLinkChunk = 64;
+constexpr int kLinkSection = 512;
+struct LinkSweep {
+  int zparts, zs;  // z-parts (8 or 1) and z-planes per part
+  int yb;          // rows of a cross-section (a divisor of Ly)
+  int cs, ncx;     // sites of a cross-section and its chunks
+};
+
+LinkSweep link_sweep(const Geom &g)
+{
+  LinkSweep w;
+  w.zparts = g.Lz % kNumXcd ? 1 : kNumXcd;
+  w.zs = g.Lz / w.zparts;
+  w.yb = 1;
+  for (int d = 1; d <= g.Ly; d++)
+    if (g.Ly % d == 0 && (long)g.Lx * d <= kLinkSection) w.yb = d;
+  w.cs = g.Lx * w.yb;
+  w.ncx = (w.cs + kLinkChunk - 1) / kLinkChunk;
+  return w;
+}
+
+long link_sweep_blocks(const Geom &g, const LinkSweep &w)
+{
+  return (long)w.zparts * 4 * w.ncx * w.zs * g.Lt * (g.Ly / w.yb);
+}
+
+// One thread per link (y, mu), its six squares one per loop iteration (no
+// loads hoisted across squares), then the hopping terms.  Output:
+// scale * TA(U G) (project) or scale * G^dagger, added to the force field of
+// the same layout as U.
 template <int HOP>
-__global__ __launch_bounds__(128) void clover_links_k(double *force, const double *K, const double *U, Geom g,
-                                                      double scale, int project, LinkHop hop)
+__global__ __launch_bounds__(kLinkChunk) void clover_links_k(double *force, const double *K, const double *U, Geom g,
+                                                             LinkSweep w, double scale, int project, LinkHop hop)
 {
   const long V = (long)g.vcb * 2;
-  // the four links of a chunk of sites as the fastest work items, the
-  // lattice swept t-slice by t-slice by all XCDs (neighbour reuse in cache)
-  int mu;
-  const long y = sweep_site(g, 4, mu);
-  if (y < 0) return;
-  int c[4];
-  {
-    long r = y;
-    c[0] = (int)(r % g.Lx);
-    r /= g.Lx;
-    c[1] = (int)(r % g.Ly);
-    r /= g.Ly;
-    c[2] = (int)(r % g.Lz);
-    c[3] = (int)(r / g.Lz);
+  long j = blockIdx.x;
+  int part = 0;
+  if (w.zparts == kNumXcd) {
+    part = (int)(j % kNumXcd);
+    j /= kNumXcd;
   }
+  const int mu = (int)(j % 4);
+  j /= 4;
+  const int ch = (int)(j % w.ncx);
+  j /= w.ncx;
+  const int zl = (int)(j % w.zs);
+  j /= w.zs;
+  const int t = (int)(j % g.Lt);
+  const int yblk = (int)(j / g.Lt);
+  const int o = ch * kLinkChunk + (int)threadIdx.x;
+  if (o >= w.cs) return;  // the last chunk of a cross-section may be partial
+  int c[4];
+  c[0] = o % g.Lx;
+  c[1] = yblk * w.yb + o / g.Lx;
+  c[2] = part * w.zs + zl;
+  c[3] = t;
+  const long y = lex_of(g, c);
   M3 G;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) G.m[i][j] = {0.0, 0.0};
-  // six squares: (nu, upper / lower) for the three nu != mu, one per
-  // iteration (no loads hoisted across squares)
+    for (int k = 0; k < 3; k++) G.m[i][k] = {0.0, 0.0};
 #pragma unroll 1
-  for (int q = 0; q < 6; q++) {
-    int nu = q >> 1;
-    if (nu >= mu) nu++;
-    const int a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
-    const int p = plane_of(a, b);
-    const int kind = (mu == a ? 0 : 2) + (q & 1);
-    if (kind == 0) {  // square based at y, the link its first factor:
-      // G += U2 (U3 (U4 K0 + K3 U4) + K2 U3 U4) + K1 U2 U3 U4
-      const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
-      const long yb = shifted(g, c, a, 0, b, 1);
-      const M3 U4 = ldl<true>(U, y, b);
-      M3 T = mul(U4, ld_k(K, V, y, p));
-      mac(T, ld_k(K, V, yb, p), U4);
-      const M3 U3 = ldl<true>(U, yb, a);
-      const M3 U34 = mul(U3, U4);
-      M3 S = mul(U3, T);
-      mac(S, ld_k(K, V, yab, p), U34);
-      const M3 U2 = ldl(U, ya, b);
-      mac(G, U2, S);
-      mac(G, ld_k(K, V, ya, p), mul(U2, U34));
-    } else if (kind == 1) {  // square based at y - b, the link its third factor (daggered):
-      // M = V4 ((K0 V1 + V1 K1) V2 + V1 V2 K2) + K3 V4 V1 V2, G += M^dag
-      const long ymb = shifted(g, c, a, 0, b, -1), ymba = shifted(g, c, a, 1, b, -1);
-      const long ya = shifted(g, c, a, 1, b, 0);
-      const M3 V1 = ldl(U, ymb, a);
-      M3 T = mul(ld_k(K, V, ymb, p), V1);
-      mac(T, V1, ld_k(K, V, ymba, p));
-      const M3 V2 = ldl(U, ymba, b);
-      const M3 V12 = mul(V1, V2);
-      M3 S = mul(T, V2);
-      mac(S, V12, ld_k(K, V, ya, p));
-      const M3 V4 = ldl<true>(U, ymb, b);
-      M3 M = mul(V4, S);
-      mac(M, ld_k(K, V, y, p), mul(V4, V12));
-      acc_dag(G, M);
-    } else if (kind == 2) {  // square based at y - a, the link its second factor:
-      // G += W3 (W4 (K0 W1 + W1 K1) + K3 W4 W1) + K2 W3 W4 W1
-      const long yma = shifted(g, c, a, -1, b, 0), ymab = shifted(g, c, a, -1, b, 1);
-      const long yb = shifted(g, c, a, 0, b, 1);
-      const M3 W1 = ldl(U, yma, a);
-      M3 X = mul(ld_k(K, V, yma, p), W1);
-      mac(X, W1, ld_k(K, V, y, p));
-      const M3 W4 = ldl<true>(U, yma, b);
-      M3 S = mul(W4, X);
-      const M3 W41 = mul(W4, W1);
-      mac(S, ld_k(K, V, ymab, p), W41);
-      const M3 W3 = ldl<true>(U, ymab, a);
-      mac(G, W3, S);
-      mac(G, ld_k(K, V, yb, p), mul(W3, W41));
-    } else {  // square based at y, the link its fourth factor (daggered):
-      // M = U1 (U2 (K2 U3 + U3 K3) + K1 U2 U3) + K0 U1 U2 U3, G += M^dag
-      const long ya = shifted(g, c, a, 1, b, 0), yab = shifted(g, c, a, 1, b, 1);
-      const long yb = shifted(g, c, a, 0, b, 1);
-      const M3 U3 = ldl<true>(U, yb, a);
-      M3 X = mul(ld_k(K, V, yab, p), U3);
-      mac(X, U3, ld_k(K, V, yb, p));
-      const M3 U2 = ldl(U, ya, b);
-      M3 S = mul(U2, X);
-      const M3 U23 = mul(U2, U3);
-      mac(S, ld_k(K, V, ya, p), U23);
-      const M3 U1 = ldl(U, y, a);
-      M3 M = mul(U1, S);
-      mac(M, ld_k(K, V, y, p), mul(U1, U23));
-      acc_dag(G, M);
-    }
-  }
+  for (int q = 0; q < 6; q++) square_g(G, K, U, g, V, c, y, mu, q);
   if constexpr (HOP) {
     const int cbi_y = (((c[3] * g.Lz + c[2]) * g.Ly + c[1]) * g.Lxh) + (c[0] >> 1);
 #pragma unroll 1
-    for (int t = 0; t < hop.n; t++) hop_g_add<double>(G.m, hop.term[t], hop.k, g, c, cbi_y, mu);
+    for (int h = 0; h < hop.n; h++) hop_g_add<double>(G.m, hop.term[h], hop.k, g, c, cbi_y, mu);
   }
   if (!project) {  // G^dagger
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        const int e = (i * 3 + j) * 2;
-        double *o = force + (y * 4 + mu) * 18 + e;
+      for (int k = 0; k < 3; k++) {
+        const int e = (i * 3 + k) * 2;
+        double *op = force + (y * 4 + mu) * 18 + e;
         double *oi = force + (y * 4 + mu) * 18 + e + 1;
-        *o += scale * G.m[j][i].re;
-        *oi -= scale * G.m[j][i].im;
+        *op += scale * G.m[k][i].re;
+        *oi -= scale * G.m[k][i].im;
       }
     return;
   }
@@ -611,13 +660,25 @@ __global__ __launch_bounds__(128) void clover_links_k(double *force, const doubl
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const int e = (i * 3 + j) * 2;
-      double *o = force + (y * 4 + mu) * 18 + e;
+    for (int k = 0; k < 3; k++) {
+      const int e = (i * 3 + k) * 2;
+      double *op = force + (y * 4 + mu) * 18 + e;
       double *oi = force + (y * 4 + mu) * 18 + e + 1;
-      *o += scale * 0.5 * (W.m[i][j].re - W.m[j][i].re);
-      *oi += scale * (0.5 * (W.m[i][j].im + W.m[j][i].im) - (i == j ? trim : 0.0));
+      *op += scale * 0.5 * (W.m[i][k].re - W.m[k][i].re);
+      *oi += scale * (0.5 * (W.m[i][k].im + W.m[k][i].im) - (i == k ? trim : 0.0));
     }
+}
+
+template <int HOP>
+hipError_t launch_clover_links(const qphix_amd_ctx *ctx, double *force, const double *K, const double *U,
+                               double scale, int project, const LinkHop &lh, hipStream_t st)
+{
+  const LinkSweep w = link_sweep(ctx->g);
+  const long nb = link_sweep_blocks(ctx->g, w);
+  if (nb > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((clover_links_k<HOP>), dim3((unsigned)nb), dim3(kLinkChunk), 0, st, force, K, U, ctx->g, w,
+                     scale, project, lh);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -645,9 +706,7 @@ hipError_t run_clover_force(const qphix_amd_ctx *ctx, double *force, double *K, 
                      w_odd, make_sigma());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((clover_links_k<0>), dim3(sweep_grid(ctx->g, 4, bs)), dim3(bs), 0, st, force, K, U, ctx->g,
-                     scale, project, LinkHop{});
-  return hipGetLastError();
+  return launch_clover_links<0>(ctx, force, K, U, scale, project, LinkHop{}, st);
 }
 
 hipError_t run_clover_hop_force(const qphix_amd_ctx *ctx, double *force, double *K, const double *const chi_cb[2],
@@ -671,9 +730,7 @@ hipError_t run_clover_hop_force(const qphix_amd_ctx *ctx, double *force, double 
   for (int i = 0; i < nhop; i++) lh.term[i] = hop[i];
   lh.n = nhop;
   lh.k = hop_unit_coef(ctx);
-  hipLaunchKernelGGL((clover_links_k<1>), dim3(sweep_grid(ctx->g, 4, bs)), dim3(bs), 0, st, force, K, U, ctx->g,
-                     scale, project, lh);
-  return hipGetLastError();
+  return launch_clover_links<1>(ctx, force, K, U, scale, project, lh, st);
 }
 
 hipError_t run_clover_blocks(const qphix_amd_ctx *ctx, double *out, const double *clov, int mode,

@@ -267,9 +267,13 @@ int qphix_amd_eo_op_create(qphix_amd_eo_op **out_op, qphix_amd_ctx *ctx, const d
   return QPHIX_AMD_OK;
 }
 
+// Frees the operator's device memory.  hipFree waits for the whole device
+// (HIP's documented behaviour) before releasing a block, so work still
+// queued on any stream that uses the operator completes first; destroy is
+// therefore a set-up call (once per gauge configuration), not one for a hot
+// loop.
 int qphix_amd_eo_op_destroy(qphix_amd_eo_op *op)
 {
-  if (op) (void)hipDeviceSynchronize();
   delete op;
   return QPHIX_AMD_OK;
 }
