@@ -562,41 +562,11 @@ struct LinkHop {
   HopCoef k;
 };
 
-// Order in which the gather visits the lattice.  A work-group is one chunk
-// of kLinkChunk sites of an (x, y) cross-section of Lx * yb sites (yb rows,
-// at most kLinkSection sites) and one direction mu.  With zparts = 8 (Lz a
-// multiple of 8) work-group b runs on XCD b % 8 (the dispatch order), which
-// owns the z-planes [k zs, (k + 1) zs); each XCD walks, for one y-block
-// after the other, its z-planes (fastest) and then t, so that a site's z-
-// and t-neighbours were read a few cross-sections earlier and are still in
-// that XCD's L2.  At 48^3 x 96 (960 B of K and links per site) a
-// cross-section is 8 rows, 369 KB, and a t-neighbour was read 2.2 MB of
-// data earlier, against 13 MB when each XCD swept whole t-slabs.
+// The gather's work-groups: one chunk of kLinkChunk sites of a cross-section
+// (cross_sweep) and one direction mu.  At 48^3 x 96 (960 B of K and links
+// per site) a cross-section is 8 rows, 369 KB, and a t-neighbour was read
+// 2.2 MB of data earlier, against 13 MB when each XCD swept whole t-slabs.
 constexpr int kLinkChunk = 64;
-constexpr int kLinkSection = 512;
-struct LinkSweep {
-  int zparts, zs;  // z-parts (8 or 1) and z-planes per part
-  int yb;          // rows of a cross-section (a divisor of Ly)
-  int cs, ncx;     // sites of a cross-section and its chunks
-};
-
-LinkSweep link_sweep(const Geom &g)
-{
-  LinkSweep w;
-  w.zparts = g.Lz % kNumXcd ? 1 : kNumXcd;
-  w.zs = g.Lz / w.zparts;
-  w.yb = 1;
-  for (int d = 1; d <= g.Ly; d++)
-    if (g.Ly % d == 0 && (long)g.Lx * d <= kLinkSection) w.yb = d;
-  w.cs = g.Lx * w.yb;
-  w.ncx = (w.cs + kLinkChunk - 1) / kLinkChunk;
-  return w;
-}
-
-long link_sweep_blocks(const Geom &g, const LinkSweep &w)
-{
-  return (long)w.zparts * 4 * w.ncx * w.zs * g.Lt * (g.Ly / w.yb);
-}
 
 // One thread per link (y, mu), its six squares one per loop iteration (no
 // loads hoisted across squares), then the hopping terms.  Output:
@@ -604,31 +574,12 @@ long link_sweep_blocks(const Geom &g, const LinkSweep &w)
 // the same layout as U.
 template <int HOP>
 __global__ __launch_bounds__(kLinkChunk) void clover_links_k(double *force, const double *K, const double *U, Geom g,
-                                                             LinkSweep w, double scale, int project, LinkHop hop)
+                                                             CrossSweep w, double scale, int project, LinkHop hop)
 {
   const long V = (long)g.vcb * 2;
-  long j = blockIdx.x;
-  int part = 0;
-  if (w.zparts == kNumXcd) {
-    part = (int)(j % kNumXcd);
-    j /= kNumXcd;
-  }
-  const int mu = (int)(j % 4);
-  j /= 4;
-  const int ch = (int)(j % w.ncx);
-  j /= w.ncx;
-  const int zl = (int)(j % w.zs);
-  j /= w.zs;
-  const int t = (int)(j % g.Lt);
-  const int yblk = (int)(j / g.Lt);
-  const int o = ch * kLinkChunk + (int)threadIdx.x;
-  if (o >= w.cs) return;  // the last chunk of a cross-section may be partial
-  int c[4];
-  c[0] = o % g.Lx;
-  c[1] = yblk * w.yb + o / g.Lx;
-  c[2] = part * w.zs + zl;
-  c[3] = t;
-  const long y = lex_of(g, c);
+  int mu, c[4];
+  const long y = cross_sweep_site(g, w, (int)threadIdx.x, mu, c);
+  if (y < 0) return;  // the last chunk of a cross-section may be partial
   M3 G;
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -673,8 +624,8 @@ template <int HOP>
 hipError_t launch_clover_links(const qphix_amd_ctx *ctx, double *force, const double *K, const double *U,
                                double scale, int project, const LinkHop &lh, hipStream_t st)
 {
-  const LinkSweep w = link_sweep(ctx->g);
-  const long nb = link_sweep_blocks(ctx->g, w);
+  const CrossSweep w = cross_sweep(ctx->g, kLinkChunk, 4);
+  const long nb = cross_sweep_blocks(ctx->g, w);
   if (nb > 0x7fffffffL) return hipErrorInvalidValue;
   hipLaunchKernelGGL((clover_links_k<HOP>), dim3((unsigned)nb), dim3(kLinkChunk), 0, st, force, K, U, ctx->g, w,
                      scale, project, lh);

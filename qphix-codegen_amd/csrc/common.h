@@ -343,4 +343,70 @@ inline int sweep_grid(const Geom &g, int sub, int bs)
   return (int)((long)kNumXcd * g.Lt * ((slab + bs - 1) / bs) * sub);
 }
 
+// Stencil kernels on the lexicographic site order visited in (x, y)
+// cross-sections: a work-group is `sub` work items (fastest, a direction for
+// example) of one chunk of bs sites of a cross-section of Lx * yb sites (yb
+// rows, at most kSectionSites sites).  With zparts = 8 (Lz a multiple of 8)
+// work-group b runs on XCD b % 8 (the dispatch order), which owns the
+// z-planes [k zs, (k + 1) zs); each XCD walks, for one y-block after the
+// other, its z-planes (fastest) and then t, so that a site's z- and
+// t-neighbours were read a few cross-sections earlier and are still in that
+// XCD's L2 (whole t-slabs per XCD put a t-neighbour a whole slab of
+// traffic back).
+constexpr int kSectionSites = 512;
+struct CrossSweep {
+  int zparts, zs;  // z-parts (8 or 1) and z-planes per part
+  int yb;          // rows of a cross-section (a divisor of Ly)
+  int cs, ncx;     // sites of a cross-section and its chunks of bs sites
+  int bs, sub;
+};
+
+inline CrossSweep cross_sweep(const Geom &g, int bs, int sub)
+{
+  CrossSweep w;
+  w.zparts = g.Lz % kNumXcd ? 1 : kNumXcd;
+  w.zs = g.Lz / w.zparts;
+  w.yb = 1;
+  for (int d = 1; d <= g.Ly; d++)
+    if (g.Ly % d == 0 && (long)g.Lx * d <= kSectionSites) w.yb = d;
+  w.cs = g.Lx * w.yb;
+  w.ncx = (w.cs + bs - 1) / bs;
+  w.bs = bs;
+  w.sub = sub;
+  return w;
+}
+
+inline long cross_sweep_blocks(const Geom &g, const CrossSweep &w)
+{
+  return (long)w.zparts * w.sub * w.ncx * w.zs * g.Lt * (g.Ly / w.yb);
+}
+
+// This thread's site (the local-th of its work-group's chunk): its
+// coordinates in c and its lexicographic index, or -1 past the end of a
+// partial chunk; item: the work item.
+__device__ __forceinline__ long cross_sweep_site(const Geom &g, const CrossSweep &w, int local, int &item, int c[4])
+{
+  long j = blockIdx.x;
+  int part = 0;
+  if (w.zparts == kNumXcd) {
+    part = (int)(j % kNumXcd);
+    j /= kNumXcd;
+  }
+  item = (int)(j % w.sub);
+  j /= w.sub;
+  const int ch = (int)(j % w.ncx);
+  j /= w.ncx;
+  const int zl = (int)(j % w.zs);
+  j /= w.zs;
+  const int t = (int)(j % g.Lt);
+  const int yblk = (int)(j / g.Lt);
+  const int o = ch * w.bs + local;
+  if (o >= w.cs) return -1;
+  c[0] = o % g.Lx;
+  c[1] = yblk * w.yb + o / g.Lx;
+  c[2] = part * w.zs + zl;
+  c[3] = t;
+  return (((long)c[3] * g.Lz + c[2]) * g.Ly + c[1]) * g.Lx + c[0];
+}
+
 }  // namespace qamd
