@@ -540,7 +540,7 @@ def main():
                          "48^3x96 block); tmclover48: the 48^3x96 twisted-mass clover block alone")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch clov_dslash, clov_dslash_achimbdpsi, Wilson dslash and clov_dslash "
-                         "on QPhiX arrays in place 10 times each plus a 1 GiB device copy (for "
+                         "on QPhiX arrays in place (soalen 16, 8, 4) 10 times each plus a 1 GiB device copy (for "
                          "rocprofv3 --pmc runs and their copy-kernel calibration), print nothing")
     ap.add_argument("--lattice", default=None,
                     help="with --kernels-only: Lx,Ly,Lz,Lt (default 32,32,32,64)")
@@ -609,13 +609,15 @@ def main():
         for i in range(10):
             s = sets[i & 1]
             wil.dslash(s["t0"], s["psi1"], s["g0"], 1, 0)
-        # clov_dslash on QPhiX arrays in place (soalen = veclen = 16)
-        lay = q.qphix_layout(q.F32, 16, 16, True)
-        psi_q = q.QPhiXSpinor(geom.spinor_to_qphix(sets[0]["psi1"], lay), lay)
-        res_q = q.QPhiXSpinor(torch.empty_like(psi_q.data), lay)
-        for i in range(10):
-            op.dslash(res_q, psi_q, sets[0]["g0"], sets[0]["inv0"], 1, 0)
-        del psi_q, res_q
+        # clov_dslash on QPhiX arrays in place (veclen 16, soalen 16, 8, 4:
+        # 10 launches each, in that order)
+        for S in (16, 8, 4):
+            lay = q.qphix_layout(q.F32, 16, S, True)
+            psi_q = q.QPhiXSpinor(geom.spinor_to_qphix(sets[0]["psi1"], lay), lay)
+            res_q = q.QPhiXSpinor(torch.empty_like(psi_q.data), lay)
+            for i in range(10):
+                op.dslash(res_q, psi_q, sets[0]["g0"], sets[0]["inv0"], 1, 0)
+            del psi_q, res_q
         del sets
         a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)

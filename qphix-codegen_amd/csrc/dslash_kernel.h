@@ -362,6 +362,148 @@ __device__ __forceinline__ void store_q_reals(S *__restrict__ base, int q, int s
       }
 }
 
+// ---- QPhiX arrays read and written as 16-byte vectors (SPL = 2) ----------
+// FP32 with soalen >= 4 (FP64: soalen >= 2): the four (two) lanes of a quad
+// (pair) hold consecutive sites of one block, whose element e sits in one
+// 16-byte vector at element-0 offset qf + e * soalen (qf: the quad's first
+// site).  Lane j of the group loads the vectors of elements e = j (mod 4)
+// (2 for FP64), so that it holds element e of all the group's sites, and a
+// transpose inside the group by DPP lane permutes gives every lane the
+// elements of its own site: 6 (12) vector loads instead of 24 scalar ones.
+// The host picks SPL = 2 only when every quad (pair) is one block's
+// consecutive sites (Lx/2 a multiple of 4 (2)) and the arrays are 16-byte
+// aligned; every lane of a quad then runs the same code.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int kDppSwapPairs = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kDppSwapHalves = 0x4E;  // quad_perm [2, 3, 0, 1]
+
+// 4 x 4 transpose across the lanes of a quad: lane j's a[k] becomes lane
+// k's a[j]
+__device__ __forceinline__ void quad_transpose(float a[4])
+{
+  const int j = (int)threadIdx.x & 3;
+  const bool odd = j & 1, hi = j & 2;
+  const float r0 = dpp_f<kDppSwapPairs>(odd ? a[0] : a[1]);
+  if (odd) a[0] = r0; else a[1] = r0;
+  const float r1 = dpp_f<kDppSwapPairs>(odd ? a[2] : a[3]);
+  if (odd) a[2] = r1; else a[3] = r1;
+  const float q0 = dpp_f<kDppSwapHalves>(hi ? a[0] : a[2]);
+  if (hi) a[0] = q0; else a[2] = q0;
+  const float q1 = dpp_f<kDppSwapHalves>(hi ? a[1] : a[3]);
+  if (hi) a[1] = q1; else a[3] = q1;
+}
+
+// 2 x 2 transpose across the lanes of a pair
+__device__ __forceinline__ void pair_transpose(double a[2])
+{
+  const bool odd = threadIdx.x & 1;
+  const double r = dpp_d<kDppSwapPairs>(odd ? a[0] : a[1]);
+  if (odd) a[0] = r; else a[1] = r;
+}
+
+// The site's 24 elements in QPhiX order E = (c * 4 + s) * 2 + r, from the
+// arrays at the group's first-site offset qf (NT: non-temporal loads)
+template <typename S, bool NT>
+__device__ __forceinline__ void load_qv_elems(const S *__restrict__ base, int qf, int sh, S out[24])
+{
+  if constexpr (std::is_same<S, float>::value) {
+    const int j = (int)threadIdx.x & 3;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const u32x4 *a = reinterpret_cast<const u32x4 *>(base + qf + ((4 * m + j) << sh));
+      const u32x4 v = (NT && QAMD_NT_STREAM) ? __builtin_nontemporal_load(a) : *a;
+      float t[4];
+      __builtin_memcpy(t, &v, 16);
+      quad_transpose(t);
+#pragma unroll
+      for (int k = 0; k < 4; k++) out[4 * m + k] = t[k];
+    }
+  } else {
+    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP32 or FP64");
+    const int j = (int)threadIdx.x & 1;
+#pragma unroll
+    for (int m = 0; m < 12; m++) {
+      const u32x4 *a = reinterpret_cast<const u32x4 *>(base + qf + ((2 * m + j) << sh));
+      const u32x4 v = (NT && QAMD_NT_STREAM) ? __builtin_nontemporal_load(a) : *a;
+      double t[2];
+      __builtin_memcpy(t, &v, 16);
+      pair_transpose(t);
+      out[2 * m] = t[0];
+      out[2 * m + 1] = t[1];
+    }
+  }
+}
+
+// element E of QPhiX order -> index in the (spin, colour, re/im) order
+__host__ __device__ constexpr int qphix_to_native(int E) { return (((E >> 1) & 3) * 3 + (E >> 3)) * 2 + (E & 1); }
+
+template <typename S, typename R>
+__device__ __forceinline__ void load_nbr_qv(const S *__restrict__ in, int qf, int sh, NbrSpinor<S, R> &n)
+{
+  S e[24];
+  load_qv_elems<S, false>(in, qf, sh, e);
+#pragma unroll
+  for (int E = 0; E < 24; E++) n.v[qphix_to_native(E)] = (R)e[E];
+}
+
+template <typename S, typename R>
+__device__ __forceinline__ void load_qv_reals(const S *__restrict__ base, int qf, int sh, R *out)
+{
+  S e[24];
+  load_qv_elems<S, true>(base, qf, sh, e);
+#pragma unroll
+  for (int E = 0; E < 24; E++) out[qphix_to_native(E)] = (R)e[E];
+}
+
+template <typename S, typename R>
+__device__ __forceinline__ void store_qv_reals(S *__restrict__ base, int qf, int sh, const R *in)
+{
+  if constexpr (std::is_same<S, float>::value) {
+    const int j = (int)threadIdx.x & 3;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      float t[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) t[k] = (float)in[qphix_to_native(4 * m + k)];
+      quad_transpose(t);
+      u32x4 *a = reinterpret_cast<u32x4 *>(base + qf + ((4 * m + j) << sh));
+      u32x4 v;
+      __builtin_memcpy(&v, t, 16);
+      if (QAMD_NT_STREAM)
+        __builtin_nontemporal_store(v, a);
+      else
+        *a = v;
+    }
+  } else {
+    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP32 or FP64");
+    const int j = (int)threadIdx.x & 1;
+#pragma unroll
+    for (int m = 0; m < 12; m++) {
+      double t[2] = {(double)in[qphix_to_native(2 * m)], (double)in[qphix_to_native(2 * m + 1)]};
+      pair_transpose(t);
+      u32x4 *a = reinterpret_cast<u32x4 *>(base + qf + ((2 * m + j) << sh));
+      u32x4 v;
+      __builtin_memcpy(&v, t, 16);
+      if (QAMD_NT_STREAM)
+        __builtin_nontemporal_store(v, a);
+      else
+        *a = v;
+    }
+  }
+}
+
 // (1 + SGN*gamma_MU) psi from raw FP16 components: every projected real is
 // a + s b = fma(b, s, a) with s = +-one, exact products and one rounding,
 // the same value as the FP32 add of the converted components.
@@ -420,7 +562,8 @@ __device__ __forceinline__ void hop_apply(cx<R> acc[4][3], const NbrSpinor<S, R>
 }
 
 // nb, nf: the neighbours' native site indices, or (SPL) their element-0
-// offsets in a QPhiX FourSpinorBlock array of soalen 1 << sh
+// offsets in a QPhiX FourSpinorBlock array of soalen 1 << sh (SPL = 2 and
+// MU > 0: those of the first site of the neighbours' quad or pair)
 template <typename S, int RECON, int MU, int ISIGN, int SPL = 0, typename R>
 __device__ __forceinline__ void hop_mu(cx<R> acc[4][3], const S *__restrict__ in,
                                        const S *__restrict__ gauge, int vcb, int site, int nb,
@@ -429,12 +572,16 @@ __device__ __forceinline__ void hop_mu(cx<R> acc[4][3], const S *__restrict__ in
   cx<R> ub[3][3], uf[3][3];
   load_link_pair<S, RECON, MU>(gauge, vcb, site, ub, uf, -R(0) * one);
   NbrSpinor<S, R> n;
-  if constexpr (SPL)
+  if constexpr (SPL == 2 && MU > 0)
+    load_nbr_qv(in, nb, sh, n);
+  else if constexpr (SPL)
     load_nbr_q(in, nb, sh, n);
   else
     load_nbr(in, vcb, nb, n);
   hop_apply<S, MU, 0, ISIGN>(acc, n, ub, bb, tbc, conj(phase), one);
-  if constexpr (SPL)
+  if constexpr (SPL == 2 && MU > 0)
+    load_nbr_qv(in, nf, sh, n);
+  else if constexpr (SPL)
     load_nbr_q(in, nf, sh, n);
   else
     load_nbr(in, vcb, nf, n);
@@ -566,6 +713,9 @@ __device__ __forceinline__ void twist_ord(const cx<R> in[4][3], cx<R> out[4][3],
 // (the links and clover blocks stay native), with the optional per-site
 // direction mask p.acc: a QPhiX host's arrays used in place, the reference's
 // body with its accumulate[] lane masks (codegen/dslash.cc:248,285-290).
+// SPL = 2 is the same with the spinors read and written as 16-byte vectors
+// (FP32 and FP64; the x-neighbours, not aligned to a lane group, element by
+// element).
 template <typename S, int RECON, int OP, bool ACHI, int ISIGN, int EPI = EPI_STORE, int SPL = 0>
 __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {
@@ -639,9 +789,13 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   R btb = c.t == 0 ? ((cm & 8) ? R(0) : p.coeff_tb * p.tbound) : p.coeff_tb;
   R btf = c.t == g.Lt - 1 ? ((cm & 8) ? R(0) : p.coeff_tf * p.tbound) : p.coeff_tf;
   // QPhiX arrays: element-0 offsets of this site and of its neighbours
-  int q0 = 0;
+  int q0 = 0, qf0 = 0;
+  // the group (quad, or pair for FP64) of lanes that SPL = 2 reads together
+  constexpr int QG = std::is_same<S, double>::value ? 2 : 4;
+  const int xg = c.xh & ~(QG - 1);
   if constexpr (SPL) {
     q0 = qphix_site(p, c.xh, c.y, c.z, c.t);
+    qf0 = qphix_site(p, xg, c.y, c.z, c.t);
     if (p.acc) {
       // the reference's accumulate[d]: a cleared bit drops direction d here
       // (a zero coefficient adds exact zeros, the neighbour being finite)
@@ -668,7 +822,9 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   auto start_acc = [&]() {
   if constexpr (ACHI) {
     R chr[24];
-    if constexpr (SPL)
+    if constexpr (SPL == 2)
+      load_qv_reals<S, R>(static_cast<const S *>(p.chi), qf0, p.q_sh, chr);
+    else if constexpr (SPL)
       load_q_reals<S, R, true>(static_cast<const S *>(p.chi), q0, p.q_sh, chr);
     else
       load_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<const S *>(p.chi), vcb, idx, chr);
@@ -721,7 +877,7 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const int tm = p.tbc_mask;
   const cx<R> ph0 = {p.tbc_re[0], p.tbc_im[0]}, ph1 = {p.tbc_re[1], p.tbc_im[1]};
   const cx<R> ph2 = {p.tbc_re[2], p.tbc_im[2]}, ph3 = {p.tbc_re[3], p.tbc_im[3]};
-  if constexpr (SPL) {
+  if constexpr (SPL == 1) {
     const int yb = c.y == 0 ? g.Ly - 1 : c.y - 1, yf = c.y == g.Ly - 1 ? 0 : c.y + 1;
     const int zb = c.z == 0 ? g.Lz - 1 : c.z - 1, zf = c.z == g.Lz - 1 ? 0 : c.z + 1;
     const int tb = c.t == 0 ? g.Lt - 1 : c.t - 1, tf = c.t == g.Lt - 1 ? 0 : c.t + 1;
@@ -743,7 +899,9 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   // work-groups) reads its neighbours from global memory.  The host checks
   // that every work-group is full (no lane leaves before the barrier, which
   // the branch below reaches uniformly per work-group) and that the rows fit
-  // the array.
+  // the array.  SPL = 2 stages the QPhiX array's sites through the lane
+  // groups' vector loads (whole groups: Lx/2 and the staged count are
+  // multiples of the group), in the native chunk order.
   int y0 = 0, r1 = 0;
   if (p.lds) {
     const unsigned base = (unsigned)(idx - (int)threadIdx.x);
@@ -763,9 +921,22 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       // staged row k is y0 + k - 1, in [-1, Ly] since k <= r1 + 2 and y0 + r1 < Ly
       int yk = y0 + k - 1;
       yk = yk < 0 ? yk + g.Ly : (yk >= g.Ly ? yk - g.Ly : yk);
-      const int src = plane + yk * g.Lxh + (j - k * g.Lxh);
+      if constexpr (SPL == 2) {
+        S e[24], nat[24];
+        load_qv_elems<S, false>(in, qphix_site(p, (j - k * g.Lxh) & ~(QG - 1), yk, c.z, c.t), p.q_sh, e);
 #pragma unroll
-      for (int ch = 0; ch < NC; ch++) lds[ch * kLdsSites + j] = vin[(unsigned)(ch * vcb + src)];
+        for (int E = 0; E < 24; E++) nat[qphix_to_native(E)] = e[E];
+#pragma unroll
+        for (int ch = 0; ch < NC; ch++) {
+          Vec16 v;
+          __builtin_memcpy(&v, nat + ch * (16 / (int)sizeof(S)), 16);
+          lds[ch * kLdsSites + j] = v;
+        }
+      } else {
+        const int src = plane + yk * g.Lxh + (j - k * g.Lxh);
+#pragma unroll
+        for (int ch = 0; ch < NC; ch++) lds[ch * kLdsSites + j] = vin[(unsigned)(ch * vcb + src)];
+      }
     }
     __syncthreads();
   }
@@ -773,6 +944,10 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
   const bool use_lds = false;
 #endif
   start_acc();
+  // SPL = 2: neighbours read from the QPhiX array by lane group (the first
+  // site of the group, xg, in the y, z and t directions)
+  const int sh = p.q_sh;
+  const int yb = c.y == 0 ? g.Ly - 1 : c.y - 1, yf = c.y == g.Ly - 1 ? 0 : c.y + 1;
 #if QAMD_LDS_XY
   if (use_lds) {
     const int rl = c.y - y0 + 1;  // staged row of this site
@@ -782,12 +957,26 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
                                    (rl + 1) * g.Lxh + c.xh, byb, byf, tm & 2, ph1, p.one);
   } else
 #endif
-  {
+  if constexpr (SPL == 2) {
+    hop_mu<S, RECON, 0, ISIGN, 2>(acc, in, gauge, vcb, idx, qphix_site(p, xb >> 1, c.y, c.z, c.t),
+                                  qphix_site(p, xf >> 1, c.y, c.z, c.t), bxb, bxf, tm & 1, ph0, p.one, sh);
+    hop_mu<S, RECON, 1, ISIGN, 2>(acc, in, gauge, vcb, idx, qphix_site(p, xg, yb, c.z, c.t),
+                                  qphix_site(p, xg, yf, c.z, c.t), byb, byf, tm & 2, ph1, p.one, sh);
+  } else {
     hop_mu<S, RECON, 0, ISIGN>(acc, in, gauge, vcb, idx, n_xb, n_xf, bxb, bxf, tm & 1, ph0, p.one);
     hop_mu<S, RECON, 1, ISIGN>(acc, in, gauge, vcb, idx, n_yb, n_yf, byb, byf, tm & 2, ph1, p.one);
   }
-  hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bzb, bzf, tm & 4, ph2, p.one);
-  hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3, p.one);
+  if constexpr (SPL == 2) {
+    const int zb = c.z == 0 ? g.Lz - 1 : c.z - 1, zf = c.z == g.Lz - 1 ? 0 : c.z + 1;
+    const int tb = c.t == 0 ? g.Lt - 1 : c.t - 1, tf = c.t == g.Lt - 1 ? 0 : c.t + 1;
+    hop_mu<S, RECON, 2, ISIGN, 2>(acc, in, gauge, vcb, idx, qphix_site(p, xg, c.y, zb, c.t),
+                                  qphix_site(p, xg, c.y, zf, c.t), bzb, bzf, tm & 4, ph2, p.one, sh);
+    hop_mu<S, RECON, 3, ISIGN, 2>(acc, in, gauge, vcb, idx, qphix_site(p, xg, c.y, c.z, tb),
+                                  qphix_site(p, xg, c.y, c.z, tf), btb, btf, tm & 8, ph3, p.one, sh);
+  } else {
+    hop_mu<S, RECON, 2, ISIGN>(acc, in, gauge, vcb, idx, n_zb, n_zf, bzb, bzf, tm & 4, ph2, p.one);
+    hop_mu<S, RECON, 3, ISIGN>(acc, in, gauge, vcb, idx, n_tb, n_tf, btb, btf, tm & 8, ph3, p.one);
+  }
   }
 
   cx<R> res[4][3];
@@ -827,7 +1016,9 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
 #pragma unroll
     for (int k = 0; k < 24; k++) o[k] = rv[k] - a * o[k];
   }
-  if constexpr (SPL)
+  if constexpr (SPL == 2)
+    store_qv_reals<S, R>(static_cast<S *>(p.out), qf0, p.q_sh, o);
+  else if constexpr (SPL)
     store_q_reals<S, R>(static_cast<S *>(p.out), q0, p.q_sh, o);
   else
     store_reals<S, spinor_w<S>(), 0, 24, R, true>(static_cast<S *>(p.out), vcb, idx, o);
@@ -862,8 +1053,8 @@ hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream
   const int grid = xcd_grid(p.nblocks);
   if constexpr (SPL) {  // QPhiX arrays: plain store only
     if (p.norm || p.step) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_STORE, 1>), dim3(grid), dim3(kDslashBlock),
-                       0, stream, p);
+    hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_STORE, SPL>), dim3(grid),
+                       dim3(kDslashBlock), 0, stream, p);
     return hipGetLastError();
   }
   if constexpr (ACHI && (OP == OP_CLOVER || OP == OP_TMCLOVER)) {
@@ -888,7 +1079,7 @@ hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream
 }
 
 // Run-time dispatch over (recon, op, achi, isign) for one storage type;
-// SPL = 1 for QPhiX spinor arrays.
+// SPL = 1 for QPhiX spinor arrays, 2 for the same read as vectors.
 template <typename S, int SPL = 0>
 hipError_t dispatch_dslash(int recon, int op, bool achi, int isign,
                            const DslashParams<typename Traits<S>::R> &p, hipStream_t st)

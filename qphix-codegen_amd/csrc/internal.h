@@ -33,7 +33,10 @@ struct DslashCall {
   const int *stop;
   // spl = 1: in, chi and out are QPhiX FourSpinorBlock arrays of the
   // geometry q (converted from a qphix_amd_qphix_layout); acc, when set, is
-  // the per-site direction mask in their site order (block * soalen + lane)
+  // the per-site direction mask in their site order (block * soalen + lane);
+  // spl = 2: the same read and written as 16-byte vectors (FP32 with soalen
+  // >= 4 and Lx/2 a multiple of 4, FP64 with soalen >= 2 and Lx/2 even;
+  // 16-byte aligned arrays)
   int spl;
   int q_sh, q_nvecs, q_pxy, q_pxyz;
   const unsigned char *acc;

@@ -261,3 +261,59 @@ def test_python_classes_take_qphix_arrays(qamd, fields):
     assert not torch.equal(full, direct)  # the mask drops hops
     with pytest.raises(qamd.QPhiXAMDError):
         qamd.Dslash(geom).dslash(out_q, psi_n, g, -1, cb)  # mixed native / QPhiX spinors
+
+
+@pytest.mark.parametrize("ft,V,S", [(1, 16, 16), (1, 16, 8), (1, 8, 4), (1, 4, 2), (0, 8, 8), (0, 8, 4), (0, 4, 2)],
+                         ids=["f32_s16", "f32_s8", "f32_s4", "f32_s2", "f64_s8", "f64_s4", "f64_s2"])
+def test_inplace_vector_and_element_access_agree(qamd, fields, ft, V, S):
+    """The in-place kernels read and write 16-byte vectors per lane group
+    where the layout and the arrays' alignment allow it (FP32 soalen >= 4,
+    FP64 soalen >= 2) and element by element otherwise.  The same arrays at
+    an offset of one element (aligned to their element type, not to 16 bytes)
+    take the element path: both give the same bits, with and without a mask,
+    and without a mask the native kernels' bits."""
+    L = qamd.lattice
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    U, psi, chi, clt = (torch.from_numpy(fields[k]).to(DEV) if k != "clt" else fields[k]
+                        for k in ("U", "psi", "chi", "clt"))
+    # padding in whole vectors of veclen / soalen blocks
+    pad = dict(pad_xy=V // S, pad_xyz=3 * (V // S))
+    lay = qamd.qphix_layout(ft, V, S, True, **pad)
+    # the mask holds one byte per slot of the padded array
+    lay_q = qb.Layout(ft, V, S, True, **pad)
+
+    def shifted(t):
+        """t's values in a buffer one element past a 16-byte boundary"""
+        buf = torch.full((t.numel() + 1,), float("nan"), dtype=t.dtype, device=DEV)
+        view = buf[1:].view(t.shape)
+        view.copy_(t)
+        assert view.data_ptr() % 16 != 0
+        return view
+
+    for kind, achi in ((1, True), (3, False), (2, True), (0, False)):
+        for isign, cb in ((1, 0), (-1, 1)):
+            print(f"case vector/element ft={ft} V={V} S={S} kind={kind} achi={achi} isign={isign}", flush=True)
+            clov = epilogue(L, clt, kind, achi, isign)
+            g = geom.pack_gauge(U, ft, 12, cb)
+            cl = None
+            if kind == 1:
+                cl = geom.pack_clover(torch.from_numpy(clov).to(DEV), ft, cb)
+            if kind == 3:
+                cl = geom.pack_full_clover(torch.from_numpy(clov).to(DEV), ft, cb)
+            psi_n, chi_n = geom.pack_spinor(psi, ft, 1 - cb), geom.pack_spinor(chi, ft, cb)
+            out_n = geom.new_spinor(ft, cb=cb)
+            call(qamd, geom, kind, achi, 12, isign, cb, out_n.data, psi_n.data, chi_n.data, g, cl)
+            psi_q, chi_q = geom.spinor_to_qphix(psi_n, lay), geom.spinor_to_qphix(chi_n, lay)
+            want = geom.spinor_to_qphix(out_n, lay, out=torch.full_like(psi_q, float("nan")))
+            mask = accumulate_mask(DIMS, cb, S, V, 5 * kind + cb + 1)
+            acc = torch.from_numpy(qphix_mask(lay_q, DIMS, cb, mask)).to(DEV)
+            assert acc.numel() == psi_q.numel() // 24
+            for a in (None, acc):
+                out_v = torch.full_like(psi_q, float("nan"))
+                call(qamd, geom, kind, achi, 12, isign, cb, out_v, psi_q, chi_q, g, cl, lay=lay, acc=a)
+                out_e = shifted(torch.full_like(psi_q, float("nan")))
+                call(qamd, geom, kind, achi, 12, isign, cb, out_e, shifted(psi_q), shifted(chi_q), g, cl, lay=lay,
+                     acc=a)
+                assert torch.equal(out_v.nan_to_num(7.0), out_e.nan_to_num(7.0)), (kind, achi, isign, a is None)
+                if a is None:
+                    assert torch.equal(out_v.nan_to_num(7.0), want.nan_to_num(7.0)), (kind, achi, isign)
