@@ -253,6 +253,52 @@ def inplace_kernels(q, geom, s, n):
     return out
 
 
+def inplace_other_precisions(q, geom, dev, n):
+    """clov_dslash on QPhiX arrays in place in FP64 (the reference's AVX-512
+    FP64 layouts, veclen 8 with soalen 8 and 4) and FP16 (its low-precision
+    build's, veclen 16 with soalen 16 and 8), each beside the native layout
+    in the same precision: HIP-event averages and fractions of 8 TB/s by the
+    native kernel's algorithmic bytes."""
+    import torch
+    L = q.lattice
+    dims = geom.dims
+    U = L.random_gauge(dims, 0.3, 1000, device=dev)
+    inv = L.invert_clover(L.random_clover(dims, 1003, device=dev))
+    psi = L.random_spinor(dims, 1001, device=dev)
+    op = q.ClovDslash(geom)
+    out = {}
+    for name, prec, sb, lays in (("f64", q.F64, 8, ((8, 8), (8, 4))), ("f16", q.F16, 2, ((16, 16), (16, 8)))):
+        g0, i0 = geom.pack_gauge(U, prec, 12, 0), geom.pack_clover(inv, prec, 0)
+        p1, t0 = geom.pack_spinor(psi, prec, 1), geom.new_spinor(prec, cb=0)
+        b = algorithmic_bytes(12, 72, False, sb) * geom.vcb
+
+        def kn(i):
+            op.dslash(t0, p1, g0, i0, 1, 0)
+
+        for i in range(3):
+            kn(i)
+        t = time_kernel(kn, n)
+        res = {"native": {"us": round(t * 1e6, 2), "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}}
+        for V, S in lays:
+            lay = q.qphix_layout(prec, V, S, True)
+            pq = q.QPhiXSpinor(geom.spinor_to_qphix(p1, lay), lay)
+            rq = q.QPhiXSpinor(torch.empty_like(pq.data), lay)
+
+            def kq(i):
+                op.dslash(rq, pq, g0, i0, 1, 0)
+
+            for i in range(3):
+                kq(i)
+            t = time_kernel(kq, n)
+            res[f"veclen{V}_soalen{S}"] = {"us": round(t * 1e6, 2), "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}
+            del pq, rq
+        out[name] = res
+        del g0, i0, p1, t0
+    del U, inv, psi
+    torch.cuda.empty_cache()
+    return out
+
+
 def _err(out, ref):
     d = (out - ref).abs().max().item()
     return {"max_abs": d, "max_abs_rel": d / ref.abs().max().item()}
@@ -644,6 +690,7 @@ def main():
     t_a = time_kernel(k_achi, args.steps)
     tm = tm_kernel(q, geom, sets[0], dev, args.steps)
     inplace = inplace_kernels(q, geom, sets[0], args.steps)
+    inplace_lp = inplace_other_precisions(q, geom, dev, args.steps)
     err = errors_vs_f64(q, geom, dev)
     # naive count (every neighbour spinor read from memory), as often quoted
     b_naive = (8 * 24 + 8 * recon + 72 + 24) * BYTES_F32 * vcb
@@ -680,6 +727,7 @@ def main():
             "parallelism": "single GPU",
             "tm_dslash": tm,
             "qphix_inplace_clov_dslash": inplace,
+            "qphix_inplace_clov_dslash_f64_f16": inplace_lp,
             "errors_vs_f64": err,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,

@@ -56,11 +56,13 @@ int run_qphix(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, const unsig
       (c.clov && !aligned16(c.clov)))
     return QPHIX_AMD_EALIGN;
   c.prec = lay->ft;
-  // whole 16-byte vectors per lane group where the layout and alignment
-  // allow them, element by element otherwise
-  const int group = lay->ft == QPHIX_AMD_F32 ? 4 : lay->ft == QPHIX_AMD_F64 ? 2 : 0;
-  const bool vec = group > 0 && (1 << q.sh) >= group && ctx->g.Lxh % group == 0 && aligned16(c.out) &&
-                   aligned16(c.in) && (!c.chi || aligned16(c.chi));
+  // vectors per lane group (16 bytes; 8 for FP16) where the layout and the
+  // alignment allow them, element by element otherwise
+  const int group = lay->ft == QPHIX_AMD_F64 ? 2 : 4;
+  const uintptr_t vm = (uintptr_t)(group * storage_bytes(lay->ft)) - 1;
+  const bool vec = (1 << q.sh) >= group && ctx->g.Lxh % group == 0 && !(reinterpret_cast<uintptr_t>(c.out) & vm) &&
+                   !(reinterpret_cast<uintptr_t>(c.in) & vm) &&
+                   (!c.chi || !(reinterpret_cast<uintptr_t>(c.chi) & vm));
   c.spl = vec ? 2 : 1;
   c.q_sh = q.sh;
   c.q_nvecs = q.nvecs;

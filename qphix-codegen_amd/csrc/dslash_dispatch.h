@@ -85,9 +85,7 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
     } else {
       // whole lane groups read as vectors (SPL = 2) stage the x/y rows like
       // the native layout; element by element (SPL = 1) they are not staged
-      if constexpr (std::is_same<S, float>::value || std::is_same<S, double>::value) {
-        if (c.spl == 2) return dispatch_dslash<S, 2>(c.recon, c.op, c.achi, c.isign, p, st);
-      }
+      if (c.spl == 2) return dispatch_dslash<S, 2>(c.recon, c.op, c.achi, c.isign, p, st);
       if (c.spl != 1) return hipErrorInvalidValue;
       p.lds = 0;
       return dispatch_dslash<S, 1>(c.recon, c.op, c.achi, c.isign, p, st);

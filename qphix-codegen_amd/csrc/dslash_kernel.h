@@ -362,11 +362,11 @@ __device__ __forceinline__ void store_q_reals(S *__restrict__ base, int q, int s
       }
 }
 
-// ---- QPhiX arrays read and written as 16-byte vectors (SPL = 2) ----------
-// FP32 with soalen >= 4 (FP64: soalen >= 2): the four (two) lanes of a quad
-// (pair) hold consecutive sites of one block, whose element e sits in one
-// 16-byte vector at element-0 offset qf + e * soalen (qf: the quad's first
-// site).  Lane j of the group loads the vectors of elements e = j (mod 4)
+// ---- QPhiX arrays read and written as vectors (SPL = 2) -------------------
+// FP32 or FP16 with soalen >= 4 (FP64: soalen >= 2): the four (two) lanes of
+// a quad (pair) hold consecutive sites of one block, whose element e sits in
+// one 16-byte (FP16: 8-byte) vector at element-0 offset qf + e * soalen (qf:
+// the quad's first site).  Lane j of the group loads the vectors of elements e = j (mod 4)
 // (2 for FP64), so that it holds element e of all the group's sites, and a
 // transpose inside the group by DPP lane permutes gives every lane the
 // elements of its own site: 6 (12) vector loads instead of 24 scalar ones.
@@ -405,6 +405,29 @@ __device__ __forceinline__ void quad_transpose(float a[4])
   if (hi) a[1] = q1; else a[3] = q1;
 }
 
+// 4 x 4 transpose of 16-bit entries across a quad: lane j's words
+// w[0] = [a_j0 | a_j1], w[1] = [a_j2 | a_j3] (low half first) become lane
+// k's [a_0k | a_1k], [a_2k | a_3k].  Within lane pairs, even lanes keep the
+// low halves and odd lanes the high halves of their own and their partner's
+// word, one v_perm_b32 with a per-lane byte select (bytes 0-3 of its second
+// operand p, 4-7 of its first w: even [w.0 w.1 p.0 p.1], odd
+// [p.2 p.3 w.2 w.3]); then the off-diagonal words are swapped between pairs.
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_u(unsigned v)
+{
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ void quad_transpose_h(unsigned w[2])
+{
+  const int j = (int)threadIdx.x & 3;
+  const bool odd = j & 1, hi = j & 2;
+  const unsigned sel = odd ? 0x07060302u : 0x01000504u;
+#pragma unroll
+  for (int k = 0; k < 2; k++) w[k] = __builtin_amdgcn_perm(w[k], dpp_u<kDppSwapPairs>(w[k]), sel);
+  const unsigned r = dpp_u<kDppSwapHalves>(hi ? w[0] : w[1]);
+  if (hi) w[0] = r; else w[1] = r;
+}
+
 // 2 x 2 transpose across the lanes of a pair
 __device__ __forceinline__ void pair_transpose(double a[2])
 {
@@ -430,8 +453,19 @@ __device__ __forceinline__ void load_qv_elems(const S *__restrict__ base, int qf
 #pragma unroll
       for (int k = 0; k < 4; k++) out[4 * m + k] = t[k];
     }
+  } else if constexpr (std::is_same<S, __half>::value) {
+    // 8-byte vectors: element e of the quad's four sites
+    const int j = (int)threadIdx.x & 3;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const u32x2 *a = reinterpret_cast<const u32x2 *>(base + qf + ((4 * m + j) << sh));
+      const u32x2 v = (NT && QAMD_NT_STREAM) ? __builtin_nontemporal_load(a) : *a;
+      unsigned w[2] = {v.x, v.y};
+      quad_transpose_h(w);
+      __builtin_memcpy(out + 4 * m, w, 8);
+    }
   } else {
-    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP32 or FP64");
+    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP64, FP32 or FP16");
     const int j = (int)threadIdx.x & 1;
 #pragma unroll
     for (int m = 0; m < 12; m++) {
@@ -455,7 +489,12 @@ __device__ __forceinline__ void load_nbr_qv(const S *__restrict__ in, int qf, in
   S e[24];
   load_qv_elems<S, false>(in, qf, sh, e);
 #pragma unroll
-  for (int E = 0; E < 24; E++) n.v[qphix_to_native(E)] = (R)e[E];
+  for (int E = 0; E < 24; E++) {
+    if constexpr (std::is_same<S, __half>::value)
+      n.v[qphix_to_native(E)] = e[E];  // raw halves, as load_nbr keeps them
+    else
+      n.v[qphix_to_native(E)] = (R)e[E];
+  }
 }
 
 template <typename S, typename R>
@@ -464,7 +503,7 @@ __device__ __forceinline__ void load_qv_reals(const S *__restrict__ base, int qf
   S e[24];
   load_qv_elems<S, true>(base, qf, sh, e);
 #pragma unroll
-  for (int E = 0; E < 24; E++) out[qphix_to_native(E)] = (R)e[E];
+  for (int E = 0; E < 24; E++) out[qphix_to_native(E)] = (R)to_real(e[E]);
 }
 
 template <typename S, typename R>
@@ -486,8 +525,27 @@ __device__ __forceinline__ void store_qv_reals(S *__restrict__ base, int qf, int
       else
         *a = v;
     }
+  } else if constexpr (std::is_same<S, __half>::value) {
+    const int j = (int)threadIdx.x & 3;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      __half h[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) h[k] = to_storage<__half>(in[qphix_to_native(4 * m + k)]);
+      unsigned w[2];
+      __builtin_memcpy(w, h, 8);
+      quad_transpose_h(w);
+      u32x2 *a = reinterpret_cast<u32x2 *>(base + qf + ((4 * m + j) << sh));
+      u32x2 v;
+      v.x = w[0];
+      v.y = w[1];
+      if (QAMD_NT_STREAM)
+        __builtin_nontemporal_store(v, a);
+      else
+        *a = v;
+    }
   } else {
-    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP32 or FP64");
+    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP64, FP32 or FP16");
     const int j = (int)threadIdx.x & 1;
 #pragma unroll
     for (int m = 0; m < 12; m++) {
@@ -713,9 +771,9 @@ __device__ __forceinline__ void twist_ord(const cx<R> in[4][3], cx<R> out[4][3],
 // (the links and clover blocks stay native), with the optional per-site
 // direction mask p.acc: a QPhiX host's arrays used in place, the reference's
 // body with its accumulate[] lane masks (codegen/dslash.cc:248,285-290).
-// SPL = 2 is the same with the spinors read and written as 16-byte vectors
-// (FP32 and FP64; the x-neighbours, not aligned to a lane group, element by
-// element).
+// SPL = 2 is the same with the spinors read and written as vectors by lane
+// group (FP64, FP32, FP16; the x/y rows staged in LDS from those vectors, or
+// else the x-neighbours, not aligned to a lane group, element by element).
 template <typename S, int RECON, int OP, bool ACHI, int ISIGN, int EPI = EPI_STORE, int SPL = 0>
 __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(DslashParams<typename Traits<S>::R> p)
 {

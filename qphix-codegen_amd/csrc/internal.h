@@ -34,9 +34,9 @@ struct DslashCall {
   // spl = 1: in, chi and out are QPhiX FourSpinorBlock arrays of the
   // geometry q (converted from a qphix_amd_qphix_layout); acc, when set, is
   // the per-site direction mask in their site order (block * soalen + lane);
-  // spl = 2: the same read and written as 16-byte vectors (FP32 with soalen
-  // >= 4 and Lx/2 a multiple of 4, FP64 with soalen >= 2 and Lx/2 even;
-  // 16-byte aligned arrays)
+  // spl = 2: the same read and written as vectors by lane group (FP32 and
+  // FP16 with soalen >= 4 and Lx/2 a multiple of 4, FP64 with soalen >= 2 and
+  // Lx/2 even; arrays aligned to the group's vector)
   int spl;
   int q_sh, q_nvecs, q_pxy, q_pxyz;
   const unsigned char *acc;

@@ -263,15 +263,17 @@ def test_python_classes_take_qphix_arrays(qamd, fields):
         qamd.Dslash(geom).dslash(out_q, psi_n, g, -1, cb)  # mixed native / QPhiX spinors
 
 
-@pytest.mark.parametrize("ft,V,S", [(1, 16, 16), (1, 16, 8), (1, 8, 4), (1, 4, 2), (0, 8, 8), (0, 8, 4), (0, 4, 2)],
-                         ids=["f32_s16", "f32_s8", "f32_s4", "f32_s2", "f64_s8", "f64_s4", "f64_s2"])
+@pytest.mark.parametrize("ft,V,S", [(1, 16, 16), (1, 16, 8), (1, 8, 4), (1, 4, 2), (0, 8, 8), (0, 8, 4), (0, 4, 2),
+                                    (2, 16, 16), (2, 16, 8), (2, 8, 4), (2, 4, 2)],
+                         ids=["f32_s16", "f32_s8", "f32_s4", "f32_s2", "f64_s8", "f64_s4", "f64_s2",
+                              "f16_s16", "f16_s8", "f16_s4", "f16_s2"])
 def test_inplace_vector_and_element_access_agree(qamd, fields, ft, V, S):
-    """The in-place kernels read and write 16-byte vectors per lane group
-    where the layout and the arrays' alignment allow it (FP32 soalen >= 4,
+    """The in-place kernels read and write vectors per lane group where the
+    layout and the arrays' alignment allow it (FP32 and FP16 soalen >= 4,
     FP64 soalen >= 2) and element by element otherwise.  The same arrays at
-    an offset of one element (aligned to their element type, not to 16 bytes)
-    take the element path: both give the same bits, with and without a mask,
-    and without a mask the native kernels' bits."""
+    an offset of one element (aligned to their element type, not to the
+    vector) take the element path: both give the same bits, with and without
+    a mask, and without a mask the native kernels' bits."""
     L = qamd.lattice
     geom = qamd.Geometry(DIMS, t_boundary=-1.0)
     U, psi, chi, clt = (torch.from_numpy(fields[k]).to(DEV) if k != "clt" else fields[k]
@@ -287,7 +289,7 @@ def test_inplace_vector_and_element_access_agree(qamd, fields, ft, V, S):
         buf = torch.full((t.numel() + 1,), float("nan"), dtype=t.dtype, device=DEV)
         view = buf[1:].view(t.shape)
         view.copy_(t)
-        assert view.data_ptr() % 16 != 0
+        assert view.data_ptr() % (8 if ft == 2 else 16) != 0  # not aligned to the lane group's vector
         return view
 
     for kind, achi in ((1, True), (3, False), (2, True), (0, False)):
