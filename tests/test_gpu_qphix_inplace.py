@@ -319,3 +319,54 @@ def test_inplace_vector_and_element_access_agree(qamd, fields, ft, V, S):
                 assert torch.equal(out_v.nan_to_num(7.0), out_e.nan_to_num(7.0)), (kind, achi, isign, a is None)
                 if a is None:
                     assert torch.equal(out_v.nan_to_num(7.0), want.nan_to_num(7.0)), (kind, achi, isign)
+
+
+@pytest.mark.parametrize("ft,V,S", [(1, 8, 8), (1, 16, 4), (0, 8, 4), (2, 16, 8)],
+                         ids=["f32_s8", "f32_s4", "f64_s4", "f16_s8"])
+def test_inplace_vector_form_without_lds_rows(qamd, ft, V, S):
+    """Lx/2 = 24 (the 48^3 lattices): a work-group's rows plus their halo do
+    not fit the LDS staging array, so the vector form reads the x-neighbours
+    element by element and the y/z/t neighbours by quad.  Same bits as the
+    element form and, without a mask, as the native kernels."""
+    dims = (48, 4, 4, 2)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    U = L.random_gauge(dims, 0.3, 71).to(DEV)
+    psi, chi = L.random_spinor(dims, 72).to(DEV), L.random_spinor(dims, 73).to(DEV)
+    clt = L.random_clover(dims, 74)
+    lay = qamd.qphix_layout(ft, V, S, True)
+    lay_q = qb.Layout(ft, V, S)
+
+    def shifted(t):
+        buf = torch.full((t.numel() + 1,), float("nan"), dtype=t.dtype, device=DEV)
+        view = buf[1:].view(t.shape)
+        view.copy_(t)
+        return view
+
+    for kind, achi in ((1, True), (3, False), (0, True)):
+        for isign, cb in ((1, 1), (-1, 0)):
+            print(f"case no-LDS ft={ft} V={V} S={S} kind={kind} achi={achi} isign={isign}", flush=True)
+            clov = epilogue(L, clt, kind, achi, isign)
+            g = geom.pack_gauge(U, ft, 12, cb)
+            cl = None
+            if kind == 1:
+                cl = geom.pack_clover(torch.from_numpy(clov).to(DEV), ft, cb)
+            if kind == 3:
+                cl = geom.pack_full_clover(torch.from_numpy(clov).to(DEV), ft, cb)
+            psi_n, chi_n = geom.pack_spinor(psi, ft, 1 - cb), geom.pack_spinor(chi, ft, cb)
+            out_n = geom.new_spinor(ft, cb=cb)
+            call(qamd, geom, kind, achi, 12, isign, cb, out_n.data, psi_n.data, chi_n.data, g, cl)
+            psi_q, chi_q = geom.spinor_to_qphix(psi_n, lay), geom.spinor_to_qphix(chi_n, lay)
+            want = geom.spinor_to_qphix(out_n, lay)
+            mask = accumulate_mask(dims, cb, S, V, 3 * kind + cb)
+            acc = torch.from_numpy(qphix_mask(lay_q, dims, cb, mask)).to(DEV)
+            assert acc.numel() == psi_q.numel() // 24
+            for a in (None, acc):
+                out_v = torch.zeros_like(psi_q)
+                call(qamd, geom, kind, achi, 12, isign, cb, out_v, psi_q, chi_q, g, cl, lay=lay, acc=a)
+                out_e = shifted(torch.zeros_like(psi_q))
+                call(qamd, geom, kind, achi, 12, isign, cb, out_e, shifted(psi_q), shifted(chi_q), g, cl, lay=lay,
+                     acc=a)
+                assert torch.equal(out_v, out_e), (kind, achi, isign, a is None)
+                if a is None:
+                    assert torch.equal(out_v, want), (kind, achi, isign)
