@@ -226,29 +226,33 @@ def tm_kernel(q, geom, s, dev, n):
             "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4), "flop_per_site": 1392}
 
 
-def inplace_kernels(q, geom, s, n):
+def inplace_kernels(q, geom, sets, n):
     """clov_dslash FP32, 12-real links, reading and writing QPhiX
     FourSpinorBlock arrays in place (qphix_amd_qphix_clov_dslash; the links
     and clover blocks are the native launch's) for the reference's AVX-512
     FP32 layouts, veclen 16 with soalen 16, 8 and 4: HIP-event average and
-    fraction of 8 TB/s by the same algorithmic bytes as the native kernel."""
+    fraction of 8 TB/s by the same algorithmic bytes as the native kernel.
+    The two input sets alternate, as for the native kernel."""
     import torch
     out = {}
     op = q.ClovDslash(geom)
     b = algorithmic_bytes(12, 72, False, BYTES_F32) * geom.vcb
     for S in (16, 8, 4):
         lay = q.qphix_layout(q.F32, 16, S, True)
-        psi_q = q.QPhiXSpinor(geom.spinor_to_qphix(s["psi1"], lay), lay)
-        res_q = q.QPhiXSpinor(torch.empty_like(psi_q.data), lay)
+        qs = []
+        for s in sets:
+            psi_q = q.QPhiXSpinor(geom.spinor_to_qphix(s["psi1"], lay), lay)
+            qs.append((psi_q, q.QPhiXSpinor(torch.empty_like(psi_q.data), lay), s))
 
         def k(i):
+            psi_q, res_q, s = qs[i & 1]
             op.dslash(res_q, psi_q, s["g0"], s["inv0"], 1, 0)
 
         for i in range(3):
             k(i)
         t = time_kernel(k, n)
         out[f"soalen{S}"] = {"us": round(t * 1e6, 2), "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}
-        del psi_q, res_q
+        del qs
     torch.cuda.empty_cache()
     return out
 
@@ -257,23 +261,26 @@ def inplace_other_precisions(q, geom, dev, n):
     """clov_dslash on QPhiX arrays in place in FP64 (the reference's AVX-512
     FP64 layouts, veclen 8 with soalen 8 and 4) and FP16 (its low-precision
     build's, veclen 16 with soalen 16 and 8), each beside the native layout
-    in the same precision: HIP-event averages and fractions of 8 TB/s by the
-    native kernel's algorithmic bytes."""
+    in the same precision, two input sets alternating: HIP-event averages
+    and fractions of 8 TB/s by the native kernel's algorithmic bytes."""
     import torch
     L = q.lattice
     dims = geom.dims
-    U = L.random_gauge(dims, 0.3, 1000, device=dev)
-    inv = L.invert_clover(L.random_clover(dims, 1003, device=dev))
-    psi = L.random_spinor(dims, 1001, device=dev)
+    lex = []
+    for k in range(2):
+        U = L.random_gauge(dims, 0.3, 1000 + 10 * k, device=dev)
+        inv = L.invert_clover(L.random_clover(dims, 1003 + 10 * k, device=dev))
+        lex.append((U, inv, L.random_spinor(dims, 1001 + 10 * k, device=dev)))
     op = q.ClovDslash(geom)
     out = {}
     for name, prec, sb, lays in (("f64", q.F64, 8, ((8, 8), (8, 4))), ("f16", q.F16, 2, ((16, 16), (16, 8)))):
-        g0, i0 = geom.pack_gauge(U, prec, 12, 0), geom.pack_clover(inv, prec, 0)
-        p1, t0 = geom.pack_spinor(psi, prec, 1), geom.new_spinor(prec, cb=0)
+        fs = [dict(g0=geom.pack_gauge(U, prec, 12, 0), i0=geom.pack_clover(inv, prec, 0),
+                   p1=geom.pack_spinor(psi, prec, 1), t0=geom.new_spinor(prec, cb=0)) for U, inv, psi in lex]
         b = algorithmic_bytes(12, 72, False, sb) * geom.vcb
 
         def kn(i):
-            op.dslash(t0, p1, g0, i0, 1, 0)
+            f = fs[i & 1]
+            op.dslash(f["t0"], f["p1"], f["g0"], f["i0"], 1, 0)
 
         for i in range(3):
             kn(i)
@@ -281,20 +288,23 @@ def inplace_other_precisions(q, geom, dev, n):
         res = {"native": {"us": round(t * 1e6, 2), "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}}
         for V, S in lays:
             lay = q.qphix_layout(prec, V, S, True)
-            pq = q.QPhiXSpinor(geom.spinor_to_qphix(p1, lay), lay)
-            rq = q.QPhiXSpinor(torch.empty_like(pq.data), lay)
+            for f in fs:
+                f["pq"] = q.QPhiXSpinor(geom.spinor_to_qphix(f["p1"], lay), lay)
+                f["rq"] = q.QPhiXSpinor(torch.empty_like(f["pq"].data), lay)
 
             def kq(i):
-                op.dslash(rq, pq, g0, i0, 1, 0)
+                f = fs[i & 1]
+                op.dslash(f["rq"], f["pq"], f["g0"], f["i0"], 1, 0)
 
             for i in range(3):
                 kq(i)
             t = time_kernel(kq, n)
             res[f"veclen{V}_soalen{S}"] = {"us": round(t * 1e6, 2), "hbm_frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}
-            del pq, rq
+            for f in fs:
+                del f["pq"], f["rq"]
         out[name] = res
-        del g0, i0, p1, t0
-    del U, inv, psi
+        del fs
+    del lex
     torch.cuda.empty_cache()
     return out
 
@@ -689,7 +699,7 @@ def main():
     t_d = time_kernel(k_dslash, args.steps)
     t_a = time_kernel(k_achi, args.steps)
     tm = tm_kernel(q, geom, sets[0], dev, args.steps)
-    inplace = inplace_kernels(q, geom, sets[0], args.steps)
+    inplace = inplace_kernels(q, geom, sets, args.steps)
     inplace_lp = inplace_other_precisions(q, geom, dev, args.steps)
     err = errors_vs_f64(q, geom, dev)
     # naive count (every neighbour spinor read from memory), as often quoted

@@ -194,10 +194,11 @@ int qphix_amd_full_clover_from_qphix(qphix_amd_ctx *ctx,
    links and clover blocks are native fields in the same precision
    (converted once, e.g. with qphix_amd_gauge_from_qphix), `recon` as
    there.  Same semantics, arguments and errors as the native entries of
-   the same family; QPhiX arrays need FT alignment only (16-byte aligned
-   FP32 arrays with soalen >= 4 and FP64 arrays, with Lx/2 a multiple of 4
-   or 2, are read and written as 16-byte vectors, the others element by
-   element; both give the same bits).
+   the same family; QPhiX arrays need FT alignment only (FP32 and FP16
+   arrays with soalen >= 4 and Lx/2 a multiple of 4, aligned to 16 and 8
+   bytes, are read and written as vectors by lane quad, and 16-byte aligned
+   FP64 arrays with Lx/2 even have their x/y rows staged from such vectors;
+   the others are read element by element; every form gives the same bits).
    accumulate (device, optional): one byte per site of the output array, in
    its site order (block * soalen + lane, padding slots included and
    ignored); bit d = 2 mu + fb set accumulates direction d at that site,

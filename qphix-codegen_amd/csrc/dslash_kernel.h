@@ -363,10 +363,12 @@ __device__ __forceinline__ void store_q_reals(S *__restrict__ base, int q, int s
 }
 
 // ---- QPhiX arrays read and written as vectors (SPL = 2) -------------------
-// FP32 or FP16 with soalen >= 4 (FP64: soalen >= 2): the four (two) lanes of
-// a quad (pair) hold consecutive sites of one block, whose element e sits in
-// one 16-byte (FP16: 8-byte) vector at element-0 offset qf + e * soalen (qf:
-// the quad's first site).  Lane j of the group loads the vectors of elements e = j (mod 4)
+// FP32 or FP16 with soalen >= 4: the four lanes of a quad hold consecutive
+// sites of one block, whose element e sits in one 16-byte (FP16: 8-byte)
+// vector at element-0 offset qf + e * soalen (qf: the quad's first site).
+// (FP64 reads its spinors element by element, 8 bytes per lane, and uses
+// 16-byte vectors of lane pairs only to stage the x/y rows in LDS: the
+// 64-bit transposes cost more than they save at one wave per SIMD.)  Lane j of the group loads the vectors of elements e = j (mod 4)
 // (2 for FP64), so that it holds element e of all the group's sites, and a
 // transpose inside the group by DPP lane permutes gives every lane the
 // elements of its own site: 6 (12) vector loads instead of 24 scalar ones.
@@ -377,14 +379,6 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
 {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v)
-{
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 constexpr int kDppSwapPairs = 0xB1;  // quad_perm [1, 0, 3, 2]
 constexpr int kDppSwapHalves = 0x4E;  // quad_perm [2, 3, 0, 1]
@@ -428,14 +422,6 @@ __device__ __forceinline__ void quad_transpose_h(unsigned w[2])
   if (hi) w[0] = r; else w[1] = r;
 }
 
-// 2 x 2 transpose across the lanes of a pair
-__device__ __forceinline__ void pair_transpose(double a[2])
-{
-  const bool odd = threadIdx.x & 1;
-  const double r = dpp_d<kDppSwapPairs>(odd ? a[0] : a[1]);
-  if (odd) a[0] = r; else a[1] = r;
-}
-
 // The site's 24 elements in QPhiX order E = (c * 4 + s) * 2 + r, from the
 // arrays at the group's first-site offset qf (NT: non-temporal loads)
 template <typename S, bool NT>
@@ -465,18 +451,7 @@ __device__ __forceinline__ void load_qv_elems(const S *__restrict__ base, int qf
       __builtin_memcpy(out + 4 * m, w, 8);
     }
   } else {
-    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP64, FP32 or FP16");
-    const int j = (int)threadIdx.x & 1;
-#pragma unroll
-    for (int m = 0; m < 12; m++) {
-      const u32x4 *a = reinterpret_cast<const u32x4 *>(base + qf + ((2 * m + j) << sh));
-      const u32x4 v = (NT && QAMD_NT_STREAM) ? __builtin_nontemporal_load(a) : *a;
-      double t[2];
-      __builtin_memcpy(t, &v, 16);
-      pair_transpose(t);
-      out[2 * m] = t[0];
-      out[2 * m + 1] = t[1];
-    }
+    static_assert(std::is_same<S, __half>::value, "vector QPhiX access: FP32 or FP16");
   }
 }
 
@@ -486,30 +461,40 @@ __host__ __device__ constexpr int qphix_to_native(int E) { return (((E >> 1) & 3
 template <typename S, typename R>
 __device__ __forceinline__ void load_nbr_qv(const S *__restrict__ in, int qf, int sh, NbrSpinor<S, R> &n)
 {
-  S e[24];
-  load_qv_elems<S, false>(in, qf, sh, e);
+  if constexpr (std::is_same<S, double>::value) {  // FP64: element by element
+    load_nbr_q(in, qf + ((int)threadIdx.x & 1), sh, n);
+  } else {
+    S e[24];
+    load_qv_elems<S, false>(in, qf, sh, e);
 #pragma unroll
-  for (int E = 0; E < 24; E++) {
-    if constexpr (std::is_same<S, __half>::value)
-      n.v[qphix_to_native(E)] = e[E];  // raw halves, as load_nbr keeps them
-    else
-      n.v[qphix_to_native(E)] = (R)e[E];
+    for (int E = 0; E < 24; E++) {
+      if constexpr (std::is_same<S, __half>::value)
+        n.v[qphix_to_native(E)] = e[E];  // raw halves, as load_nbr keeps them
+      else
+        n.v[qphix_to_native(E)] = (R)e[E];
+    }
   }
 }
 
 template <typename S, typename R>
 __device__ __forceinline__ void load_qv_reals(const S *__restrict__ base, int qf, int sh, R *out)
 {
-  S e[24];
-  load_qv_elems<S, true>(base, qf, sh, e);
+  if constexpr (std::is_same<S, double>::value) {  // FP64: element by element
+    load_q_reals<S, R, true>(base, qf + ((int)threadIdx.x & 1), sh, out);
+  } else {
+    S e[24];
+    load_qv_elems<S, true>(base, qf, sh, e);
 #pragma unroll
-  for (int E = 0; E < 24; E++) out[qphix_to_native(E)] = (R)to_real(e[E]);
+    for (int E = 0; E < 24; E++) out[qphix_to_native(E)] = (R)to_real(e[E]);
+  }
 }
 
 template <typename S, typename R>
 __device__ __forceinline__ void store_qv_reals(S *__restrict__ base, int qf, int sh, const R *in)
 {
-  if constexpr (std::is_same<S, float>::value) {
+  if constexpr (std::is_same<S, double>::value) {  // FP64: element by element
+    store_q_reals<S, R>(base, qf + ((int)threadIdx.x & 1), sh, in);
+  } else if constexpr (std::is_same<S, float>::value) {
     const int j = (int)threadIdx.x & 3;
 #pragma unroll
     for (int m = 0; m < 6; m++) {
@@ -545,20 +530,7 @@ __device__ __forceinline__ void store_qv_reals(S *__restrict__ base, int qf, int
         *a = v;
     }
   } else {
-    static_assert(std::is_same<S, double>::value, "vector QPhiX access: FP64, FP32 or FP16");
-    const int j = (int)threadIdx.x & 1;
-#pragma unroll
-    for (int m = 0; m < 12; m++) {
-      double t[2] = {(double)in[qphix_to_native(2 * m)], (double)in[qphix_to_native(2 * m + 1)]};
-      pair_transpose(t);
-      u32x4 *a = reinterpret_cast<u32x4 *>(base + qf + ((2 * m + j) << sh));
-      u32x4 v;
-      __builtin_memcpy(&v, t, 16);
-      if (QAMD_NT_STREAM)
-        __builtin_nontemporal_store(v, a);
-      else
-        *a = v;
-    }
+    static_assert(std::is_same<S, __half>::value, "vector QPhiX access: FP32 or FP16");
   }
 }
 
@@ -979,7 +951,23 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       // staged row k is y0 + k - 1, in [-1, Ly] since k <= r1 + 2 and y0 + r1 < Ly
       int yk = y0 + k - 1;
       yk = yk < 0 ? yk + g.Ly : (yk >= g.Ly ? yk - g.Ly : yk);
-      if constexpr (SPL == 2) {
+      if constexpr (SPL == 2 && std::is_same<S, double>::value) {
+        // each lane's vector holds one real part of one chunk of the pair's
+        // two sites: written to LDS as it is, no transpose
+        const int jb = j & 1, j0 = j - jb;
+        const int qf = qphix_site(p, (j - k * g.Lxh) & ~1, yk, c.z, c.t);
+        double *ld = reinterpret_cast<double *>(lds);
+#pragma unroll
+        for (int m = 0; m < 12; m++) {
+          const u32x4 *a = reinterpret_cast<const u32x4 *>(static_cast<const double *>(in) + qf + ((2 * m + jb) << p.q_sh));
+          const u32x4 v = *a;
+          double t[2];
+          __builtin_memcpy(t, &v, 16);
+          const int ch = qphix_to_native(2 * m) >> 1;  // the chunk of elements 2m, 2m + 1
+          ld[(ch * kLdsSites + j0) * 2 + jb] = t[0];
+          ld[(ch * kLdsSites + j0 + 1) * 2 + jb] = t[1];
+        }
+      } else if constexpr (SPL == 2) {
         S e[24], nat[24];
         load_qv_elems<S, false>(in, qphix_site(p, (j - k * g.Lxh) & ~(QG - 1), yk, c.z, c.t), p.q_sh, e);
 #pragma unroll
