@@ -442,6 +442,44 @@ int qphix_amd_tm_clov_face_finish(qphix_amd_ctx *ctx, int prec, int recon,
                                   void *out, const void *buf,
                                   const void *gauge, const void *fclover,
                                   double beta, void *stream);
+/* The same on QPhiX FourSpinorBlock arrays in place (FT = lay->ft, the
+   layouts of the qphix_amd_qphix_* operators): face_pack reads `in`, the
+   finishes read and write `out` where a QPhiX host keeps them, so a host
+   that splits its lattice completes the in-place operators without a
+   conversion pass.  Buffers, links and clover blocks as above; the arrays
+   need FT alignment only.  These are the reference's face templates on
+   its own arrays (face_proj_dir_{plus,minus}, face_finish_dir_*,
+   face_clov_finish_dir_*, tm_face_finish_dir_*, tm_clov_face_finish_dir_*:
+   codegen/jinja/{,clov_,tm_,tm_clov_}dslash_decl.h.j2, codegen/dslash.cc:
+   433-553), one call per face instead of one per block.                 */
+int qphix_amd_qphix_face_pack(qphix_amd_ctx *ctx,
+                              const qphix_amd_qphix_layout *lay, int isign,
+                              int cb, int dim, int fb, void *buf,
+                              const void *in, void *stream);
+int qphix_amd_qphix_face_finish(qphix_amd_ctx *ctx,
+                                const qphix_amd_qphix_layout *lay, int recon,
+                                int isign, int cb, int dim, int fb, void *out,
+                                const void *buf, const void *gauge,
+                                double beta, void *stream);
+int qphix_amd_qphix_clov_face_finish(qphix_amd_ctx *ctx,
+                                     const qphix_amd_qphix_layout *lay,
+                                     int recon, int isign, int cb, int dim,
+                                     int fb, void *out, const void *buf,
+                                     const void *gauge, const void *invclover,
+                                     double beta, void *stream);
+int qphix_amd_qphix_tm_face_finish(qphix_amd_ctx *ctx,
+                                   const qphix_amd_qphix_layout *lay,
+                                   int recon, int isign, int cb, int dim,
+                                   int fb, void *out, const void *buf,
+                                   const void *gauge, double beta, double mu,
+                                   double muinv, void *stream);
+int qphix_amd_qphix_tm_clov_face_finish(qphix_amd_ctx *ctx,
+                                        const qphix_amd_qphix_layout *lay,
+                                        int recon, int isign, int cb, int dim,
+                                        int fb, void *out, const void *buf,
+                                        const void *gauge,
+                                        const void *fclover, double beta,
+                                        void *stream);
 
 /* ---- differentiation (no counterpart in the reference) ------------------
    For L = Re<chi, D psi>, D = the hop above with output checkerboard cb:

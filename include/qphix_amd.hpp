@@ -180,6 +180,13 @@ class Geometry {
   {
     check(qphix_amd_face_pack(ctx_, prec, isign, cb, dim, fb, buf, in, stream), "qphix_amd_face_pack");
   }
+  /* the same from a QPhiX FourSpinorBlock array `in` of layout lay, read in
+     place */
+  void face_pack(const qphix_amd_qphix_layout &lay, int isign, int cb, int dim, int fb, void *buf, const void *in,
+                 void *stream = nullptr) const
+  {
+    check(qphix_amd_qphix_face_pack(ctx_, &lay, isign, cb, dim, fb, buf, in, stream), "qphix_amd_qphix_face_pack");
+  }
 
   /* differentiation of L = Re<chi, D psi> (no QPhiX counterpart): the
      adjoint D^dagger chi on checkerboard 1-cb, and the link gradient
@@ -232,17 +239,45 @@ class Dslash {
   {
   }
 
+  /* Operate on a QPhiX host's FourSpinorBlock arrays in place: after this
+     call res, psi and chi of every method (and res of the face finishes)
+     are arrays of the layout lay (FT = the class's precision), with the
+     optional per-site accumulate mask (one byte per slot of the padded
+     array); links and clover blocks stay native fields.
+     use_native_spinors() switches back. */
+  void use_qphix_arrays(const qphix_amd_qphix_layout &lay, const unsigned char *accumulate = nullptr)
+  {
+    if (lay.ft != prec_) throw Error(QPHIX_AMD_EINVAL, "use_qphix_arrays: layout FT is not the class's precision");
+    lay_ = lay;
+    q_ = true;
+    acc_ = accumulate;
+  }
+  void use_native_spinors()
+  {
+    q_ = false;
+    acc_ = nullptr;
+  }
+
   void dslash(void *res, const void *psi, const void *u, int isign, int cb) const
   {
-    check(qphix_amd_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, stream_),
-          "qphix_amd_dslash");
+    if (q_)
+      check(qphix_amd_qphix_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, acc_, cs_, ct_, ct_, stream_),
+            "qphix_amd_qphix_dslash");
+    else
+      check(qphix_amd_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, stream_),
+            "qphix_amd_dslash");
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, double alpha,
                             double beta, int isign, int cb) const
   {
-    check(qphix_amd_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha, beta * cs_,
-                                      beta * ct_, beta * ct_, stream_),
-          "qphix_amd_dslash_achimbdpsi");
+    if (q_)
+      check(qphix_amd_qphix_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, acc_, alpha,
+                                              beta * cs_, beta * ct_, beta * ct_, stream_),
+            "qphix_amd_qphix_dslash_achimbdpsi");
+    else
+      check(qphix_amd_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha, beta * cs_,
+                                        beta * ct_, beta * ct_, stream_),
+            "qphix_amd_dslash_achimbdpsi");
   }
   /* complete the sites of face (dim, fb) from a received face buffer:
      after dslash (beta = -coeff; each family has its own), and after
@@ -255,8 +290,12 @@ class Dslash {
   void face_finish_achimbdpsi(void *res, const void *buf, const void *u, double beta, int isign, int cb, int dim,
                               int fb) const
   {
-    check(qphix_amd_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_),
-          "qphix_amd_face_finish");
+    if (q_)
+      check(qphix_amd_qphix_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_),
+            "qphix_amd_qphix_face_finish");
+    else
+      check(qphix_amd_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_),
+            "qphix_amd_face_finish");
   }
 
  protected:
@@ -264,6 +303,9 @@ class Dslash {
   int prec_, recon_;
   double cs_, ct_;
   void *stream_;
+  bool q_ = false;  // spinors are QPhiX arrays of lay_
+  qphix_amd_qphix_layout lay_{};
+  const unsigned char *acc_ = nullptr;
 };
 
 /* Wilson-clover (QPhiX ClovDslash):
@@ -275,23 +317,38 @@ class ClovDslash : public Dslash {
 
   void dslash(void *res, const void *psi, const void *u, const void *invclov, int isign, int cb) const
   {
-    check(qphix_amd_clov_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, invclov, cs_, ct_, ct_, stream_),
-          "qphix_amd_clov_dslash");
+    if (q_)
+      check(qphix_amd_qphix_clov_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, invclov, acc_, cs_, ct_, ct_,
+                                        stream_),
+            "qphix_amd_qphix_clov_dslash");
+    else
+      check(qphix_amd_clov_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, invclov, cs_, ct_, ct_, stream_),
+            "qphix_amd_clov_dslash");
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, const void *clov,
                             double beta, int isign, int cb, double prec_mass_rho = 0.0) const
   {
-    check(qphix_amd_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, clov, beta * cs_,
-                                           beta * ct_, beta * ct_, prec_mass_rho, stream_),
-          "qphix_amd_clov_dslash_achimbdpsi");
+    if (q_)
+      check(qphix_amd_qphix_clov_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, clov, acc_,
+                                                   beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_),
+            "qphix_amd_qphix_clov_dslash_achimbdpsi");
+    else
+      check(qphix_amd_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, clov, beta * cs_,
+                                             beta * ct_, beta * ct_, prec_mass_rho, stream_),
+            "qphix_amd_clov_dslash_achimbdpsi");
   }
   /* face completion of dslash (beta = +coeff) */
   void face_finish_dslash(void *res, const void *buf, const void *u, const void *invclov, double beta, int isign,
                           int cb, int dim, int fb) const
   {
-    check(qphix_amd_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invclov, beta,
-                                     stream_),
-          "qphix_amd_clov_face_finish");
+    if (q_)
+      check(qphix_amd_qphix_clov_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, invclov, beta,
+                                             stream_),
+            "qphix_amd_qphix_clov_face_finish");
+    else
+      check(qphix_amd_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invclov, beta,
+                                       stream_),
+            "qphix_amd_clov_face_finish");
   }
 };
 
@@ -314,24 +371,39 @@ class TMDslash : public Dslash {
   void dslash(void *res, const void *psi, const void *u, int isign, int cb) const
   {
     const double mu = mu_phys_ / alpha_, muinv = alpha_ / (alpha_ * alpha_ + mu_phys_ * mu_phys_);
-    check(qphix_amd_tm_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, mu, muinv, stream_),
-          "qphix_amd_tm_dslash");
+    if (q_)
+      check(qphix_amd_qphix_tm_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, acc_, cs_, ct_, ct_, mu, muinv,
+                                      stream_),
+            "qphix_amd_qphix_tm_dslash");
+    else
+      check(qphix_amd_tm_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, mu, muinv, stream_),
+            "qphix_amd_tm_dslash");
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, double alpha,
                             double beta, int isign, int cb) const
   {
-    check(qphix_amd_tm_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha, beta * cs_,
-                                         beta * ct_, beta * ct_, mu_phys_ / alpha, stream_),
-          "qphix_amd_tm_dslash_achimbdpsi");
+    if (q_)
+      check(qphix_amd_qphix_tm_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, acc_, alpha,
+                                                 beta * cs_, beta * ct_, beta * ct_, mu_phys_ / alpha, stream_),
+            "qphix_amd_qphix_tm_dslash_achimbdpsi");
+    else
+      check(qphix_amd_tm_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha, beta * cs_,
+                                           beta * ct_, beta * ct_, mu_phys_ / alpha, stream_),
+            "qphix_amd_tm_dslash_achimbdpsi");
   }
   /* face completion of dslash (beta = -coeff), with the twisted epilogue */
   void face_finish_dslash(void *res, const void *buf, const void *u, double beta, int isign, int cb, int dim,
                           int fb) const
   {
     const double mu = mu_phys_ / alpha_, muinv = alpha_ / (alpha_ * alpha_ + mu_phys_ * mu_phys_);
-    check(qphix_amd_tm_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu, muinv,
-                                   stream_),
-          "qphix_amd_tm_face_finish");
+    if (q_)
+      check(qphix_amd_qphix_tm_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu, muinv,
+                                           stream_),
+            "qphix_amd_qphix_tm_face_finish");
+    else
+      check(qphix_amd_tm_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu, muinv,
+                                     stream_),
+            "qphix_amd_tm_face_finish");
   }
 
  private:
@@ -347,24 +419,40 @@ class TMClovDslash : public Dslash {
 
   void dslash(void *res, const void *psi, const void *u, const void *invfclov, int isign, int cb) const
   {
-    check(qphix_amd_tm_clov_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, invfclov, cs_, ct_, ct_,
-                                   stream_),
-          "qphix_amd_tm_clov_dslash");
+    if (q_)
+      check(qphix_amd_qphix_tm_clov_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, invfclov, acc_, cs_, ct_,
+                                           ct_, stream_),
+            "qphix_amd_qphix_tm_clov_dslash");
+    else
+      check(qphix_amd_tm_clov_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, invfclov, cs_, ct_, ct_,
+                                     stream_),
+            "qphix_amd_tm_clov_dslash");
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, const void *fclov,
                             double beta, int isign, int cb, double prec_mass_rho = 0.0) const
   {
-    check(qphix_amd_tm_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, fclov,
-                                              beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_),
-          "qphix_amd_tm_clov_dslash_achimbdpsi");
+    if (q_)
+      check(qphix_amd_qphix_tm_clov_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, fclov,
+                                                      acc_, beta * cs_, beta * ct_, beta * ct_, prec_mass_rho,
+                                                      stream_),
+            "qphix_amd_qphix_tm_clov_dslash_achimbdpsi");
+    else
+      check(qphix_amd_tm_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, fclov,
+                                                beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_),
+            "qphix_amd_tm_clov_dslash_achimbdpsi");
   }
   /* face completion of dslash (beta = +coeff) */
   void face_finish_dslash(void *res, const void *buf, const void *u, const void *invfclov, double beta, int isign,
                           int cb, int dim, int fb) const
   {
-    check(qphix_amd_tm_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invfclov, beta,
-                                        stream_),
-          "qphix_amd_tm_clov_face_finish");
+    if (q_)
+      check(qphix_amd_qphix_tm_clov_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, invfclov,
+                                                beta, stream_),
+            "qphix_amd_qphix_tm_clov_face_finish");
+    else
+      check(qphix_amd_tm_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invfclov, beta,
+                                          stream_),
+            "qphix_amd_tm_clov_face_finish");
   }
 };
 

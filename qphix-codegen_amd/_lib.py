@@ -112,6 +112,15 @@ SIGNATURES = {
                                           _c_double, _c_double, _c_double, _vp]),
     "qphix_amd_tm_clov_face_finish": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
                                                _vp, _vp, _c_double, _vp]),
+    "qphix_amd_qphix_face_pack": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "qphix_amd_qphix_face_finish": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                             _c_double, _vp]),
+    "qphix_amd_qphix_clov_face_finish": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
+                                                  _vp, _vp, _c_double, _vp]),
+    "qphix_amd_qphix_tm_face_finish": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
+                                                _vp, _c_double, _c_double, _c_double, _vp]),
+    "qphix_amd_qphix_tm_clov_face_finish": (_c_int, [_vp, _lay, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
+                                                     _vp, _vp, _vp, _c_double, _vp]),
     "qphix_amd_dslash_adjoint": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                           _c_double, _c_double, _c_double, _vp]),
     "qphix_amd_dslash_force": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_double,

@@ -76,6 +76,30 @@ __device__ __forceinline__ int native_index(const Geom &g, const SiteCoord &c)
   return ((c.t * g.Lz + c.z) * g.Ly + c.y) * g.Lxh + c.xh;
 }
 
+// Spinor addressing of the face kernels: the native layout, or (sh >= 0)
+// QPhiX FourSpinorBlock arrays of soalen 1 << sh with block strides nvecs
+// (y), pxy (z), pxyz (t), read and written element by element in place
+// (codegen/include/decl_common.h:11).
+struct FaceSpinorGeom {
+  int sh, nvecs, pxy, pxyz;
+};
+
+__device__ __forceinline__ int qphix_offset(const FaceSpinorGeom &q, const SiteCoord &c)
+{
+  const int blk = c.t * q.pxyz + c.z * q.pxy + c.y * q.nvecs + (c.xh >> q.sh);
+  return ((blk * 24) << q.sh) + (c.xh & ((1 << q.sh) - 1));
+}
+
+template <typename S, typename R>
+__device__ __forceinline__ void load_site(const S *a, const Geom &g, const FaceSpinorGeom &q, const SiteCoord &c,
+                                          R *v)
+{
+  if (q.sh >= 0)
+    load_q_reals<S, R>(a, qphix_offset(q, c), q.sh, v);
+  else
+    load_reals<S, spinor_w<S>(), 0, 24>(a, g.vcb, native_index(g, c), v);
+}
+
 template <typename R>
 __device__ __forceinline__ void project_rt(int mu, int sgn, const R *psi, cx<R> h[2][3])
 {
@@ -108,15 +132,15 @@ __device__ __forceinline__ void reconstruct_rt(int mu, int sgn, cx<R> acc[4][3],
 
 // buf[f] = (1 + sgn gamma_mu) in(site f), in on checkerboard cb_in.
 template <typename S>
-__global__ __launch_bounds__(kFaceBlock) void face_pack_k(S *buf, const S *in, Geom g, int cb_in, int mu,
-                                                          int side, int sgn, int nface)
+__global__ __launch_bounds__(kFaceBlock) void face_pack_k(S *buf, const S *in, Geom g, FaceSpinorGeom q, int cb_in,
+                                                          int mu, int side, int sgn, int nface)
 {
   using R = typename Traits<S>::R;
   const int f = blockIdx.x * kFaceBlock + threadIdx.x;
   if (f >= nface) return;
   const SiteCoord c = face_site(g, mu, side, f, cb_in);
   R psi[24];
-  load_reals<S, spinor_w<S>(), 0, 24>(in, g.vcb, native_index(g, c), psi);
+  load_site(in, g, q, c, psi);
   cx<R> h[2][3];
   project_rt(mu, sgn, psi, h);
   R o[12];
@@ -149,7 +173,7 @@ template <typename R> struct FaceArgs {
 // R the reconstruction of the half-spinor to the full spinor.
 template <typename S, int RECON, int OP>
 __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf, const S *gauge,
-                                                            const S *clov, Geom g,
+                                                            const S *clov, Geom g, FaceSpinorGeom q,
                                                             FaceArgs<typename Traits<S>::R> a)
 {
   using R = typename Traits<S>::R;
@@ -190,7 +214,7 @@ __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf
     }
   const int sgn = a.side ? -a.isign : a.isign;
   R o[24];
-  load_reals<S, spinor_w<S>(), 0, 24>(out, vcb, idx, o);
+  load_site(out, g, q, c, o);
   cx<R> acc[4][3];
 #pragma unroll
   for (int s = 0; s < 4; s++)
@@ -230,7 +254,18 @@ __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf
       o[(s * 3 + k) * 2] = acc[s][k].re;
       o[(s * 3 + k) * 2 + 1] = acc[s][k].im;
     }
-  store_reals<S, spinor_w<S>(), 0, 24>(out, vcb, idx, o);
+  if (q.sh >= 0) {
+    const int qo = qphix_offset(q, c);
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+          out[qo + (((cc * 4 + sp) * 2 + r) << q.sh)] = to_storage<S>(o[(sp * 3 + cc) * 2 + r]);
+  } else {
+    store_reals<S, spinor_w<S>(), 0, 24>(out, vcb, idx, o);
+  }
 }
 
 template <typename F> hipError_t with_prec(int prec, F &&fn)
@@ -245,24 +280,39 @@ template <typename F> hipError_t with_prec(int prec, F &&fn)
 }
 
 template <typename S, int OP>
-hipError_t launch_finish(int recon, const Geom &g, S *out, const S *buf, const S *gauge, const S *clov,
-                         const FaceArgs<typename Traits<S>::R> &a, hipStream_t st)
+hipError_t launch_finish(int recon, const Geom &g, const FaceSpinorGeom &q, S *out, const S *buf, const S *gauge,
+                         const S *clov, const FaceArgs<typename Traits<S>::R> &a, hipStream_t st)
 {
   const dim3 grid(blocks_for(a.nface, kFaceBlock)), block(kFaceBlock);
   if (recon == 12)
-    hipLaunchKernelGGL((face_finish_k<S, 12, OP>), grid, block, 0, st, out, buf, gauge, clov, g, a);
+    hipLaunchKernelGGL((face_finish_k<S, 12, OP>), grid, block, 0, st, out, buf, gauge, clov, g, q, a);
   else
-    hipLaunchKernelGGL((face_finish_k<S, 18, OP>), grid, block, 0, st, out, buf, gauge, clov, g, a);
+    hipLaunchKernelGGL((face_finish_k<S, 18, OP>), grid, block, 0, st, out, buf, gauge, clov, g, q, a);
   return hipGetLastError();
 }
+
+constexpr FaceSpinorGeom kNativeSpinor = {-1, 0, 0, 0};
+
+// The FourSpinorBlock geometry of a QPhiX layout, or false when the lattice
+// does not admit it (or its FT is not a QPhiX one).
+bool face_qgeom(const qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, FaceSpinorGeom &out)
+{
+  QSpinorGeom q;
+  if (!lay || lay->ft == QPHIX_AMD_BF16 || !qphix_spinor_geometry(ctx, lay, q)) return false;
+  out = {q.sh, q.nvecs, q.pxy, q.pxyz};
+  return true;
+}
+
+int storage_size(int prec) { return prec == QPHIX_AMD_F64 ? 8 : prec == QPHIX_AMD_F32 ? 4 : 2; }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 int status(hipError_t e) { return e == hipSuccess ? QPHIX_AMD_OK : QPHIX_AMD_EHIP; }
 
-int finish(qphix_amd_ctx *ctx, int op, int prec, int recon, int isign, int cb, int dim, int fb,
-           void *out, const void *buf, const void *gauge, const void *clov, double beta, double mu,
-           double muinv, void *stream)
+// lay: null for native spinors, else out is a QPhiX array of that layout
+int finish(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int op, int prec, int recon, int isign, int cb,
+           int dim, int fb, void *out, const void *buf, const void *gauge, const void *clov, double beta,
+           double mu, double muinv, void *stream)
 {
   if (!ctx || prec < QPHIX_AMD_F64 || prec > QPHIX_AMD_BF16 || (recon != 12 && recon != 18))
     return QPHIX_AMD_EINVAL;
@@ -270,7 +320,11 @@ int finish(qphix_amd_ctx *ctx, int op, int prec, int recon, int isign, int cb, i
     return QPHIX_AMD_EINVAL;
   if (!out || !buf || !gauge || out == buf) return QPHIX_AMD_EINVAL;
   if ((op == OP_CLOVER || op == OP_TMCLOVER) && !clov) return QPHIX_AMD_EINVAL;
-  if (!aligned16(out) || !aligned16(buf) || !aligned16(gauge) || (clov && !aligned16(clov)))
+  FaceSpinorGeom q = kNativeSpinor;
+  if (lay && !face_qgeom(ctx, lay, q)) return QPHIX_AMD_EINVAL;
+  // QPhiX arrays need their FT's alignment only (element accesses)
+  const bool out_ok = lay ? (reinterpret_cast<uintptr_t>(out) % storage_size(prec)) == 0 : aligned16(out);
+  if (!out_ok || !aligned16(buf) || !aligned16(gauge) || (clov && !aligned16(clov)))
     return QPHIX_AMD_EALIGN;
   const Geom &g = ctx->g;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -294,10 +348,10 @@ int finish(qphix_amd_ctx *ctx, int op, int prec, int recon, int isign, int cb, i
     const S *b = static_cast<const S *>(buf), *u = static_cast<const S *>(gauge);
     const S *cl = static_cast<const S *>(clov);
     switch (op) {
-      case OP_WILSON: return launch_finish<S, OP_WILSON>(recon, g, o, b, u, cl, a, st);
-      case OP_CLOVER: return launch_finish<S, OP_CLOVER>(recon, g, o, b, u, cl, a, st);
-      case OP_TM: return launch_finish<S, OP_TM>(recon, g, o, b, u, cl, a, st);
-      default: return launch_finish<S, OP_TMCLOVER>(recon, g, o, b, u, cl, a, st);
+      case OP_WILSON: return launch_finish<S, OP_WILSON>(recon, g, q, o, b, u, cl, a, st);
+      case OP_CLOVER: return launch_finish<S, OP_CLOVER>(recon, g, q, o, b, u, cl, a, st);
+      case OP_TM: return launch_finish<S, OP_TM>(recon, g, q, o, b, u, cl, a, st);
+      default: return launch_finish<S, OP_TMCLOVER>(recon, g, q, o, b, u, cl, a, st);
     }
   }));
 }
@@ -328,33 +382,57 @@ long qphix_amd_face_sites(const qphix_amd_ctx *ctx, int dim)
   return qamd::face_count(ctx->g, dim);
 }
 
-int qphix_amd_face_pack(qphix_amd_ctx *ctx, int prec, int isign, int cb, int dim, int fb, void *buf,
-                        const void *in, void *stream)
+}  // extern "C"
+
+namespace qamd {
+namespace {
+// lay: null for a native spinor in, else a QPhiX array of that layout
+int pack(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int prec, int isign, int cb, int dim, int fb,
+         void *buf, const void *in, void *stream)
 {
   if (!ctx || prec < QPHIX_AMD_F64 || prec > QPHIX_AMD_BF16) return QPHIX_AMD_EINVAL;
   if ((isign != 1 && isign != -1) || (cb != 0 && cb != 1) || dim < 0 || dim > 3 || (fb != 0 && fb != 1))
     return QPHIX_AMD_EINVAL;
   if (!buf || !in || buf == in) return QPHIX_AMD_EINVAL;
-  if (!qamd::aligned16(buf) || !qamd::aligned16(in)) return QPHIX_AMD_EALIGN;
-  const qamd::Geom &g = ctx->g;
+  FaceSpinorGeom q = kNativeSpinor;
+  if (lay && !face_qgeom(ctx, lay, q)) return QPHIX_AMD_EINVAL;
+  const bool in_ok = lay ? (reinterpret_cast<uintptr_t>(in) % storage_size(prec)) == 0 : aligned16(in);
+  if (!aligned16(buf) || !in_ok) return QPHIX_AMD_EALIGN;
+  const Geom &g = ctx->g;
   const int nface = qamd::face_count(g, dim);
   // sent backwards: the receiver's forward hop, projector (1 - isign gamma);
   // sent forwards: its backward hop, (1 + isign gamma) (codegen/dslash.cc:451-457)
   const int sgn = fb == 0 ? -isign : isign;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  return qamd::status(qamd::with_prec(prec, [&](auto *tag) -> hipError_t {
+  return status(with_prec(prec, [&](auto *tag) -> hipError_t {
     using S = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(qamd::face_pack_k<S>, dim3(qamd::blocks_for(nface, qamd::kFaceBlock)),
-                       dim3(qamd::kFaceBlock), 0, st, static_cast<S *>(buf),
-                       static_cast<const S *>(in), g, 1 - cb, dim, fb, sgn, nface);
+    hipLaunchKernelGGL(face_pack_k<S>, dim3(blocks_for(nface, kFaceBlock)), dim3(kFaceBlock), 0, st,
+                       static_cast<S *>(buf), static_cast<const S *>(in), g, q, 1 - cb, dim, fb, sgn, nface);
     return hipGetLastError();
   }));
+}
+}  // namespace
+}  // namespace qamd
+
+extern "C" {
+
+int qphix_amd_face_pack(qphix_amd_ctx *ctx, int prec, int isign, int cb, int dim, int fb, void *buf,
+                        const void *in, void *stream)
+{
+  return qamd::pack(ctx, nullptr, prec, isign, cb, dim, fb, buf, in, stream);
+}
+
+int qphix_amd_qphix_face_pack(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int isign, int cb, int dim,
+                              int fb, void *buf, const void *in, void *stream)
+{
+  if (!lay) return QPHIX_AMD_EINVAL;
+  return qamd::pack(ctx, lay, lay->ft, isign, cb, dim, fb, buf, in, stream);
 }
 
 int qphix_amd_face_finish(qphix_amd_ctx *ctx, int prec, int recon, int isign, int cb, int dim, int fb,
                           void *out, const void *buf, const void *gauge, double beta, void *stream)
 {
-  return qamd::finish(ctx, OP_WILSON, prec, recon, isign, cb, dim, fb, out, buf, gauge, nullptr, beta,
+  return qamd::finish(ctx, nullptr, OP_WILSON, prec, recon, isign, cb, dim, fb, out, buf, gauge, nullptr, beta,
                       0.0, 0.0, stream);
 }
 
@@ -362,7 +440,7 @@ int qphix_amd_clov_face_finish(qphix_amd_ctx *ctx, int prec, int recon, int isig
                                int fb, void *out, const void *buf, const void *gauge,
                                const void *invclov, double beta, void *stream)
 {
-  return qamd::finish(ctx, OP_CLOVER, prec, recon, isign, cb, dim, fb, out, buf, gauge, invclov, beta,
+  return qamd::finish(ctx, nullptr, OP_CLOVER, prec, recon, isign, cb, dim, fb, out, buf, gauge, invclov, beta,
                       0.0, 0.0, stream);
 }
 
@@ -370,7 +448,7 @@ int qphix_amd_tm_face_finish(qphix_amd_ctx *ctx, int prec, int recon, int isign,
                              void *out, const void *buf, const void *gauge, double beta, double mu,
                              double muinv, void *stream)
 {
-  return qamd::finish(ctx, OP_TM, prec, recon, isign, cb, dim, fb, out, buf, gauge, nullptr, beta, mu,
+  return qamd::finish(ctx, nullptr, OP_TM, prec, recon, isign, cb, dim, fb, out, buf, gauge, nullptr, beta, mu,
                       muinv, stream);
 }
 
@@ -378,7 +456,45 @@ int qphix_amd_tm_clov_face_finish(qphix_amd_ctx *ctx, int prec, int recon, int i
                                   int fb, void *out, const void *buf, const void *gauge,
                                   const void *fclov, double beta, void *stream)
 {
-  return qamd::finish(ctx, OP_TMCLOVER, prec, recon, isign, cb, dim, fb, out, buf, gauge, fclov, beta,
+  return qamd::finish(ctx, nullptr, OP_TMCLOVER, prec, recon, isign, cb, dim, fb, out, buf, gauge, fclov, beta,
+                      0.0, 0.0, stream);
+}
+
+// QPhiX arrays in place: out a FourSpinorBlock array of the layout lay
+// (FT = lay->ft), buffers, links and clover blocks as in the native calls
+int qphix_amd_qphix_face_finish(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon, int isign,
+                                int cb, int dim, int fb, void *out, const void *buf, const void *gauge,
+                                double beta, void *stream)
+{
+  if (!lay) return QPHIX_AMD_EINVAL;
+  return qamd::finish(ctx, lay, OP_WILSON, lay->ft, recon, isign, cb, dim, fb, out, buf, gauge, nullptr, beta,
+                      0.0, 0.0, stream);
+}
+
+int qphix_amd_qphix_clov_face_finish(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon, int isign,
+                                     int cb, int dim, int fb, void *out, const void *buf, const void *gauge,
+                                     const void *invclov, double beta, void *stream)
+{
+  if (!lay) return QPHIX_AMD_EINVAL;
+  return qamd::finish(ctx, lay, OP_CLOVER, lay->ft, recon, isign, cb, dim, fb, out, buf, gauge, invclov, beta,
+                      0.0, 0.0, stream);
+}
+
+int qphix_amd_qphix_tm_face_finish(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon, int isign,
+                                   int cb, int dim, int fb, void *out, const void *buf, const void *gauge,
+                                   double beta, double mu, double muinv, void *stream)
+{
+  if (!lay) return QPHIX_AMD_EINVAL;
+  return qamd::finish(ctx, lay, OP_TM, lay->ft, recon, isign, cb, dim, fb, out, buf, gauge, nullptr, beta, mu,
+                      muinv, stream);
+}
+
+int qphix_amd_qphix_tm_clov_face_finish(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int recon,
+                                        int isign, int cb, int dim, int fb, void *out, const void *buf,
+                                        const void *gauge, const void *fclov, double beta, void *stream)
+{
+  if (!lay) return QPHIX_AMD_EINVAL;
+  return qamd::finish(ctx, lay, OP_TMCLOVER, lay->ft, recon, isign, cb, dim, fb, out, buf, gauge, fclov, beta,
                       0.0, 0.0, stream);
 }
 

@@ -232,13 +232,25 @@ class Geometry:
         return f
 
 
+def _qphix_spinor_ok(geom, f):
+    n = geom.qphix_field_bytes(f.layout, FIELD_SPINOR) // f.data.element_size()
+    if f.data.numel() < n:
+        raise QPhiXAMDError("QPhiX array smaller than the layout requires")
+
+
 def face_pack(geom, buf, psi, isign, cb, dim, fb):
     """buf <- projected half-spinors of psi (checkerboard 1-cb) on the slice
-    x_dim = 0 (fb 0, sent backwards) or L_dim - 1 (fb 1, sent forwards)."""
+    x_dim = 0 (fb 0, sent backwards) or L_dim - 1 (fb 1, sent forwards); psi
+    a native spinor or a QPhiXSpinor (read in place)."""
     if buf.dtype != TORCH_DTYPE[psi.prec] or buf.numel() < geom.face_sites(dim) * 12:
         raise QPhiXAMDError("face buffer of the wrong type or size")
-    check(lib().qphix_amd_face_pack(geom.ctx, psi.prec, isign, cb, dim, fb, _ptr(buf), psi.ptr(), _stream()),
-          "qphix_amd_face_pack")
+    if isinstance(psi, QPhiXSpinor):
+        _qphix_spinor_ok(geom, psi)
+        st = lib().qphix_amd_qphix_face_pack(geom.ctx, ctypes.byref(psi.layout), isign, cb, dim, fb, _ptr(buf),
+                                             psi.ptr(), _stream())
+    else:
+        st = lib().qphix_amd_face_pack(geom.ctx, psi.prec, isign, cb, dim, fb, _ptr(buf), psi.ptr(), _stream())
+    check(st, "face pack")
     return buf
 
 
@@ -249,15 +261,25 @@ def face_finish(geom, kind, out, buf, u, isign, cb, dim, fb, beta, clov=None, mu
     with r the reconstructed beta * U h."""
     if buf.dtype != TORCH_DTYPE[out.prec] or buf.numel() < geom.face_sites(dim) * 12:
         raise QPhiXAMDError("face buffer of the wrong type or size")
-    l, a = lib(), (geom.ctx, out.prec, u.recon, isign, cb, dim, fb, out.ptr(), _ptr(buf), u.ptr())
-    if kind == 0:
-        st = l.qphix_amd_face_finish(*a, float(beta), _stream())
-    elif kind == 1:
-        st = l.qphix_amd_clov_face_finish(*a, clov.ptr(), float(beta), _stream())
-    elif kind == 2:
-        st = l.qphix_amd_tm_face_finish(*a, float(beta), float(mu), float(muinv), _stream())
+    if u.prec != out.prec or (clov is not None and clov.prec != out.prec):
+        raise QPhiXAMDError("out, u and clov must share a storage precision")
+    if kind in (1, 3) and clov is None:
+        raise QPhiXAMDError("the clover finishes need clov")
+    l = lib()
+    if isinstance(out, QPhiXSpinor):  # the output array updated in place
+        _qphix_spinor_ok(geom, out)
+        a, pre = (geom.ctx, ctypes.byref(out.layout), u.recon, isign, cb, dim, fb, out.ptr(), _ptr(buf),
+                  u.ptr()), "qphix_amd_qphix_"
     else:
-        st = l.qphix_amd_tm_clov_face_finish(*a, clov.ptr(), float(beta), _stream())
+        a, pre = (geom.ctx, out.prec, u.recon, isign, cb, dim, fb, out.ptr(), _ptr(buf), u.ptr()), "qphix_amd_"
+    if kind == 0:
+        st = getattr(l, pre + "face_finish")(*a, float(beta), _stream())
+    elif kind == 1:
+        st = getattr(l, pre + "clov_face_finish")(*a, clov.ptr(), float(beta), _stream())
+    elif kind == 2:
+        st = getattr(l, pre + "tm_face_finish")(*a, float(beta), float(mu), float(muinv), _stream())
+    else:
+        st = getattr(l, pre + "tm_clov_face_finish")(*a, clov.ptr(), float(beta), _stream())
     check(st, "face finish")
     return out
 

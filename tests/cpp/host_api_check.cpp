@@ -179,25 +179,49 @@ int gpu_mode(const char *in_path, const char *out_path)
       geom.pack_clover(Ai.p, prec, inv.d(), cb);
       geom.pack_full_clover(B.p, prec, (isign > 0 ? Bp : Bm).d(), cb);
       geom.pack_full_clover(F.p, prec, (isign > 0 ? Fp : Fm).d(), cb);
+      // the same classes on a QPhiX host's FourSpinorBlock arrays in place
+      // (veclen 4 soalen 2 in FP64, 8 and 4 in FP32): the same bits
+      const qphix_amd_qphix_layout lay = {prec, prec == QPHIX_AMD_F64 ? 4 : 8, prec == QPHIX_AMD_F64 ? 2 : 4, 1, 0, 0};
+      const size_t qb = geom.qphix_field_bytes(lay, QPHIX_AMD_FIELD_SPINOR);
+      const size_t nb = geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, prec);
+      Dev pq(qb), cq(qb), rq(qb), r2(nb);
+      geom.spinor_to_qphix(lay, pq.p, p.p, prec, 1 - cb);
+      geom.spinor_to_qphix(lay, cq.p, c.p, prec, cb);
+      std::vector<char> n1(nb), n2(nb);
       for (int kind = 0; kind < 4; kind++) {
         for (int achi = 0; achi < 2; achi++) {
-          switch (kind) {
-            case 0:
-              if (achi) W.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, alpha, beta, isign, cb);
-              else W.dslash(r.p, p.p, g.p, isign, cb);
-              break;
-            case 1:
-              if (achi) C.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, A.p, beta, isign, cb, rho);
-              else C.dslash(r.p, p.p, g.p, Ai.p, isign, cb);
-              break;
-            case 2:
-              if (achi) T.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, alpha, beta, isign, cb);
-              else T.dslash(r.p, p.p, g.p, isign, cb);
-              break;
-            default:
-              if (achi) TC.dslashAChiMinusBDPsi(r.p, p.p, c.p, g.p, F.p, beta, isign, cb, rho);
-              else TC.dslash(r.p, p.p, g.p, B.p, isign, cb);
+          for (int inplace = 0; inplace < 2; inplace++) {
+            void *res = inplace ? rq.p : r.p;
+            const void *in = inplace ? pq.p : p.p, *ch = inplace ? cq.p : c.p;
+            Dslash &base = kind == 0 ? W : kind == 1 ? static_cast<Dslash &>(C)
+                                       : kind == 2 ? static_cast<Dslash &>(T) : static_cast<Dslash &>(TC);
+            if (inplace)
+              base.use_qphix_arrays(lay);
+            else
+              base.use_native_spinors();
+            switch (kind) {
+              case 0:
+                if (achi) W.dslashAChiMinusBDPsi(res, in, ch, g.p, alpha, beta, isign, cb);
+                else W.dslash(res, in, g.p, isign, cb);
+                break;
+              case 1:
+                if (achi) C.dslashAChiMinusBDPsi(res, in, ch, g.p, A.p, beta, isign, cb, rho);
+                else C.dslash(res, in, g.p, Ai.p, isign, cb);
+                break;
+              case 2:
+                if (achi) T.dslashAChiMinusBDPsi(res, in, ch, g.p, alpha, beta, isign, cb);
+                else T.dslash(res, in, g.p, isign, cb);
+                break;
+              default:
+                if (achi) TC.dslashAChiMinusBDPsi(res, in, ch, g.p, F.p, beta, isign, cb, rho);
+                else TC.dslash(res, in, g.p, B.p, isign, cb);
+            }
+            base.use_native_spinors();
           }
+          geom.spinor_from_qphix(lay, r2.p, prec, rq.p, cb);
+          HIPCHECK(hipMemcpy(n1.data(), r.p, nb, hipMemcpyDeviceToHost));
+          HIPCHECK(hipMemcpy(n2.data(), r2.p, nb, hipMemcpyDeviceToHost));
+          if (std::memcmp(n1.data(), n2.data(), nb) != 0) return fail("in-place class call differs from native");
           HIPCHECK(hipMemset(outlex.p, 0, nS * 8));
           geom.unpack_spinor(outlex.d(), r.p, prec, cb);
           HIPCHECK(hipMemcpy(host.data(), outlex.p, nS * 8, hipMemcpyDeviceToHost));
@@ -326,74 +350,127 @@ int split_mode(const char *in_path, const char *out_path)
       src += fd.n;
     }
   }
+  // the same split again with the spinors kept in QPhiX FourSpinorBlock
+  // arrays (FP64, veclen = soalen = 2): the classes in their in-place mode,
+  // faces packed from and finished into those arrays (must give the same
+  // bits as the native run)
+  const qphix_amd_qphix_layout lq = {QPHIX_AMD_F64, 2, 2, 0, 0, 0};
+  std::vector<std::unique_ptr<Dev>> qarr;  // per half: psi, chi, res
+  for (int k = 0; k < 2; k++) {
+    Half &H = *half[k];
+    const size_t qb = H.geom.qphix_field_bytes(lq, QPHIX_AMD_FIELD_SPINOR);
+    for (int a = 0; a < 3; a++) qarr.emplace_back(new Dev(qb));
+    H.geom.spinor_to_qphix(lq, qarr[3 * k]->p, H.psi.p, prec, 1 - cb);
+    H.geom.spinor_to_qphix(lq, qarr[3 * k + 1]->p, H.chi.p, prec, cb);
+  }
+  const size_t nb = half[0]->geom.field_bytes(QPHIX_AMD_FIELD_SPINOR, prec);
+  Dev back(nb);
+  std::vector<char> hn(nb), hq(nb);
+
+  // one (family, kernel) on both halves: interior, faces, finishes
+  auto run = [&](int kind, int achi, bool inplace) {
+    for (int k = 0; k < 2; k++) {
+      Half &H = *half[k];
+      void *res = inplace ? qarr[3 * k + 2]->p : H.res.p;
+      const void *psi = inplace ? qarr[3 * k]->p : H.psi.p, *chi = inplace ? qarr[3 * k + 1]->p : H.chi.p;
+      Dslash W(H.geom, prec, recon, aniso_s, aniso_t);
+      ClovDslash C(H.geom, prec, recon, aniso_s, aniso_t);
+      TMDslash Tm(H.geom, prec, recon, mass, tmu, aniso_s, aniso_t);
+      TMClovDslash TC(H.geom, prec, recon, aniso_s, aniso_t);
+      if (inplace) {
+        W.use_qphix_arrays(lq);
+        C.use_qphix_arrays(lq);
+        Tm.use_qphix_arrays(lq);
+        TC.use_qphix_arrays(lq);
+      }
+      switch (kind) {  // the interior part
+        case 0:
+          if (achi) W.dslashAChiMinusBDPsi(res, psi, chi, H.g.p, alpha, beta, isign, cb);
+          else W.dslash(res, psi, H.g.p, isign, cb);
+          break;
+        case 1:
+          if (achi) C.dslashAChiMinusBDPsi(res, psi, chi, H.g.p, H.A.p, beta, isign, cb, rho);
+          else C.dslash(res, psi, H.g.p, H.Ai.p, isign, cb);
+          break;
+        case 2:
+          if (achi) Tm.dslashAChiMinusBDPsi(res, psi, chi, H.g.p, alpha, beta, isign, cb);
+          else Tm.dslash(res, psi, H.g.p, isign, cb);
+          break;
+        default:
+          if (achi) TC.dslashAChiMinusBDPsi(res, psi, chi, H.g.p, H.F.p, beta, isign, cb, rho);
+          else TC.dslash(res, psi, H.g.p, H.B.p, isign, cb);
+      }
+      for (int fb = 0; fb < 2; fb++) {
+        if (inplace)
+          H.geom.face_pack(lq, isign, cb, T, fb, H.face(fb), psi);
+        else
+          H.geom.face_pack(prec, isign, cb, T, fb, H.face(fb), psi);
+      }
+    }
+    for (int k = 0; k < 2; k++) {
+      Half &H = *half[k];
+      void *res = inplace ? qarr[3 * k + 2]->p : H.res.p;
+      Dslash W(H.geom, prec, recon, aniso_s, aniso_t);
+      ClovDslash C(H.geom, prec, recon, aniso_s, aniso_t);
+      TMDslash Tm(H.geom, prec, recon, mass, tmu, aniso_s, aniso_t);
+      TMClovDslash TC(H.geom, prec, recon, aniso_s, aniso_t);
+      if (inplace) {
+        W.use_qphix_arrays(lq);
+        C.use_qphix_arrays(lq);
+        Tm.use_qphix_arrays(lq);
+        TC.use_qphix_arrays(lq);
+      }
+      for (int fb = 0; fb < 2; fb++) {
+        // fb 0 completes t = 0 from the half behind (its forward face),
+        // fb 1 t = Lt - 1 from the half ahead (its backward face)
+        const Half &nb = *half[1 - k];
+        const void *buf = nb.face(1 - fb);
+        const bool edge = fb == 0 ? k == 0 : k == 1;  // the global time boundary
+        const double coeff = aniso_t * (edge ? t_boundary : 1.0);
+        if (achi) {
+          const double b = beta * coeff;
+          switch (kind) {
+            case 0: W.face_finish_achimbdpsi(res, buf, H.g.p, b, isign, cb, T, fb); break;
+            case 1: C.face_finish_achimbdpsi(res, buf, H.g.p, b, isign, cb, T, fb); break;
+            case 2: Tm.face_finish_achimbdpsi(res, buf, H.g.p, b, isign, cb, T, fb); break;
+            default: TC.face_finish_achimbdpsi(res, buf, H.g.p, b, isign, cb, T, fb);
+          }
+        } else {
+          switch (kind) {
+            case 0: W.face_finish_dslash(res, buf, H.g.p, -coeff, isign, cb, T, fb); break;
+            case 1: C.face_finish_dslash(res, buf, H.g.p, H.Ai.p, coeff, isign, cb, T, fb); break;
+            case 2: Tm.face_finish_dslash(res, buf, H.g.p, -coeff, isign, cb, T, fb); break;
+            default: TC.face_finish_dslash(res, buf, H.g.p, H.B.p, coeff, isign, cb, T, fb);
+          }
+        }
+      }
+    }
+  };
+
   FILE *o = std::fopen(out_path, "wb");
   if (!o) return fail("cannot open output");
   std::vector<double> host(nS);
   for (int kind = 0; kind < 4; kind++)
     for (int achi = 0; achi < 2; achi++) {
+      run(kind, achi, false);
       for (int k = 0; k < 2; k++) {
         Half &H = *half[k];
-        Dslash W(H.geom, prec, recon, aniso_s, aniso_t);
-        ClovDslash C(H.geom, prec, recon, aniso_s, aniso_t);
-        TMDslash Tm(H.geom, prec, recon, mass, tmu, aniso_s, aniso_t);
-        TMClovDslash TC(H.geom, prec, recon, aniso_s, aniso_t);
-        switch (kind) {  // the interior part
-          case 0:
-            if (achi) W.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, alpha, beta, isign, cb);
-            else W.dslash(H.res.p, H.psi.p, H.g.p, isign, cb);
-            break;
-          case 1:
-            if (achi) C.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, H.A.p, beta, isign, cb, rho);
-            else C.dslash(H.res.p, H.psi.p, H.g.p, H.Ai.p, isign, cb);
-            break;
-          case 2:
-            if (achi) Tm.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, alpha, beta, isign, cb);
-            else Tm.dslash(H.res.p, H.psi.p, H.g.p, isign, cb);
-            break;
-          default:
-            if (achi) TC.dslashAChiMinusBDPsi(H.res.p, H.psi.p, H.chi.p, H.g.p, H.F.p, beta, isign, cb, rho);
-            else TC.dslash(H.res.p, H.psi.p, H.g.p, H.B.p, isign, cb);
-        }
-        for (int fb = 0; fb < 2; fb++) H.geom.face_pack(prec, isign, cb, T, fb, H.face(fb), H.psi.p);
-      }
-      for (int k = 0; k < 2; k++) {
-        Half &H = *half[k];
-        Dslash W(H.geom, prec, recon, aniso_s, aniso_t);
-        ClovDslash C(H.geom, prec, recon, aniso_s, aniso_t);
-        TMDslash Tm(H.geom, prec, recon, mass, tmu, aniso_s, aniso_t);
-        TMClovDslash TC(H.geom, prec, recon, aniso_s, aniso_t);
-        for (int fb = 0; fb < 2; fb++) {
-          // fb 0 completes t = 0 from the half behind (its forward face),
-          // fb 1 t = Lt - 1 from the half ahead (its backward face)
-          const Half &nb = *half[1 - k];
-          const void *buf = nb.face(1 - fb);
-          const bool edge = fb == 0 ? k == 0 : k == 1;  // the global time boundary
-          const double coeff = aniso_t * (edge ? t_boundary : 1.0);
-          if (achi) {
-            const double b = beta * coeff;
-            switch (kind) {
-              case 0: W.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb); break;
-              case 1: C.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb); break;
-              case 2: Tm.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb); break;
-              default: TC.face_finish_achimbdpsi(H.res.p, buf, H.g.p, b, isign, cb, T, fb);
-            }
-          } else {
-            switch (kind) {
-              case 0: W.face_finish_dslash(H.res.p, buf, H.g.p, -coeff, isign, cb, T, fb); break;
-              case 1: C.face_finish_dslash(H.res.p, buf, H.g.p, H.Ai.p, coeff, isign, cb, T, fb); break;
-              case 2: Tm.face_finish_dslash(H.res.p, buf, H.g.p, -coeff, isign, cb, T, fb); break;
-              default: TC.face_finish_dslash(H.res.p, buf, H.g.p, H.B.p, coeff, isign, cb, T, fb);
-            }
-          }
-        }
         HIPCHECK(hipMemset(H.outlex.p, 0, nS * 8));
         H.geom.unpack_spinor(H.outlex.d(), H.res.p, prec, cb);
         HIPCHECK(hipMemcpy(host.data(), H.outlex.p, nS * 8, hipMemcpyDeviceToHost));
         std::fwrite(host.data(), sizeof(double), nS, o);
       }
+      run(kind, achi, true);
+      for (int k = 0; k < 2; k++) {
+        Half &H = *half[k];
+        H.geom.spinor_from_qphix(lq, back.p, prec, qarr[3 * k + 2]->p, cb);
+        HIPCHECK(hipMemcpy(hn.data(), H.res.p, nb, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hq.data(), back.p, nb, hipMemcpyDeviceToHost));
+        if (std::memcmp(hn.data(), hq.data(), nb) != 0) return fail("in-place split differs from native split");
+      }
     }
   std::fclose(o);
-  std::printf("split ok\n");
+  std::printf("split ok (in place on QPhiX arrays: same bits)\n");
   return 0;
 }
 

@@ -4,7 +4,10 @@ host program tests/cpp/host_api_check, built with the library.
 16x16x8x8: every family (Dslash, ClovDslash, TMDslash, TMClovDslash) x
 {dslash, dslashAChiMinusBDPsi} x isign +1 (cb 0) / -1 (cb 1), FP64 and FP32
 storage with 12-real links, anisotropic coefficients and antiperiodic time,
-against the FP64 oracle (1e-13 / 1e-6 relative); then the even-odd clover
+against the FP64 oracle (1e-13 / 1e-6 relative), and each call again in the
+classes' in-place mode on QPhiX FourSpinorBlock arrays (use_qphix_arrays),
+which the host program requires to give the same bits; then the even-odd
+clover
 operator's FP64 CG and FP16 mixed CG to a true residual of 1e-10 and their
 agreement; its force and force_clover at the FP64 solution against the same
 calls through the Python binding."""
@@ -105,7 +108,10 @@ def test_cpp_face_wrappers_split_t(qamd, orc, tmp_path):
     half runs the interior operator, packs its t faces, and completes its
     boundary sites from the other half's faces; the halves assembled must
     equal the whole-lattice FP64 oracle (1e-13), for every family, dslash and
-    achimbdpsi (isign +1, output checkerboard 0, antiperiodic time)."""
+    achimbdpsi (isign +1, output checkerboard 0, antiperiodic time).  The host
+    program then repeats the split with the spinors in QPhiX FourSpinorBlock
+    arrays (classes in their in-place mode, Geometry::face_pack from the
+    arrays, finishes into them) and requires the same bits."""
     import qphix_blocks as qb
     assert os.path.exists(EXE), "tests/cpp/host_api_check is not built (build() builds it)"
     G = (16, 8, 8, 8)

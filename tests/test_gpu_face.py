@@ -87,8 +87,33 @@ def _run(lib, geom, kind, achi, prec, recon, isign, cb, res, psi, chi, g, cl):
     assert rc == 0
 
 
-def split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon, tbc=None):
-    """The operator on the whole lattice G assembled from its parts."""
+def _run_q(lib, geom, kind, achi, recon, isign, cb, res, psi, chi, g, cl):
+    """_run on QPhiX arrays in place (qphix_amd_qphix_*), the same scalars."""
+    import ctypes
+    a = (geom.ctx, ctypes.byref(res.layout), recon, isign, cb, res.ptr(), psi.ptr())
+    if kind == 0:
+        rc = (lib.qphix_amd_qphix_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), None, ALPHA, CS, CTF, CTB, None)
+              if achi else lib.qphix_amd_qphix_dslash(*a, g.ptr(), None, CS, CTF, CTB, None))
+    elif kind == 1:
+        rc = (lib.qphix_amd_qphix_clov_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), cl.ptr(), None, CS, CTF, CTB, RHO,
+                                                         None)
+              if achi else lib.qphix_amd_qphix_clov_dslash(*a, g.ptr(), cl.ptr(), None, CS, CTF, CTB, None))
+    elif kind == 2:
+        rc = (lib.qphix_amd_qphix_tm_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), None, ALPHA, CS, CTF, CTB, MU,
+                                                       None)
+              if achi else lib.qphix_amd_qphix_tm_dslash(*a, g.ptr(), None, CS, CTF, CTB, MU, MUINV, None))
+    else:
+        rc = (lib.qphix_amd_qphix_tm_clov_dslash_achimbdpsi(*a, chi.ptr(), g.ptr(), cl.ptr(), None, CS, CTF, CTB,
+                                                            RHO, None)
+              if achi else lib.qphix_amd_qphix_tm_clov_dslash(*a, g.ptr(), cl.ptr(), None, CS, CTF, CTB, None))
+    assert rc == 0
+
+
+def split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon, tbc=None, qv=None):
+    """The operator on the whole lattice G assembled from its parts.  qv =
+    (veclen, soalen): the parts keep psi, chi and the result as QPhiX
+    FourSpinorBlock arrays, which the operators, face_pack and face_finish
+    read and write in place."""
     lib = qamd.lib()
     Lloc = tuple(n // p for n, p in zip(G, grid))
     comm = [int(p > 1) for p in grid]
@@ -112,7 +137,14 @@ def split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon, tbc=None
         elif kind == 3:
             cl = geom.pack_full_clover(torch.from_numpy(ep[gi]).to(DEV), prec, cb)
         res = geom.new_spinor(prec, cb=cb)
-        _run(lib, geom, kind, achi, prec, recon, isign, cb, res, psi, chi, g, cl)
+        if qv is None:
+            _run(lib, geom, kind, achi, prec, recon, isign, cb, res, psi, chi, g, cl)
+        else:
+            ql = qamd.qphix_layout(prec, qv[0], qv[1], recon == 12)
+            psi = qamd.QPhiXSpinor(geom.spinor_to_qphix(psi, ql), ql)
+            chi = qamd.QPhiXSpinor(geom.spinor_to_qphix(chi, ql), ql)
+            res = qamd.QPhiXSpinor(torch.zeros_like(psi.data), ql)
+            _run_q(lib, geom, kind, achi, recon, isign, cb, res, psi, chi, g, cl)
         faces = {}
         for mu in range(4):
             if comm[mu]:
@@ -144,7 +176,10 @@ def split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon, tbc=None
                                      clov=p["cl"], mu=MU, muinv=MUINV)
     out = np.zeros_like(lat["psi"])
     for p in parts.values():
-        loc = p["geom"].unpack_spinor(p["res"]).cpu().numpy()
+        res = p["res"]
+        if qv is not None:
+            res = p["geom"].spinor_from_qphix(res.data, res.layout, prec, cb)
+        loc = p["geom"].unpack_spinor(res).cpu().numpy()
         m = cb_mask(p["geom"].dims, cb)
         out[p["gi"][m]] = loc[m]
     return out
@@ -244,3 +279,26 @@ def test_face_pack_is_the_projected_boundary(qamd, orc):
                     hc = h[fi, :, :, 0] + 1j * h[fi, :, :, 1]
                     # the projected spinor is determined by its upper two spins
                     np.testing.assert_allclose(hc, full[:2], atol=1e-14)
+
+
+@pytest.mark.parametrize("split,ft,qv", [("t2", 1, (8, 4)), ("xyzt16", 0, (4, 2)), ("x2_rows", 1, (16, 16)),
+                                         ("y2_rows", 2, (16, 8)), ("t2", 2, (4, 4))],
+                         ids=["t2_f32_s4", "xyzt16_f64_s2", "x2rows_f32_s16", "y2rows_f16_s8", "t2_f16_s4"])
+def test_split_lattice_in_place_on_qphix_arrays(qamd, split, ft, qv):
+    """A split lattice whose parts keep their spinors in QPhiX
+    FourSpinorBlock arrays: the in-place operators give the interior, the
+    faces are packed from and finished into those arrays in place
+    (qphix_amd_qphix_face_pack, qphix_amd_qphix_*face_finish).  The
+    assembled result has the same bits as the split lattice on native
+    spinors, for every family, dslash and achimbdpsi, both isigns."""
+    G, grid = SPLITS[split]
+    lat = _lattice(qamd, G, 710)
+    prec = (qamd.F64, qamd.F32, qamd.F16)[ft]
+    recon = 12
+    for kind, achi in itertools.product(range(4), (0, 1)):
+        for isign, cb in ((1, 0), (-1, 1)):
+            print(f"case in-place split={split} ft={ft} qv={qv} kind={kind} achi={achi} isign={isign}", flush=True)
+            want = split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon)
+            got = split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon, qv=qv)
+            m = cb_mask(G, cb)
+            assert np.array_equal(got[m], want[m]), (split, kind, achi, isign, cb)
