@@ -30,35 +30,85 @@ struct ForceParams {
 };
 
 // One thread per link U_mu(y), the four links of a site in adjacent lanes:
-// a wave's link and force accesses cover 16 consecutive site records
-// ([4][18] reals each) contiguously.
+// a work-group's kForceSites sites are consecutive, so its link and force
+// records ([4][18] reals per site, link j of the group at reals
+// [18 j, 18 j + 18)) form one contiguous run.  That run moves between
+// memory and LDS in coalesced copies (each load or store instruction of a
+// wave covers 512 or 1024 consecutive bytes); a lane alone would read its
+// 144-byte record in pieces 144 bytes apart from its neighbours'.
 constexpr int kForceSites = kForceBlock / 4;
+constexpr int kForceRec = 18;  // reals of one link record
+
+using f64x2 = double __attribute__((ext_vector_type(2)));
+
+// Coalesced copy of the group's first n reals (n = 18 x valid links) from
+// global memory into LDS; 16-byte pieces when the run is 16-byte aligned.
+__device__ __forceinline__ void force_run_load(double *lds, const double *src, int n)
+{
+  if (((uintptr_t)src & 15) == 0 && (n & 1) == 0) {
+    const f64x2 *s2 = reinterpret_cast<const f64x2 *>(src);
+    f64x2 *l2 = reinterpret_cast<f64x2 *>(lds);
+    for (int i = threadIdx.x; i < n / 2; i += kForceBlock) l2[i] = s2[i];
+  } else {
+    for (int i = threadIdx.x; i < n; i += kForceBlock) lds[i] = src[i];
+  }
+}
+
+// dst[i] += scale * lds[i] for the group's first n reals, coalesced
+__device__ __forceinline__ void force_run_add(double *dst, const double *lds, int n, double scale)
+{
+  if (((uintptr_t)dst & 15) == 0 && (n & 1) == 0) {
+    f64x2 *d2 = reinterpret_cast<f64x2 *>(dst);
+    const f64x2 *l2 = reinterpret_cast<const f64x2 *>(lds);
+    for (int i = threadIdx.x; i < n / 2; i += kForceBlock) {
+      f64x2 o = d2[i];
+      const f64x2 v = l2[i];
+      o.x += scale * v.x;
+      o.y += scale * v.y;
+      d2[i] = o;
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += kForceBlock) dst[i] += scale * lds[i];
+  }
+}
 
 template <typename S>
 __global__ __launch_bounds__(kForceBlock) void force_k(ForceParams p)
 {
+  __shared__ __align__(16) double run[kForceBlock * kForceRec];
   // the lattice swept t-slice by t-slice by all XCDs (neighbour reuse in cache)
   int unused;
   const int mu = threadIdx.x & 3;
   const long y = sweep_site(p.g, 1, unused, kForceSites, threadIdx.x >> 2);
-  if (y < 0) return;
+  // the valid sites of a group are a prefix of it
+  const int nlinks = __syncthreads_count(y >= 0);
+  if (nlinks == 0) return;  // uniform over the group
+  const long y0 = sweep_site(p.g, 1, unused, kForceSites, 0);
   int c[4];
-  long r = y;
+  long r = y < 0 ? 0 : y;  // lanes past the end compute nothing
   c[0] = (int)(r % p.g.Lx);
   r /= p.g.Lx;
   c[1] = (int)(r % p.g.Ly);
   r /= p.g.Ly;
   c[2] = (int)(r % p.g.Lz);
   c[3] = (int)(r / p.g.Lz);
+  const long rec0 = y0 * 4 * kForceRec;
+  const int nreal = nlinks * kForceRec;
+  if (p.project) force_run_load(run, p.U + rec0, nreal);
   const int cbi_y = (((c[3] * p.g.Lz + c[2]) * p.g.Ly + c[1]) * p.g.Lxh) + (c[0] >> 1);
   cx<double> G[3][3];
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++) G[i][j] = {0.0, 0.0};
-  hop_g_add<S>(G, p.term[0], p.k, p.g, c, cbi_y, mu);
-  if (p.nterm > 1) hop_g_add<S>(G, p.term[1], p.k, p.g, c, cbi_y, mu);
-  double *out = p.force + (y * 4 + mu) * 18;
+  if (y >= 0) {
+    hop_g_add<S>(G, p.term[0], p.k, p.g, c, cbi_y, mu);
+    if (p.nterm > 1) hop_g_add<S>(G, p.term[1], p.k, p.g, c, cbi_y, mu);
+  }
+  // this lane's record in the LDS run (the link on entry, the force
+  // contribution on exit)
+  double *rec = run + threadIdx.x * kForceRec;
+  double f[kForceRec];
   if (!p.project) {
     // G^dagger
 #pragma unroll
@@ -66,32 +116,35 @@ __global__ __launch_bounds__(kForceBlock) void force_k(ForceParams p)
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         const cx<double> v = conj(G[j][i]);
-        out[(i * 3 + j) * 2] += p.scale * v.re;
-        out[(i * 3 + j) * 2 + 1] += p.scale * v.im;
+        f[(i * 3 + j) * 2] = v.re;
+        f[(i * 3 + j) * 2 + 1] = v.im;
       }
-    return;
-  }
-  // W = U G, F = (W - W^+)/2 - tr(W - W^+)/6
-  const double *u_ = p.U + (y * 4 + mu) * 18;
-  cx<double> W[3][3];
+  } else {
+    // W = U G, F = (W - W^+)/2 - tr(W - W^+)/6
+    __syncthreads();  // the link run is in LDS
+    cx<double> W[3][3];
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    cx<double> u[3];
+    for (int i = 0; i < 3; i++) {
+      cx<double> u[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) u[k] = {u_[(i * 3 + k) * 2], u_[(i * 3 + k) * 2 + 1]};
+      for (int k = 0; k < 3; k++) u[k] = {rec[(i * 3 + k) * 2], rec[(i * 3 + k) * 2 + 1]};
 #pragma unroll
-    for (int j = 0; j < 3; j++) W[i][j] = u[0] * G[0][j] + u[1] * G[1][j] + u[2] * G[2][j];
-  }
-  const double trim = (W[0][0].im + W[1][1].im + W[2][2].im) / 3.0;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const double re = 0.5 * (W[i][j].re - W[j][i].re);
-      const double im = 0.5 * (W[i][j].im + W[j][i].im) - (i == j ? trim : 0.0);
-      out[(i * 3 + j) * 2] += p.scale * re;
-      out[(i * 3 + j) * 2 + 1] += p.scale * im;
+      for (int j = 0; j < 3; j++) W[i][j] = u[0] * G[0][j] + u[1] * G[1][j] + u[2] * G[2][j];
     }
+    const double trim = (W[0][0].im + W[1][1].im + W[2][2].im) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        f[(i * 3 + j) * 2] = 0.5 * (W[i][j].re - W[j][i].re);
+        f[(i * 3 + j) * 2 + 1] = 0.5 * (W[i][j].im + W[j][i].im) - (i == j ? trim : 0.0);
+      }
+    __syncthreads();  // every lane has read its link
+  }
+#pragma unroll
+  for (int e = 0; e < kForceRec; e++) rec[e] = f[e];
+  __syncthreads();
+  force_run_add(p.force + rec0, run, nreal, p.scale);
 }
 
 template <typename S>
