@@ -43,6 +43,9 @@ inline void check(int rc, const char *what)
   if (rc != QPHIX_AMD_OK) throw Error(rc, what);
 }
 
+/* call C entry point fn and throw its status, named, if it is not OK */
+#define QPHIX_AMD_CALL(fn, ...) ::qphix_amd::check(fn(__VA_ARGS__), #fn)
+
 /* Storage precision of a QPhiX floating type (float16 storage: pass
    QPHIX_AMD_F16 explicitly). */
 template <typename FT> struct Prec;
@@ -55,7 +58,7 @@ class Geometry {
  public:
   Geometry(int Lx, int Ly, int Lz, int Lt, double t_boundary = -1.0) : ctx_(nullptr)
   {
-    check(qphix_amd_ctx_create(&ctx_, Lx, Ly, Lz, Lt, t_boundary), "qphix_amd_ctx_create");
+    QPHIX_AMD_CALL(qphix_amd_ctx_create, &ctx_, Lx, Ly, Lz, Lt, t_boundary);
   }
   ~Geometry()
   {
@@ -84,38 +87,38 @@ class Geometry {
       p[2 * m] = phases[m][0];
       p[2 * m + 1] = phases[m][1];
     }
-    check(qphix_amd_set_tbc(ctx_, t, p), "qphix_amd_set_tbc");
+    QPHIX_AMD_CALL(qphix_amd_set_tbc, ctx_, t, p);
   }
   /* split dimensions of a lattice spread over several GPUs */
   void set_comm(const bool comm[4])
   {
     int c[4];
     for (int m = 0; m < 4; m++) c[m] = comm[m] ? 1 : 0;
-    check(qphix_amd_set_comm(ctx_, c), "qphix_amd_set_comm");
+    QPHIX_AMD_CALL(qphix_amd_set_comm, ctx_, c);
   }
-  void set_traversal(int order) { check(qphix_amd_set_traversal(ctx_, order), "qphix_amd_set_traversal"); }
+  void set_traversal(int order) { QPHIX_AMD_CALL(qphix_amd_set_traversal, ctx_, order); }
   int traversal() const { return qphix_amd_traversal(ctx_); }
 
   /* device lexicographic FP64 <-> native (see qphix_amd.h for the layouts) */
   void pack_spinor(void *dst, int prec, const double *lex, int cb, void *stream = nullptr) const
   {
-    check(qphix_amd_pack_spinor(ctx_, dst, prec, lex, cb, stream), "qphix_amd_pack_spinor");
+    QPHIX_AMD_CALL(qphix_amd_pack_spinor, ctx_, dst, prec, lex, cb, stream);
   }
   void unpack_spinor(double *lex, const void *src, int prec, int cb, void *stream = nullptr) const
   {
-    check(qphix_amd_unpack_spinor(ctx_, lex, src, prec, cb, stream), "qphix_amd_unpack_spinor");
+    QPHIX_AMD_CALL(qphix_amd_unpack_spinor, ctx_, lex, src, prec, cb, stream);
   }
   void pack_gauge(void *dst, int prec, int recon, const double *u_lex, int cb, void *stream = nullptr) const
   {
-    check(qphix_amd_pack_gauge(ctx_, dst, prec, recon, u_lex, cb, stream), "qphix_amd_pack_gauge");
+    QPHIX_AMD_CALL(qphix_amd_pack_gauge, ctx_, dst, prec, recon, u_lex, cb, stream);
   }
   void pack_clover(void *dst, int prec, const double *clov_lex, int cb, void *stream = nullptr) const
   {
-    check(qphix_amd_pack_clover(ctx_, dst, prec, clov_lex, cb, stream), "qphix_amd_pack_clover");
+    QPHIX_AMD_CALL(qphix_amd_pack_clover, ctx_, dst, prec, clov_lex, cb, stream);
   }
   void pack_full_clover(void *dst, int prec, const double *fclov_lex, int cb, void *stream = nullptr) const
   {
-    check(qphix_amd_pack_full_clover(ctx_, dst, prec, fclov_lex, cb, stream), "qphix_amd_pack_full_clover");
+    QPHIX_AMD_CALL(qphix_amd_pack_full_clover, ctx_, dst, prec, fclov_lex, cb, stream);
   }
 
   /* clover term from links and its inverse / twisted-mass blocks (device
@@ -123,46 +126,43 @@ class Geometry {
   void clover_from_links(double *clov_lex, const double *u_lex, double alpha, double csw,
                          void *stream = nullptr) const
   {
-    check(qphix_amd_clover_from_links(ctx_, clov_lex, u_lex, alpha, csw, stream), "qphix_amd_clover_from_links");
+    QPHIX_AMD_CALL(qphix_amd_clover_from_links, ctx_, clov_lex, u_lex, alpha, csw, stream);
   }
   void clover_invert(double *inv_lex, const double *clov_lex, void *stream = nullptr) const
   {
-    check(qphix_amd_clover_invert(ctx_, inv_lex, clov_lex, stream), "qphix_amd_clover_invert");
+    QPHIX_AMD_CALL(qphix_amd_clover_invert, ctx_, inv_lex, clov_lex, stream);
   }
   void tmclover_blocks(double *out_lex, const double *clov_lex, double mu, int sign, bool invert,
                        void *stream = nullptr) const
   {
-    check(qphix_amd_tmclover_blocks(ctx_, out_lex, clov_lex, mu, sign, invert ? 1 : 0, stream),
-          "qphix_amd_tmclover_blocks");
+    QPHIX_AMD_CALL(qphix_amd_tmclover_blocks, ctx_, out_lex, clov_lex, mu, sign, invert ? 1 : 0, stream);
   }
 
   /* device QPhiX block arrays -> native */
   void spinor_from_qphix(const qphix_amd_qphix_layout &lay, void *dst, int prec, const void *src, int cb,
                          void *stream = nullptr) const
   {
-    check(qphix_amd_spinor_from_qphix(ctx_, &lay, dst, prec, src, cb, stream), "qphix_amd_spinor_from_qphix");
+    QPHIX_AMD_CALL(qphix_amd_spinor_from_qphix, ctx_, &lay, dst, prec, src, cb, stream);
   }
   void spinor_to_qphix(const qphix_amd_qphix_layout &lay, void *dst, const void *src, int prec, int cb,
                        void *stream = nullptr) const
   {
-    check(qphix_amd_spinor_to_qphix(ctx_, &lay, dst, src, prec, cb, stream), "qphix_amd_spinor_to_qphix");
+    QPHIX_AMD_CALL(qphix_amd_spinor_to_qphix, ctx_, &lay, dst, src, prec, cb, stream);
   }
   void gauge_from_qphix(const qphix_amd_qphix_layout &lay, void *dst, int prec, int recon, const void *src,
                         int cb, void *stream = nullptr) const
   {
-    check(qphix_amd_gauge_from_qphix(ctx_, &lay, dst, prec, recon, src, cb, stream),
-          "qphix_amd_gauge_from_qphix");
+    QPHIX_AMD_CALL(qphix_amd_gauge_from_qphix, ctx_, &lay, dst, prec, recon, src, cb, stream);
   }
   void clover_from_qphix(const qphix_amd_qphix_layout &lay, void *dst, int prec, const void *src, int cb,
                          void *stream = nullptr) const
   {
-    check(qphix_amd_clover_from_qphix(ctx_, &lay, dst, prec, src, cb, stream), "qphix_amd_clover_from_qphix");
+    QPHIX_AMD_CALL(qphix_amd_clover_from_qphix, ctx_, &lay, dst, prec, src, cb, stream);
   }
   void full_clover_from_qphix(const qphix_amd_qphix_layout &lay, void *dst, int prec, const void *src, int cb,
                               void *stream = nullptr) const
   {
-    check(qphix_amd_full_clover_from_qphix(ctx_, &lay, dst, prec, src, cb, stream),
-          "qphix_amd_full_clover_from_qphix");
+    QPHIX_AMD_CALL(qphix_amd_full_clover_from_qphix, ctx_, &lay, dst, prec, src, cb, stream);
   }
   /* bytes of a QPhiX block array of this lattice in layout lay */
   size_t qphix_field_bytes(const qphix_amd_qphix_layout &lay, int field) const
@@ -178,14 +178,14 @@ class Geometry {
   void face_pack(int prec, int isign, int cb, int dim, int fb, void *buf, const void *in,
                  void *stream = nullptr) const
   {
-    check(qphix_amd_face_pack(ctx_, prec, isign, cb, dim, fb, buf, in, stream), "qphix_amd_face_pack");
+    QPHIX_AMD_CALL(qphix_amd_face_pack, ctx_, prec, isign, cb, dim, fb, buf, in, stream);
   }
   /* the same from a QPhiX FourSpinorBlock array `in` of layout lay, read in
      place */
   void face_pack(const qphix_amd_qphix_layout &lay, int isign, int cb, int dim, int fb, void *buf, const void *in,
                  void *stream = nullptr) const
   {
-    check(qphix_amd_qphix_face_pack(ctx_, &lay, isign, cb, dim, fb, buf, in, stream), "qphix_amd_qphix_face_pack");
+    QPHIX_AMD_CALL(qphix_amd_qphix_face_pack, ctx_, &lay, isign, cb, dim, fb, buf, in, stream);
   }
 
   /* differentiation of L = Re<chi, D psi> (no QPhiX counterpart): the
@@ -195,17 +195,15 @@ class Geometry {
                       const void *gauge, double coeff_s = 1.0, double coeff_t_f = 1.0, double coeff_t_b = 1.0,
                       void *stream = nullptr) const
   {
-    check(qphix_amd_dslash_adjoint(ctx_, prec, recon, isign, cb, grad_psi, chi, gauge, coeff_s, coeff_t_f,
-                                   coeff_t_b, stream),
-          "qphix_amd_dslash_adjoint");
+    QPHIX_AMD_CALL(qphix_amd_dslash_adjoint, ctx_, prec, recon, isign, cb, grad_psi, chi, gauge, coeff_s,
+                   coeff_t_f, coeff_t_b, stream);
   }
   void dslash_force(int prec, int isign, int cb, double *force_lex, const void *chi, const void *psi,
                     const double *u_lex, double scale = 1.0, bool project = true, double coeff_s = 1.0,
                     double coeff_t_f = 1.0, double coeff_t_b = 1.0, void *stream = nullptr) const
   {
-    check(qphix_amd_dslash_force(ctx_, prec, isign, cb, force_lex, chi, psi, u_lex, coeff_s, coeff_t_f, coeff_t_b,
-                                 scale, project ? 1 : 0, stream),
-          "qphix_amd_dslash_force");
+    QPHIX_AMD_CALL(qphix_amd_dslash_force, ctx_, prec, isign, cb, force_lex, chi, psi, u_lex, coeff_s, coeff_t_f,
+                   coeff_t_b, scale, project ? 1 : 0, stream);
   }
   /* link gradient of Re sum_x w(x) chi^dag A psi, A = clover_from_links(.., csw),
      w = scale_even / scale_odd by site parity (lexicographic FP64 fields) */
@@ -213,14 +211,13 @@ class Geometry {
                     double csw, double scale_even = 1.0, double scale_odd = 1.0, bool project = true,
                     void *stream = nullptr) const
   {
-    check(qphix_amd_clover_force(ctx_, force_lex, chi_lex, psi_lex, u_lex, csw, scale_even, scale_odd,
-                                 project ? 1 : 0, stream),
-          "qphix_amd_clover_force");
+    QPHIX_AMD_CALL(qphix_amd_clover_force, ctx_, force_lex, chi_lex, psi_lex, u_lex, csw, scale_even, scale_odd,
+                   project ? 1 : 0, stream);
   }
   /* project lexicographic FP64 links back onto SU(3), in place */
   void reunitarize(double *u_lex, void *stream = nullptr) const
   {
-    check(qphix_amd_reunitarize(ctx_, u_lex, stream), "qphix_amd_reunitarize");
+    QPHIX_AMD_CALL(qphix_amd_reunitarize, ctx_, u_lex, stream);
   }
 
  private:
@@ -261,23 +258,20 @@ class Dslash {
   void dslash(void *res, const void *psi, const void *u, int isign, int cb) const
   {
     if (q_)
-      check(qphix_amd_qphix_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, acc_, cs_, ct_, ct_, stream_),
-            "qphix_amd_qphix_dslash");
+      QPHIX_AMD_CALL(qphix_amd_qphix_dslash, ctx_, &lay_, recon_, isign, cb, res, psi, u, acc_, cs_, ct_, ct_,
+                     stream_);
     else
-      check(qphix_amd_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, stream_),
-            "qphix_amd_dslash");
+      QPHIX_AMD_CALL(qphix_amd_dslash, ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, stream_);
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, double alpha,
                             double beta, int isign, int cb) const
   {
     if (q_)
-      check(qphix_amd_qphix_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, acc_, alpha,
-                                              beta * cs_, beta * ct_, beta * ct_, stream_),
-            "qphix_amd_qphix_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_qphix_dslash_achimbdpsi, ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, acc_,
+                     alpha, beta * cs_, beta * ct_, beta * ct_, stream_);
     else
-      check(qphix_amd_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha, beta * cs_,
-                                        beta * ct_, beta * ct_, stream_),
-            "qphix_amd_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_dslash_achimbdpsi, ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha,
+                     beta * cs_, beta * ct_, beta * ct_, stream_);
   }
   /* complete the sites of face (dim, fb) from a received face buffer:
      after dslash (beta = -coeff; each family has its own), and after
@@ -291,11 +285,10 @@ class Dslash {
                               int fb) const
   {
     if (q_)
-      check(qphix_amd_qphix_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_),
-            "qphix_amd_qphix_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_qphix_face_finish, ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, beta,
+                     stream_);
     else
-      check(qphix_amd_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_),
-            "qphix_amd_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_face_finish, ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, stream_);
   }
 
  protected:
@@ -318,37 +311,32 @@ class ClovDslash : public Dslash {
   void dslash(void *res, const void *psi, const void *u, const void *invclov, int isign, int cb) const
   {
     if (q_)
-      check(qphix_amd_qphix_clov_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, invclov, acc_, cs_, ct_, ct_,
-                                        stream_),
-            "qphix_amd_qphix_clov_dslash");
+      QPHIX_AMD_CALL(qphix_amd_qphix_clov_dslash, ctx_, &lay_, recon_, isign, cb, res, psi, u, invclov, acc_, cs_,
+                     ct_, ct_, stream_);
     else
-      check(qphix_amd_clov_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, invclov, cs_, ct_, ct_, stream_),
-            "qphix_amd_clov_dslash");
+      QPHIX_AMD_CALL(qphix_amd_clov_dslash, ctx_, prec_, recon_, isign, cb, res, psi, u, invclov, cs_, ct_, ct_,
+                     stream_);
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, const void *clov,
                             double beta, int isign, int cb, double prec_mass_rho = 0.0) const
   {
     if (q_)
-      check(qphix_amd_qphix_clov_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, clov, acc_,
-                                                   beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_),
-            "qphix_amd_qphix_clov_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_qphix_clov_dslash_achimbdpsi, ctx_, &lay_, recon_, isign, cb, res, psi, chi, u,
+                     clov, acc_, beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_);
     else
-      check(qphix_amd_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, clov, beta * cs_,
-                                             beta * ct_, beta * ct_, prec_mass_rho, stream_),
-            "qphix_amd_clov_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_clov_dslash_achimbdpsi, ctx_, prec_, recon_, isign, cb, res, psi, chi, u, clov,
+                     beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_);
   }
   /* face completion of dslash (beta = +coeff) */
   void face_finish_dslash(void *res, const void *buf, const void *u, const void *invclov, double beta, int isign,
                           int cb, int dim, int fb) const
   {
     if (q_)
-      check(qphix_amd_qphix_clov_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, invclov, beta,
-                                             stream_),
-            "qphix_amd_qphix_clov_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_qphix_clov_face_finish, ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u,
+                     invclov, beta, stream_);
     else
-      check(qphix_amd_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invclov, beta,
-                                       stream_),
-            "qphix_amd_clov_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_clov_face_finish, ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invclov,
+                     beta, stream_);
   }
 };
 
@@ -372,24 +360,21 @@ class TMDslash : public Dslash {
   {
     const double mu = mu_phys_ / alpha_, muinv = alpha_ / (alpha_ * alpha_ + mu_phys_ * mu_phys_);
     if (q_)
-      check(qphix_amd_qphix_tm_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, acc_, cs_, ct_, ct_, mu, muinv,
-                                      stream_),
-            "qphix_amd_qphix_tm_dslash");
+      QPHIX_AMD_CALL(qphix_amd_qphix_tm_dslash, ctx_, &lay_, recon_, isign, cb, res, psi, u, acc_, cs_, ct_, ct_,
+                     mu, muinv, stream_);
     else
-      check(qphix_amd_tm_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, mu, muinv, stream_),
-            "qphix_amd_tm_dslash");
+      QPHIX_AMD_CALL(qphix_amd_tm_dslash, ctx_, prec_, recon_, isign, cb, res, psi, u, cs_, ct_, ct_, mu, muinv,
+                     stream_);
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, double alpha,
                             double beta, int isign, int cb) const
   {
     if (q_)
-      check(qphix_amd_qphix_tm_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, acc_, alpha,
-                                                 beta * cs_, beta * ct_, beta * ct_, mu_phys_ / alpha, stream_),
-            "qphix_amd_qphix_tm_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_qphix_tm_dslash_achimbdpsi, ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, acc_,
+                     alpha, beta * cs_, beta * ct_, beta * ct_, mu_phys_ / alpha, stream_);
     else
-      check(qphix_amd_tm_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha, beta * cs_,
-                                           beta * ct_, beta * ct_, mu_phys_ / alpha, stream_),
-            "qphix_amd_tm_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_tm_dslash_achimbdpsi, ctx_, prec_, recon_, isign, cb, res, psi, chi, u, alpha,
+                     beta * cs_, beta * ct_, beta * ct_, mu_phys_ / alpha, stream_);
   }
   /* face completion of dslash (beta = -coeff), with the twisted epilogue */
   void face_finish_dslash(void *res, const void *buf, const void *u, double beta, int isign, int cb, int dim,
@@ -397,13 +382,11 @@ class TMDslash : public Dslash {
   {
     const double mu = mu_phys_ / alpha_, muinv = alpha_ / (alpha_ * alpha_ + mu_phys_ * mu_phys_);
     if (q_)
-      check(qphix_amd_qphix_tm_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu, muinv,
-                                           stream_),
-            "qphix_amd_qphix_tm_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_qphix_tm_face_finish, ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, beta,
+                     mu, muinv, stream_);
     else
-      check(qphix_amd_tm_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu, muinv,
-                                     stream_),
-            "qphix_amd_tm_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_tm_face_finish, ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, beta, mu,
+                     muinv, stream_);
   }
 
  private:
@@ -420,39 +403,32 @@ class TMClovDslash : public Dslash {
   void dslash(void *res, const void *psi, const void *u, const void *invfclov, int isign, int cb) const
   {
     if (q_)
-      check(qphix_amd_qphix_tm_clov_dslash(ctx_, &lay_, recon_, isign, cb, res, psi, u, invfclov, acc_, cs_, ct_,
-                                           ct_, stream_),
-            "qphix_amd_qphix_tm_clov_dslash");
+      QPHIX_AMD_CALL(qphix_amd_qphix_tm_clov_dslash, ctx_, &lay_, recon_, isign, cb, res, psi, u, invfclov, acc_,
+                     cs_, ct_, ct_, stream_);
     else
-      check(qphix_amd_tm_clov_dslash(ctx_, prec_, recon_, isign, cb, res, psi, u, invfclov, cs_, ct_, ct_,
-                                     stream_),
-            "qphix_amd_tm_clov_dslash");
+      QPHIX_AMD_CALL(qphix_amd_tm_clov_dslash, ctx_, prec_, recon_, isign, cb, res, psi, u, invfclov, cs_, ct_,
+                     ct_, stream_);
   }
   void dslashAChiMinusBDPsi(void *res, const void *psi, const void *chi, const void *u, const void *fclov,
                             double beta, int isign, int cb, double prec_mass_rho = 0.0) const
   {
     if (q_)
-      check(qphix_amd_qphix_tm_clov_dslash_achimbdpsi(ctx_, &lay_, recon_, isign, cb, res, psi, chi, u, fclov,
-                                                      acc_, beta * cs_, beta * ct_, beta * ct_, prec_mass_rho,
-                                                      stream_),
-            "qphix_amd_qphix_tm_clov_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_qphix_tm_clov_dslash_achimbdpsi, ctx_, &lay_, recon_, isign, cb, res, psi, chi, u,
+                     fclov, acc_, beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_);
     else
-      check(qphix_amd_tm_clov_dslash_achimbdpsi(ctx_, prec_, recon_, isign, cb, res, psi, chi, u, fclov,
-                                                beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_),
-            "qphix_amd_tm_clov_dslash_achimbdpsi");
+      QPHIX_AMD_CALL(qphix_amd_tm_clov_dslash_achimbdpsi, ctx_, prec_, recon_, isign, cb, res, psi, chi, u, fclov,
+                     beta * cs_, beta * ct_, beta * ct_, prec_mass_rho, stream_);
   }
   /* face completion of dslash (beta = +coeff) */
   void face_finish_dslash(void *res, const void *buf, const void *u, const void *invfclov, double beta, int isign,
                           int cb, int dim, int fb) const
   {
     if (q_)
-      check(qphix_amd_qphix_tm_clov_face_finish(ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u, invfclov,
-                                                beta, stream_),
-            "qphix_amd_qphix_tm_clov_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_qphix_tm_clov_face_finish, ctx_, &lay_, recon_, isign, cb, dim, fb, res, buf, u,
+                     invfclov, beta, stream_);
     else
-      check(qphix_amd_tm_clov_face_finish(ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invfclov, beta,
-                                          stream_),
-            "qphix_amd_tm_clov_face_finish");
+      QPHIX_AMD_CALL(qphix_amd_tm_clov_face_finish, ctx_, prec_, recon_, isign, cb, dim, fb, res, buf, u, invfclov,
+                     beta, stream_);
   }
 };
 
@@ -466,8 +442,7 @@ class EvenOddOperator {
                   int recon_sloppy, void *stream = nullptr)
       : op_(nullptr), stream_(stream)
   {
-    check(qphix_amd_eo_op_create(&op_, geom.ctx(), u_lex, clov_lex, mu, prec_sloppy, recon_sloppy, stream),
-          "qphix_amd_eo_op_create");
+    QPHIX_AMD_CALL(qphix_amd_eo_op_create, &op_, geom.ctx(), u_lex, clov_lex, mu, prec_sloppy, recon_sloppy, stream);
   }
   ~EvenOddOperator()
   {
@@ -480,7 +455,7 @@ class EvenOddOperator {
   /* out = M in (dagger false) or M^dagger in; prec F64 or the sloppy one */
   void apply(void *out, const void *in, int prec, bool dagger = false) const
   {
-    check(qphix_amd_eo_op_apply(op_, prec, dagger ? 1 : 0, out, in, stream_), "qphix_amd_eo_op_apply");
+    QPHIX_AMD_CALL(qphix_amd_eo_op_apply, op_, prec, dagger ? 1 : 0, out, in, stream_);
   }
   /* solve M^dag M x = b (FP64 native spinors on cb 1); throws on failure,
      including QPHIX_AMD_ENOCONV when max_iters is reached */
@@ -493,7 +468,7 @@ class EvenOddOperator {
     p.delta = delta;
     p.mixed = mixed ? 1 : 0;
     qphix_amd_cg_result r;
-    check(qphix_amd_cg_solve(op_, x, b, &p, &r, stream_), "qphix_amd_cg_solve");
+    QPHIX_AMD_CALL(qphix_amd_cg_solve, op_, x, b, &p, &r, stream_);
     return r;
   }
   /* accumulate scale * dS/dU of S = phi^dag (M^dag M)^-1 phi at
@@ -503,15 +478,14 @@ class EvenOddOperator {
   void force(double *force_lex, const void *x, const double *u_lex, double scale = 1.0,
              bool project = true) const
   {
-    check(qphix_amd_eo_op_force(op_, force_lex, x, u_lex, scale, project ? 1 : 0, stream_), "qphix_amd_eo_op_force");
+    QPHIX_AMD_CALL(qphix_amd_eo_op_force, op_, force_lex, x, u_lex, scale, project ? 1 : 0, stream_);
   }
   /* the same with the clover term, for an operator built on
      clover_from_links(.., csw) of the same links */
   void force_clover(double *force_lex, const void *x, const double *u_lex, double csw, double scale = 1.0,
                     bool project = true) const
   {
-    check(qphix_amd_eo_op_force_clover(op_, force_lex, x, u_lex, csw, scale, project ? 1 : 0, stream_),
-          "qphix_amd_eo_op_force_clover");
+    QPHIX_AMD_CALL(qphix_amd_eo_op_force_clover, op_, force_lex, x, u_lex, csw, scale, project ? 1 : 0, stream_);
   }
 
  private:
@@ -520,5 +494,7 @@ class EvenOddOperator {
 };
 
 }  // namespace qphix_amd
+
+#undef QPHIX_AMD_CALL
 
 #endif /* QPHIX_AMD_HPP */
