@@ -3,10 +3,10 @@
 // per-call hipMalloc, for callers that cannot allocate device memory (code
 // under HIP graph capture, tight HMC loops, harnesses that forbid it).
 //
-// The workspace is the field of clover weights K, 6 planes x 18 reals per
-// site (component-major, K[(plane * 18 + e) * V + site]), written in full
-// by the first kernel of the call before the second reads it, so nothing
-// in it needs clearing between calls.  The array is QPHIX_AMD_NOALLOC_
+// The workspace is the field of clover weights K, 6 planes x 8 reals per
+// site (kKSiteReals = 48; component-major, K[(plane * 8 + e) * V + site]),
+// written in full by the first kernel of the call before the second reads
+// it, so nothing in it needs clearing between calls.  The array is QPHIX_AMD_NOALLOC_
 // SCRATCH_BYTES long on every GPU the library runs on and is held until the
 // process ends.  Every call on one GPU uses that GPU's array: calls that
 // overlap on one GPU (two streams, two threads) race on it.
