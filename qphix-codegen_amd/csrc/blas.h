@@ -19,6 +19,7 @@ struct CGScalars {
   // rest < delta rmax or rest^2 < target2 (mixed; plain CG: rest^2 <
   // target2) or rr is not finite, else rmax = max(rmax, rest).
   double sigma, rmax, delta, target2;
+  double one;  // 1: the step of the true residual b - 1 * M^dag M x
   int mixed;
   int stop;   // set: the iteration kernels still queued return at once
   int iters;  // iterations completed since the host last set the scalars

@@ -45,9 +45,11 @@ template <typename R> struct DslashParams {
   // when set: per work-group sums of |out|^2 over the stored values, at
   // norm[first site / kDslashBlock] (every work-group full)
   double *norm;
-  // when set (with norm): out holds a CG residual r, and the launch stores
-  // r - (*step) * result in its place; norm then sums |r|^2
+  // when set (with norm): the launch stores r - (*step) * result in out,
+  // r read from rin (a CG residual in out itself when rin == out, or a
+  // right-hand side b elsewhere); norm then sums the stored |r|^2
   const double *step;
+  const void *rin;
   // when set and *stop != 0 the launch does nothing (CG iterations queued
   // past a reliable update)
   const int *stop;
@@ -1055,9 +1057,10 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       o[(s * 3 + k) * 2 + 1] = res[s][k].im;
     }
   if constexpr (EPI == EPI_RESIDUAL) {
-    // r - step * result; each site reads and writes only its own r
+    // r - step * result; each site reads only its own r (in place: its own
+    // element of out)
     R rv[24];
-    load_reals<S, spinor_w<S>(), 0, 24, R>(static_cast<const S *>(p.out), vcb, idx, rv);
+    load_reals<S, spinor_w<S>(), 0, 24, R>(static_cast<const S *>(p.rin), vcb, idx, rv);
     const R a = (R)*p.step;
 #pragma unroll
     for (int k = 0; k < 24; k++) o[k] = rv[k] - a * o[k];

@@ -26,8 +26,10 @@ struct DslashCall {
   // vcb is a multiple of 128)
   double *norm;
   // optional, with norm, for the clover achimbdpsi kernels with isign -1:
-  // out holds a residual r and receives r - (*step) * result
+  // out receives r - (*step) * result, with r read from rin (null: out
+  // itself holds r)
   const double *step;
+  const void *rin;
   // optional (CG iterations): when it points to a set flag the launch does
   // nothing
   const int *stop;

@@ -100,8 +100,9 @@ int level_prec(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? QPHIX_AMD
 int level_recon(const qphix_amd_eo_op *op, int lvl) { return lvl == 0 ? 12 : op->recon_s; }
 
 // norm: when set, per work-group partials of |out|^2 (vcb / 128 of them);
-// step: when set too (dagger only), out holds a residual r, which receives
-// r - (*step) M^dag in, and norm gets the partials of its |r|^2
+// step: when set too (dagger only), out receives r - (*step) M^dag in, with
+// r read from rin (null: out itself holds r), and norm gets the partials of
+// that |r|^2
 // first stage of M: out (cb 0) = F_ee^-1 D_eo in (dagger: the dagger blocks
 // and isign -1)
 hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
@@ -125,7 +126,7 @@ hipError_t eo_stage1(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
 // second stage: out (cb 1) = F_oo in - 1/4 D_oe t, t the first stage's result
 hipError_t eo_stage2(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *t, const void *in,
                      hipStream_t st, double *norm = nullptr, const double *step = nullptr,
-                     const int *stop = nullptr)
+                     const int *stop = nullptr, const void *rin = nullptr)
 {
   DslashCall c{};
   c.stop = stop;
@@ -144,17 +145,18 @@ hipError_t eo_stage2(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const 
   c.rho = 0.0;
   c.norm = norm;
   c.step = step;
+  c.rin = rin;
   return run_dslash_any(op->ctx, c, st);
 }
 
 // out = M in (dagger: M^dag), through the operator's cb-0 temporary
 hipError_t eo_apply(qphix_amd_eo_op *op, int lvl, int dagger, void *out, const void *in,
                     hipStream_t st, double *norm = nullptr, const double *step = nullptr,
-                    const int *stop = nullptr)
+                    const int *stop = nullptr, const void *rin = nullptr)
 {
   hipError_t e = eo_stage1(op, lvl, dagger, op->tmp[lvl].p, in, st, stop);
   if (e != hipSuccess) return e;
-  return eo_stage2(op, lvl, dagger, out, op->tmp[lvl].p, in, st, norm, step, stop);
+  return eo_stage2(op, lvl, dagger, out, op->tmp[lvl].p, in, st, norm, step, stop, rin);
 }
 
 // out = M^dag M in, via the cb-1 temporary mid = M in
@@ -424,7 +426,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   DevBuf &rs = op->wrs, &p = op->wp, &ap = op->wap, &xs = op->wxs;
   DevBuf &scb = op->wscb, &part = op->wpart, &pnorm = op->wpnorm;
   if (fuse && pnorm.reserve(nwg * 8) != hipSuccess) return QPHIX_AMD_ENOMEM;
-  if (r64.reserve(n * 8) != hipSuccess || ax64.reserve(n * 8) != hipSuccess ||
+  if (r64.reserve(n * 8) != hipSuccess || (!fuse && ax64.reserve(n * 8) != hipSuccess) ||
       rs.reserve(n * sbytes(ps)) != hipSuccess || p.reserve(n * sbytes(ps)) != hipSuccess ||
       (!fuse && ap.reserve(n * sbytes(ps)) != hipSuccess) || xs.reserve(n * sbytes(px)) != hipSuccess ||
       mid.reserve(n * (size_t)sbytes(QPHIX_AMD_F64)) != hipSuccess ||
@@ -453,17 +455,29 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
     h.delta = delta;
     h.mixed = mixed ? 1 : 0;
     h.target2 = target2;
+    h.one = 1.0;
     blas_set_scalars(sc, h, st);
     return hipGetLastError();
   };
 
   // r64 = b - M^dag M x64, returns |r64|^2 in *r2
   auto true_residual = [&](double *r2) -> hipError_t {
-    hipError_t e = eo_normal(op, 0, ax64.p, mid.p, x, st);
-    if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(r64.p, b, n * 8, hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) return e;
-    blas_axpby(QPHIX_AMD_F64, ax64.p, QPHIX_AMD_F64, r64.p, -1.0, 1.0, n, partial, sc, st);
+    hipError_t e;
+    if (fuse) {
+      // the operator's last kernel stores b - 1 * M^dag (M x64) into r64 and
+      // sums its |r64|^2 per work-group (M^dag M x64 is never stored)
+      e = eo_apply(op, 0, 0, mid.p, x, st);
+      if (e == hipSuccess)
+        e = eo_apply(op, 0, 1, r64.p, mid.p, st, (double *)pnorm.p, &sc->one, nullptr, b);
+      if (e != hipSuccess) return e;
+      blas_finalize((const double *)pnorm.p, nwg, sc, 2, partial, st);
+    } else {
+      e = eo_normal(op, 0, ax64.p, mid.p, x, st);
+      if (e != hipSuccess) return e;
+      e = hipMemcpyAsync(r64.p, b, n * 8, hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) return e;
+      blas_axpby(QPHIX_AMD_F64, ax64.p, QPHIX_AMD_F64, r64.p, -1.0, 1.0, n, partial, sc, st);
+    }
     e = fetch();
     *r2 = hsc->tmp;
     return e;
