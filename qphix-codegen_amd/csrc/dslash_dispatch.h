@@ -71,7 +71,7 @@ hipError_t run_dslash_typed(const qphix_amd_ctx *ctx, const DslashCall &c, hipSt
   p.one = (R)1;
   p.norm = c.norm;
   p.step = c.step;
-  p.rin = c.rin ? c.rin : c.out;
+  p.rin = c.rin;
   p.stop = c.stop;
   p.q_sh = c.q_sh;
   p.q_nvecs = c.q_nvecs;

@@ -45,11 +45,9 @@ template <typename R> struct DslashParams {
   // when set: per work-group sums of |out|^2 over the stored values, at
   // norm[first site / kDslashBlock] (every work-group full)
   double *norm;
-  // when set (with norm): the launch stores r - (*step) * result in out,
-  // r read from rin (a CG residual in out itself when rin == out, or a
-  // right-hand side b elsewhere); norm then sums the stored |r|^2
+  // when set (with norm): out holds a CG residual r, and the launch stores
+  // r - (*step) * result in its place; norm then sums |r|^2
   const double *step;
-  const void *rin;
   // when set and *stop != 0 the launch does nothing (CG iterations queued
   // past a reliable update)
   const int *stop;
@@ -58,12 +56,16 @@ template <typename R> struct DslashParams {
   int q_sh, q_nvecs, q_pxy, q_pxyz;
   // per-site direction mask (bit d = 2 mu + fb), QPhiX site order, or null
   const unsigned char *acc;
+  // EPI_TRUE_RESIDUAL: the right-hand side b of out = b - (*step) * result
+  const void *rin;
 };
 
 // What a launch does with its result at a site: store it; store it and sum
-// |out|^2 per work-group; or subtract it, times the device-resident CG step
-// length, from the residual held in out, and sum |r|^2 per work-group.
-enum { EPI_STORE = 0, EPI_NORM = 1, EPI_RESIDUAL = 2 };
+// |out|^2 per work-group; subtract it, times the device-resident CG step
+// length, from the residual held in out, and sum |r|^2 per work-group; or
+// (FP64 only, the CG's true residual b - M^dag M x) store b - step * result
+// with b read from rin, and sum its |.|^2.
+enum { EPI_STORE = 0, EPI_NORM = 1, EPI_RESIDUAL = 2, EPI_TRUE_RESIDUAL = 3 };
 
 #ifndef QAMD_XCD_REMAP
 #define QAMD_XCD_REMAP 1
@@ -1056,11 +1058,12 @@ __global__ __launch_bounds__(kDslashBlock) QAMD_WAVES_ATTR void dslash_kernel(Ds
       o[(s * 3 + k) * 2] = res[s][k].re;
       o[(s * 3 + k) * 2 + 1] = res[s][k].im;
     }
-  if constexpr (EPI == EPI_RESIDUAL) {
-    // r - step * result; each site reads only its own r (in place: its own
-    // element of out)
+  if constexpr (EPI == EPI_RESIDUAL || EPI == EPI_TRUE_RESIDUAL) {
+    // r - step * result; each site reads and writes only its own r (b for
+    // the true residual)
     R rv[24];
-    load_reals<S, spinor_w<S>(), 0, 24, R>(static_cast<const S *>(p.rin), vcb, idx, rv);
+    load_reals<S, spinor_w<S>(), 0, 24, R>(
+        static_cast<const S *>(EPI == EPI_RESIDUAL ? p.out : p.rin), vcb, idx, rv);
     const R a = (R)*p.step;
 #pragma unroll
     for (int k = 0; k < 24; k++) o[k] = rv[k] - a * o[k];
@@ -1114,10 +1117,17 @@ hipError_t launch_dslash(const DslashParams<typename Traits<S>::R> &p, hipStream
     }
     // the residual update follows M^dag (isign -1), the second half of M^dag M
     if constexpr (ISIGN == -1) {
-      if (p.norm && p.step) {
+      if (p.norm && p.step && !p.rin) {
         hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_RESIDUAL>), dim3(grid),
                            dim3(kDslashBlock), 0, stream, p);
         return hipGetLastError();
+      }
+      if constexpr (std::is_same<S, double>::value) {
+        if (p.norm && p.step && p.rin) {
+          hipLaunchKernelGGL((dslash_kernel<S, RECON, OP, ACHI, ISIGN, EPI_TRUE_RESIDUAL>), dim3(grid),
+                             dim3(kDslashBlock), 0, stream, p);
+          return hipGetLastError();
+        }
       }
     }
   }

@@ -26,8 +26,9 @@ struct DslashCall {
   // vcb is a multiple of 128)
   double *norm;
   // optional, with norm, for the clover achimbdpsi kernels with isign -1:
-  // out receives r - (*step) * result, with r read from rin (null: out
-  // itself holds r)
+  // out holds a residual r and receives r - (*step) * result; with rin set
+  // too (FP64 only), out receives rin - (*step) * result instead (the CG's
+  // true residual b - M^dag M x)
   const double *step;
   const void *rin;
   // optional (CG iterations): when it points to a set flag the launch does
