@@ -48,6 +48,35 @@ def test_force_kernel_vs_oracle(qamd, orc, small, prec, tol):
                 assert err < tol, (isign, cb, project, err)
 
 
+@pytest.mark.parametrize("offset", [0, 1], ids=["aligned", "offset8"])
+def test_force_kernel_partial_groups_vs_oracle(qamd, orc, offset):
+    """6^4: 1296 sites, so the last work-group of 32 sites is half full and
+    Lz is not a multiple of 8 (the other traversal); offset8 passes force
+    and link arrays 8 bytes past a 16-byte boundary, so the work-groups'
+    link and force runs move element by element."""
+    dims = (6, 6, 6, 6)
+    print(f"case force partial groups {dims} offset={offset}", flush=True)
+    L = qamd.lattice
+    U, chi, psi = L.random_gauge(dims, 0.4, 64), L.random_spinor(dims, 65), L.random_spinor(dims, 66)
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    ubuf = torch.zeros(U.numel() + 2, dtype=torch.float64, device=DEV)
+    Ud = ubuf[offset:offset + U.numel()].view(U.shape)
+    Ud.copy_(U.to(DEV))
+    assert (Ud.data_ptr() % 16 == 0) == (offset == 0)
+    for isign, cb, project in ((1, 0, True), (-1, 1, True), (1, 1, False)):
+        c = geom.pack_spinor(chi.to(DEV), 0, cb)
+        p = geom.pack_spinor(psi.to(DEV), 0, 1 - cb)
+        fbuf = torch.zeros(U.numel() + 2, dtype=torch.float64, device=DEV)
+        F = fbuf[offset:offset + U.numel()].view(U.shape)
+        F = qamd.dslash_force(geom, F, c, p, Ud, isign, cb, 1.0, project)
+        ref = orc.force(dims, on_cb(chi.numpy(), dims, cb), on_cb(psi.numpy(), dims, 1 - cb),
+                        U.numpy(), isign, cb, project=project)
+        err = np.abs(F.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < 1e-13, (isign, cb, project, err)
+        # the guard elements around the array are untouched
+        assert fbuf[:offset].count_nonzero() == 0 and fbuf[offset + U.numel():].count_nonzero() == 0
+
+
 def test_adjoint_kernel_vs_oracle(qamd, orc, small):
     dims, U, chi, _ = small
     geom = qamd.Geometry(dims, t_boundary=-1.0)
