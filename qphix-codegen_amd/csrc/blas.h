@@ -55,6 +55,14 @@ void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipSt
                   const int *stop = nullptr);
 // x += alpha p, then (update_p) p = r + beta p: the vector half of a CG step
 // whose r -= alpha Ap and |r|^2 the operator's last kernel did
+// A reliable update's fold: (pending) xs += alpha p, then x += sigma xs
+// (x FP64, xs the FP32/FP64 increment), then xs = 0, in one pass.
+void blas_cg_fold(int prec, int prec_x, const void *p, void *xs, double *x, double sigma, long n,
+                  CGScalars *sc, bool pending, hipStream_t st);
+// The restart after a reliable update: rs = scale * r64 in storage prec,
+// then p = rs + beta p, in one pass.
+void blas_cg_restart(int prec, void *rs, void *p, const double *r64, double scale, long n, CGScalars *sc,
+                     hipStream_t st);
 // *sc = v in stream order, without a host synchronisation
 void blas_set_scalars(CGScalars *sc, const CGScalars &v, hipStream_t st);
 void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long n, CGScalars *sc,

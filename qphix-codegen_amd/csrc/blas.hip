@@ -225,6 +225,75 @@ __global__ __launch_bounds__(kBlasBlock) void cg_xpay_x_k(S *__restrict__ p, SXS
   }
 }
 
+// A reliable update's fold in one pass: (PENDING) the last iteration's
+// xs += alpha p, then x = a xs + b x into the FP64 solution (a = sigma,
+// b = 1), then xs = 0.  The arithmetic is that of cg_xpay_x_k (xs) and
+// axpby_k (x); xs has its arithmetic type as storage (FP32 or FP64), so the
+// updated xs needs no rounding.
+template <typename S, typename SXS, bool PENDING>
+__global__ __launch_bounds__(kBlasBlock) void cg_fold_k(const S *__restrict__ p, SXS *__restrict__ xs,
+                                                        double *__restrict__ x, const CGScalars *sc,
+                                                        double a, double b, int vcb)
+{
+  using R = typename Traits<S>::R;
+  using RX = typename Traits<SXS>::R;
+  static_assert(std::is_same<RX, SXS>::value, "the increment is stored in its arithmetic type");
+  const R alpha = (R)sc->alpha;
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      RX xv[kChunk];
+      ldc<C>(xs, vcb, site, xv);
+      if constexpr (PENDING) {
+        R pv[kChunk];
+        ldc<C>(p, vcb, site, pv);
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) xv[k] = xv[k] + (RX)(alpha * pv[k]);
+      }
+      double yv[kChunk], o[kChunk];
+      ldc<C>(x, vcb, site, yv);
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) o[k] = a * (double)xv[k] + b * yv[k];
+      stc<C>(x, vcb, site, o);
+      RX z[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) z[k] = (RX)0;
+      stc<C>(xs, vcb, site, z);
+    });
+  }
+}
+
+// A restart after a reliable update in one pass: rs = a r64 + b 0 (a =
+// 1 / sigma, b = 0) in storage S, then p = rs + beta p with rs as stored.
+// The arithmetic is that of axpby_k with b = 0 (rs) and cg_xpay_k (p).
+template <typename S>
+__global__ __launch_bounds__(kBlasBlock) void cg_restart_k(S *__restrict__ rs, S *__restrict__ p,
+                                                           const double *__restrict__ r64,
+                                                           const CGScalars *sc, double a, double b, int vcb)
+{
+  using R = typename Traits<S>::R;
+  const R beta = (R)sc->beta;
+  QAMD_SITE_LOOP(site, vcb)
+  {
+    for_chunks([&](auto c) {
+      constexpr int C = decltype(c)::value;
+      double xv[kChunk];
+      ldc<C>(r64, vcb, site, xv);
+      R o[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) o[k] = (R)(a * xv[k] + b * 0.0);
+      stc<C>(rs, vcb, site, o);
+      round_to<S>(o);
+      R pv[kChunk];
+      ldc<C>(p, vcb, site, pv);
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) pv[k] = o[k] + beta * pv[k];
+      stc<C>(p, vcb, site, pv);
+    });
+  }
+}
+
 // First stage of a long sum: work-group w adds partial[w*2048 .. +2048)
 // (strided over its threads, then the fixed work-group tree) into out[w].
 constexpr int kReduceChunk = 8 * kBlasBlock;
@@ -395,6 +464,43 @@ void blas_cg_xpay_x(int prec, int prec_x, void *p, void *x, const void *r, long 
 #undef QAMD_XPX
 #undef QAMD_XPX_X
 #undef QAMD_XPX_P
+}
+
+void blas_cg_fold(int prec, int prec_x, const void *p, void *xs, double *x, double sigma, long n,
+                  CGScalars *sc, bool pending, hipStream_t st)
+{
+  const int ng = (int)(n / 24);  // sites
+  const int grid = blocks_for(ng, kBlasBlock);
+#define QAMD_FOLD_P(T, TX, P)                                                                          \
+  hipLaunchKernelGGL((cg_fold_k<T, TX, P>), dim3(grid), dim3(kBlasBlock), 0, st, cp<T>(p), mp<TX>(xs), x, \
+                     sc, sigma, 1.0, ng)
+#define QAMD_FOLD_X(T, TX)       \
+  if (pending)                   \
+    QAMD_FOLD_P(T, TX, true);    \
+  else                           \
+    QAMD_FOLD_P(T, TX, false);
+#define QAMD_FOLD(T)                  \
+  if (prec_x == QPHIX_AMD_F64) {      \
+    QAMD_FOLD_X(T, double)            \
+  } else {                            \
+    QAMD_FOLD_X(T, float)             \
+  }
+  QAMD_PREC_SWITCH(prec, QAMD_FOLD)
+#undef QAMD_FOLD
+#undef QAMD_FOLD_X
+#undef QAMD_FOLD_P
+}
+
+void blas_cg_restart(int prec, void *rs, void *p, const double *r64, double scale, long n, CGScalars *sc,
+                     hipStream_t st)
+{
+  const int ng = (int)(n / 24);  // sites
+  const int grid = blocks_for(ng, kBlasBlock);
+#define QAMD_RESTART(T)                                                                               \
+  hipLaunchKernelGGL(cg_restart_k<T>, dim3(grid), dim3(kBlasBlock), 0, st, mp<T>(rs), mp<T>(p), r64, sc, \
+                     scale, 0.0, ng)
+  QAMD_PREC_SWITCH(prec, QAMD_RESTART)
+#undef QAMD_RESTART
 }
 
 void blas_cg_xpay(int prec, void *p, const void *r, long n, CGScalars *sc, hipStream_t st, const int *stop)

@@ -550,12 +550,10 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
       rc = QPHIX_AMD_ENOCONV;
       break;
     }
-    // reliable update: fold the increment into x, recompute the true
-    // residual in FP64
-    if (fuse) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, false, st);
-    blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
-    e = hipMemsetAsync(xs.p, 0, n * sbytes(px), st);
-    if (e != hipSuccess) break;
+    // reliable update: fold the increment into x (with the last
+    // iteration's x += alpha p, which the fused step leaves to the next
+    // iteration) and clear it, then recompute the true residual in FP64
+    blas_cg_fold(ps, px, p.p, xs.p, static_cast<double *>(x), sigma, n, sc, fuse, st);
     e = true_residual(&r2);
     if (e != hipSuccess) break;
     updates++;
@@ -566,11 +564,11 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
     const double sig_new = mixed ? std::sqrt(r2 / (double)n) : 1.0;
     // beta from the true residual against the previous iterated one
     const double beta = r2 / (hsc->rr_prev * sigma * sigma);
-    blas_axpby(QPHIX_AMD_F64, r64.p, ps, rs.p, 1.0 / sig_new, 0.0, n, partial, nullptr, st);
     rmax = std::sqrt(r2);
     e = set_scalars(r2 / (sig_new * sig_new), beta * sigma / sig_new, sig_new, rmax);
     if (e != hipSuccess) break;
-    blas_cg_xpay(ps, p.p, rs.p, n, sc, st);
+    // r in the new units, then p = r + beta p
+    blas_cg_restart(ps, rs.p, p.p, static_cast<const double *>(r64.p), 1.0 / sig_new, n, sc, st);
     sigma = sig_new;
     last_len = iters - base;
     base = iters;
@@ -580,8 +578,7 @@ int qphix_amd_cg_solve(qphix_amd_eo_op *op, void *x, const void *b, const qphix_
   if (e == hipSuccess && !converged && rc == QPHIX_AMD_OK) {
     // report the true residual of the final iterate (its last x += alpha p
     // first, when an iteration is pending)
-    if (fuse && !restart) blas_cg_xpay_x(ps, px, p.p, xs.p, rs.p, n, sc, false, st);
-    blas_axpby(px, xs.p, QPHIX_AMD_F64, x, sigma, 1.0, n, partial, nullptr, st);
+    blas_cg_fold(ps, px, p.p, xs.p, static_cast<double *>(x), sigma, n, sc, fuse && !restart, st);
     e = true_residual(&r2);
     converged = r2 < target2;
   }
