@@ -10,7 +10,7 @@ factor, a wrong gamma matrix) fails at least one of them:
 * gamma_5 hermiticity: D_+^dag = D_- = g5 D_+ g5, per operator family;
 * gauge covariance: D[U^g](g psi) = g D[U] psi;
 * twisted BC phase e^{i theta} per hop equals a U(1) rotation of psi;
-* a brute-force dense matrix built with numpy translations on 2^4;
+* a brute-force dense matrix built with numpy translations on 2^4 and 4^4;
 * clover with identity blocks = Wilson; TM with mu = 0 = hop/alpha;
   A A^-1 = 1; (A + i mu g5) B = 1;
 * clover from links: unit gauge -> alpha*1, gauge covariance, and the closed
@@ -177,12 +177,15 @@ def test_twisted_bc_is_u1_rotation(qamd, orc, mu):
     assert rel_err(c[m], orc.dslash(DIMS, psi, U, 1, 0)[m]) == 0.0
 
 
-def test_dense_bruteforce_2x4(qamd, orc):
-    dims = (2, 2, 2, 2)
+@pytest.mark.parametrize("dims", [(2, 2, 2, 2), (4, 4, 4, 4)], ids=["2x4", "4x4"])
+def test_dense_bruteforce(qamd, orc, dims):
+    """The oracle against a dense matrix built site by site with numpy
+    translations: on 2^4 the forward and backward neighbours of a direction
+    are one site, on 4^4 (3072 x 3072) all eight are distinct."""
     L = qamd.lattice
     U = L.random_gauge(dims, 0.5, 3).numpy()
     psi = L.random_spinor(dims, 4).numpy()
-    V = 16
+    V = int(np.prod(dims))
     X = coords(dims)
     Uc = cplx(U)
     Lv = np.array(dims)
@@ -194,7 +197,7 @@ def test_dense_bruteforce_2x4(qamd, orc):
                 for fb in (0, 1):
                     Y = X[x].copy()
                     Y[mu] = (Y[mu] + (1 if fb else -1)) % Lv[mu]
-                    y = ((Y[3] * 2 + Y[2]) * 2 + Y[1]) * 2 + Y[0]
+                    y = ((Y[3] * dims[2] + Y[2]) * dims[1] + Y[1]) * dims[0] + Y[0]
                     if mu < 3:
                         beta = cs
                     elif fb:
