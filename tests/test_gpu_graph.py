@@ -143,6 +143,37 @@ def test_in_place_qphix_arrays_in_graph(qamd):
     _replay_vs_eager(_capture(fn), fn, load, [psi_q.data, res_q.data])
 
 
+def test_split_lattice_faces_in_graph(qamd):
+    """One part of a lattice split in t (FP32): the interior clover dslash,
+    then the packing of a face and the completion of the other from a
+    buffer, in one graph."""
+    print(f"case graph faces F32 {DIMS} split t", flush=True)
+    prec = qamd.F32
+    L = qamd.lattice
+    geom = qamd.Geometry(DIMS, t_boundary=-1.0)
+    geom.set_comm([0, 0, 0, 1])
+    U = L.random_gauge(DIMS, 0.3, 800, device=DEV)
+    clov = L.random_clover(DIMS, 801, device=DEV)
+    g0 = geom.pack_gauge(U, prec, 12, 0)
+    inv0 = geom.pack_clover(L.invert_clover(clov), prec, 0)
+    psi = geom.pack_spinor(L.random_spinor(DIMS, 802, device=DEV), prec, 1)
+    res = geom.new_spinor(prec, cb=0)
+    send = geom.new_face(prec, 3)
+    recv = geom.new_face(prec, 3)
+    op = qamd.ClovDslash(geom)
+
+    def fn():
+        op.dslash(res, psi, g0, inv0, 1, 0)
+        qamd.face_pack(geom, send, psi, 1, 0, 3, 1)
+        qamd.face_finish(geom, 1, res, recv, g0, 1, 0, 3, 0, 1.0, clov=inv0)
+
+    def load(k):
+        psi.data.copy_(geom.pack_spinor(L.random_spinor(DIMS, 810 + k, device=DEV), prec, 1).data)
+        recv.copy_(torch.randn(recv.shape, dtype=recv.dtype, device=DEV, generator=None))
+
+    _replay_vs_eager(_capture(fn), fn, load, [res.data, send])
+
+
 def test_noalloc_clover_force_in_graph(qamd):
     """qphix_amd_clover_force_noalloc (workspace in the library's static
     array) accumulating into a zeroed force field."""
