@@ -489,6 +489,105 @@ def _force(q, op, U, x, csw):
     return F, sorted(times)[len(times) // 2]
 
 
+def shard(q, dev, n=100):
+    """The 32^3x64 clover operator of the headline line split in t over N
+    GPUs (N = 2, 4, 8), on one GPU's share: a local lattice 32^3 x (64/N)
+    whose t wrap-around hops are dropped (qphix_amd_set_comm), FP32, 12-real
+    links.  Times (HIP events around replays of a HIP graph of the calls,
+    so that no host launch time sits between the small face kernels) of the
+    interior clov_dslash (cb 0) and clov_dslash_achimbdpsi (cb 1), and for
+    each of the two the face pack and the face finish of both t faces; the
+    bytes of one face message.  The exchange itself needs a second GPU and
+    is not timed.  critical_path_us: both kernels' interiors and finishes
+    in sequence, the packs and the exchange overlapped with the interiors on
+    a second stream; efficiency: the same two kernels on the whole 32^3x64
+    lattice, divided by N, over that."""
+    import torch
+    L = q.lattice
+    prec = q.F32
+    beta = 0.25
+
+    def fields(dims, seed, comm):
+        geom = q.Geometry(dims, t_boundary=-1.0)
+        if comm:
+            geom.set_comm([0, 0, 0, 1])
+        U = L.random_gauge(dims, 0.3, seed, device=dev)
+        clov = L.random_clover(dims, seed + 1, device=dev)
+        f = dict(g0=geom.pack_gauge(U, prec, 12, 0), g1=geom.pack_gauge(U, prec, 12, 1),
+                 inv0=geom.pack_clover(L.invert_clover(clov), prec, 0), clov1=geom.pack_clover(clov, prec, 1),
+                 psi1=geom.pack_spinor(L.random_spinor(dims, seed + 2, device=dev), prec, 1),
+                 t0=geom.new_spinor(prec, cb=0), out1=geom.new_spinor(prec, cb=1))
+        del U, clov
+        return geom, f
+
+    def timed(fn, reps=50):
+        """us per fn(), from replays of a HIP graph of reps calls (no host
+        launch time between the small face kernels)."""
+        st = torch.cuda.Stream()
+        st.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(st):
+            fn(0)
+        torch.cuda.current_stream().wait_stream(st)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for i in range(reps):
+                fn(i)
+        g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n // 10):
+            g.replay()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / (reps * (n // 10))
+
+    geom, f = fields(DIMS, 1500, False)
+    op = q.ClovDslash(geom)
+    whole = timed(lambda i: (op.dslash(f["t0"], f["psi1"], f["g0"], f["inv0"], 1, 0),
+                             op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"],
+                                                     beta, 1, 1)))
+    del geom, f, op
+    torch.cuda.empty_cache()
+    out = {"workload": "32^3x64 clover operator (clov_dslash + clov_dslash_achimbdpsi, FP32, 12-real links) "
+                       "split in t, one GPU's share",
+           "whole_lattice_step_us": round(whole, 2), "parts": {}}
+    for N in (2, 4, 8):
+        dims = (DIMS[0], DIMS[1], DIMS[2], DIMS[3] // N)
+        geom, f = fields(dims, 1510 + N, True)
+        op = q.ClovDslash(geom)
+        buf = {fb: geom.new_face(prec, 3) for fb in (0, 1)}
+        t_d = timed(lambda i: op.dslash(f["t0"], f["psi1"], f["g0"], f["inv0"], 1, 0))
+        t_a = timed(lambda i: op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"],
+                                                      beta, 1, 1))
+        # the first kernel's faces: half-spinors of psi (cb 1), finished into t (cb 0) through
+        # the clover epilogue; the second's: of t, finished into out (cb 1) with +coeff
+        p_d = timed(lambda i: [q.face_pack(geom, buf[fb], f["psi1"], 1, 0, 3, fb) for fb in (0, 1)])
+        f_d = timed(lambda i: [q.face_finish(geom, 1, f["t0"], buf[fb], f["g0"], 1, 0, 3, fb, 1.0,
+                                             clov=f["inv0"]) for fb in (0, 1)])
+        p_a = timed(lambda i: [q.face_pack(geom, buf[fb], f["t0"], 1, 1, 3, fb) for fb in (0, 1)])
+        f_a = timed(lambda i: [q.face_finish(geom, 0, f["out1"], buf[fb], f["g1"], 1, 1, 3, fb, beta)
+                               for fb in (0, 1)])
+        face_bytes = geom.face_sites(3) * 12 * BYTES_F32
+        # each kernel's faces packed on a second stream while its interior runs, then
+        # exchanged; its two finishes follow the interior on the critical path
+        crit = t_d + f_d + t_a + f_a
+        out["parts"][str(N)] = {
+            "local_lattice": list(dims), "sites_per_cb": geom.vcb,
+            "interior_us": {"clov_dslash": round(t_d, 2), "clov_dslash_achimbdpsi": round(t_a, 2)},
+            "face_pack_us_two_faces": {"clov_dslash": round(p_d, 2), "clov_dslash_achimbdpsi": round(p_a, 2)},
+            "face_finish_us_two_faces": {"clov_dslash": round(f_d, 2), "clov_dslash_achimbdpsi": round(f_a, 2)},
+            "face_message_bytes": face_bytes,
+            "critical_path_us": round(crit, 2),
+            "whole_lattice_step_over_n_us": round(whole / N, 2),
+            "efficiency_if_exchange_hidden": round(whole / N / crit, 3),
+        }
+        del geom, f, op, buf
+        torch.cuda.empty_cache()
+    return out
+
+
 def cfg4(q, dev, cg=True):
     """The 48^3x96 (5,308,416 sites/cb) twisted-mass clover workload, mu = 0.05,
     kappa = 0.1355, csw = 1.5 (clover from the links), links sigma = 0.2.
@@ -591,9 +690,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG solves (32^3x64 and 48^3x96)")
     ap.add_argument("--no-cfg4", action="store_true", help="skip the 48^3x96 twisted-mass clover block")
-    ap.add_argument("--workload", default="clover32", choices=["clover32", "tmclover48"],
+    ap.add_argument("--workload", default="clover32", choices=["clover32", "tmclover48", "shard"],
                     help="clover32: the 32^3x64 clover operator (the headline line, with the "
-                         "48^3x96 block); tmclover48: the 48^3x96 twisted-mass clover block alone")
+                         "48^3x96 block); tmclover48: the 48^3x96 twisted-mass clover block alone; "
+                         "shard: one GPU's share of the 32^3x64 operator split in t over 2, 4, 8 GPUs")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch clov_dslash, clov_dslash_achimbdpsi, Wilson dslash and clov_dslash "
                          "on QPhiX arrays in place (soalen 16, 8, 4) 10 times each plus a 1 GiB device copy (for "
@@ -625,6 +725,9 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == "tmclover48":
         print(json.dumps(cfg4(q, dev, cg=not args.no_cg)), flush=True)
+        return
+    if args.workload == "shard":
+        print(json.dumps(shard(q, dev)), flush=True)
         return
     prec, recon = q.F32, 12
     dims = tuple(int(v) for v in args.lattice.split(",")) if args.lattice else DIMS

@@ -23,7 +23,10 @@
 namespace qamd {
 namespace {
 
-constexpr int kFaceBlock = 256;
+// One wave per work-group: a face is small (16,384 sites for a 32^3 t-face),
+// and its sites' chains of dependent loads are latency-bound, so it is spread
+// over as many CUs as possible (256 work-groups for that face, one per CU).
+constexpr int kFaceBlock = 64;
 
 template <typename S> constexpr int half_w() { return pick_w<S>(12); }
 
