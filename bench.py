@@ -15,8 +15,10 @@ the dominant kernel from HIP-event timing, every reduced-precision kernel's
 error against the FP64 kernel, "cg" = FP64 vs mixed-precision CG time to a
 true residual of 1e-10 (the 32^3x64 Wilson-clover solve), "cfg4" = the
 48^3x96 twisted-mass clover case (dslash throughput and error, FP64 vs
-FP16-mixed CG, the pseudofermion force and its error), and the CPU baseline
-(the reference's own AVX-512/AVX2 kernels on the host cores).
+FP16-mixed CG, the pseudofermion force and its error), "shard" = one GPU's
+share of the 32^3x64 operator split in t over 2, 4 and 8 GPUs (interior and
+face kernels, critical path), and the CPU baseline (the reference's own
+AVX-512/AVX2 kernels on the host cores).
 """
 import argparse
 import json
@@ -690,6 +692,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG solves (32^3x64 and 48^3x96)")
     ap.add_argument("--no-cfg4", action="store_true", help="skip the 48^3x96 twisted-mass clover block")
+    ap.add_argument("--no-shard", action="store_true", help="skip the split-lattice (one GPU's share) block")
     ap.add_argument("--workload", default="clover32", choices=["clover32", "tmclover48", "shard"],
                     help="clover32: the 32^3x64 clover operator (the headline line, with the "
                          "48^3x96 block); tmclover48: the 48^3x96 twisted-mass clover block alone; "
@@ -864,6 +867,8 @@ def main():
         line["cg"]["sloppy_kernels"] = sloppy_kernels(q, dev)
     if not args.no_cfg4:
         line["cfg4"] = cfg4(q, dev, cg=not args.no_cg)
+    if not args.no_shard:
+        line["shard"] = shard(q, dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
