@@ -498,8 +498,8 @@ def shard(q, dev, n=100):
     links.  Times (HIP events around replays of a HIP graph of the calls,
     so that no host launch time sits between the small face kernels) of the
     interior clov_dslash (cb 0) and clov_dslash_achimbdpsi (cb 1), and for
-    each of the two the face pack and the face finish of both t faces; the
-    bytes of one face message.  The exchange itself needs a second GPU and
+    each of the two the face pack and the face finish of both t faces (one
+    launch each, fb = 2); the bytes of one face message.  The exchange itself needs a second GPU and
     is not timed.  critical_path_us: both kernels' interiors and finishes
     in sequence, the packs and the exchange overlapped with the interiors on
     a second stream; efficiency: the same two kernels on the whole 32^3x64
@@ -559,18 +559,16 @@ def shard(q, dev, n=100):
         dims = (DIMS[0], DIMS[1], DIMS[2], DIMS[3] // N)
         geom, f = fields(dims, 1510 + N, True)
         op = q.ClovDslash(geom)
-        buf = {fb: geom.new_face(prec, 3) for fb in (0, 1)}
+        buf = geom.new_face(prec, 3, both=True)  # both t faces, packed and finished in one launch each
         t_d = timed(lambda i: op.dslash(f["t0"], f["psi1"], f["g0"], f["inv0"], 1, 0))
         t_a = timed(lambda i: op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"],
                                                       beta, 1, 1))
         # the first kernel's faces: half-spinors of psi (cb 1), finished into t (cb 0) through
         # the clover epilogue; the second's: of t, finished into out (cb 1) with +coeff
-        p_d = timed(lambda i: [q.face_pack(geom, buf[fb], f["psi1"], 1, 0, 3, fb) for fb in (0, 1)])
-        f_d = timed(lambda i: [q.face_finish(geom, 1, f["t0"], buf[fb], f["g0"], 1, 0, 3, fb, 1.0,
-                                             clov=f["inv0"]) for fb in (0, 1)])
-        p_a = timed(lambda i: [q.face_pack(geom, buf[fb], f["t0"], 1, 1, 3, fb) for fb in (0, 1)])
-        f_a = timed(lambda i: [q.face_finish(geom, 0, f["out1"], buf[fb], f["g1"], 1, 1, 3, fb, beta)
-                               for fb in (0, 1)])
+        p_d = timed(lambda i: q.face_pack(geom, buf, f["psi1"], 1, 0, 3, 2))
+        f_d = timed(lambda i: q.face_finish(geom, 1, f["t0"], buf, f["g0"], 1, 0, 3, 2, 1.0, clov=f["inv0"]))
+        p_a = timed(lambda i: q.face_pack(geom, buf, f["t0"], 1, 1, 3, 2))
+        f_a = timed(lambda i: q.face_finish(geom, 0, f["out1"], buf, f["g1"], 1, 1, 3, 2, beta))
         face_bytes = geom.face_sites(3) * 12 * BYTES_F32
         # each kernel's faces packed on a second stream while its interior runs, then
         # exchanged; its two finishes follow the interior on the critical path

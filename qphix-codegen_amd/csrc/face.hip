@@ -133,14 +133,28 @@ __device__ __forceinline__ void reconstruct_rt(int mu, int sgn, cx<R> acc[4][3],
   }
 }
 
-// buf[f] = (1 + sgn gamma_mu) in(site f), in on checkerboard cb_in.
+// buf[f] = (1 + sgn gamma_mu) in(site f), in on checkerboard cb_in; side 2:
+// both faces in one launch, face 0 (sgn = -isign) then face 1 (sgn = isign),
+// each in its own nface-site buffer, one after the other.
 template <typename S>
 __global__ __launch_bounds__(kFaceBlock) void face_pack_k(S *buf, const S *in, Geom g, FaceSpinorGeom q, int cb_in,
-                                                          int mu, int side, int sgn, int nface)
+                                                          int mu, int side, int isign, int nface)
 {
   using R = typename Traits<S>::R;
-  const int f = blockIdx.x * kFaceBlock + threadIdx.x;
-  if (f >= nface) return;
+  int f = blockIdx.x * kFaceBlock + threadIdx.x;
+  if (side == 2) {
+    if (f >= 2 * nface) return;
+    side = f >= nface;
+    if (side) {
+      f -= nface;
+      buf += (size_t)12 * nface;
+    }
+  } else if (f >= nface) {
+    return;
+  }
+  // sent backwards: the receiver's forward hop, projector (1 - isign gamma);
+  // sent forwards: its backward hop, (1 + isign gamma) (codegen/dslash.cc:451-457)
+  const int sgn = side == 0 ? -isign : isign;
   const SiteCoord c = face_site(g, mu, side, f, cb_in);
   R psi[24];
   load_site(in, g, q, c, psi);
@@ -180,9 +194,19 @@ __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf
                                                             FaceArgs<typename Traits<S>::R> a)
 {
   using R = typename Traits<S>::R;
-  const int f = blockIdx.x * kFaceBlock + threadIdx.x;
-  if (f >= a.nface) return;
-  const SiteCoord c = face_site(g, a.mu, a.side, f, a.cb);
+  int f = blockIdx.x * kFaceBlock + threadIdx.x;
+  int side = a.side;
+  if (side == 2) {  // both faces: face 0's buffer, then face 1's
+    if (f >= 2 * a.nface) return;
+    side = f >= a.nface;
+    if (side) {
+      f -= a.nface;
+      buf += (size_t)12 * a.nface;
+    }
+  } else if (f >= a.nface) {
+    return;
+  }
+  const SiteCoord c = face_site(g, a.mu, side, f, a.cb);
   const int idx = native_index(g, c);
   const int vcb = g.vcb;
   R hr[12];
@@ -200,22 +224,22 @@ __global__ __launch_bounds__(kFaceBlock) void face_finish_k(S *out, const S *buf
     case 2: load_link_pair<S, RECON, 2>(gauge, vcb, idx, ub, uf, nz); break;
     default: load_link_pair<S, RECON, 3>(gauge, vcb, idx, ub, uf, nz); break;
   }
-  const cx<R> ph = {a.ph_re, a.side ? a.ph_im : -a.ph_im};
+  const cx<R> ph = {a.ph_re, side ? a.ph_im : -a.ph_im};
   cx<R> w[2][3];
 #pragma unroll
   for (int k = 0; k < 2; k++)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      const cx<R> u0 = a.side ? uf[i][0] : ub[i][0];
-      const cx<R> u1 = a.side ? uf[i][1] : ub[i][1];
-      const cx<R> u2 = a.side ? uf[i][2] : ub[i][2];
+      const cx<R> u0 = side ? uf[i][0] : ub[i][0];
+      const cx<R> u1 = side ? uf[i][1] : ub[i][1];
+      const cx<R> u2 = side ? uf[i][2] : ub[i][2];
       if constexpr (std::is_same<S, __half>::value)
         w[k][i] = cdot3(u0, u1, u2, h[k][0], h[k][1], h[k][2], nz);
       else
         w[k][i] = cdot3(u0, u1, u2, h[k][0], h[k][1], h[k][2]);
       if (a.tbc) w[k][i] = cmul_ord(ph, w[k][i]);
     }
-  const int sgn = a.side ? -a.isign : a.isign;
+  const int sgn = side ? -a.isign : a.isign;
   R o[24];
   load_site(out, g, q, c, o);
   cx<R> acc[4][3];
@@ -286,7 +310,7 @@ template <typename S, int OP>
 hipError_t launch_finish(int recon, const Geom &g, const FaceSpinorGeom &q, S *out, const S *buf, const S *gauge,
                          const S *clov, const FaceArgs<typename Traits<S>::R> &a, hipStream_t st)
 {
-  const dim3 grid(blocks_for(a.nface, kFaceBlock)), block(kFaceBlock);
+  const dim3 grid(blocks_for(a.side == 2 ? 2L * a.nface : a.nface, kFaceBlock)), block(kFaceBlock);
   if (recon == 12)
     hipLaunchKernelGGL((face_finish_k<S, 12, OP>), grid, block, 0, st, out, buf, gauge, clov, g, q, a);
   else
@@ -319,7 +343,7 @@ int finish(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int op, int pr
 {
   if (!ctx || prec < QPHIX_AMD_F64 || prec > QPHIX_AMD_BF16 || (recon != 12 && recon != 18))
     return QPHIX_AMD_EINVAL;
-  if ((isign != 1 && isign != -1) || (cb != 0 && cb != 1) || dim < 0 || dim > 3 || (fb != 0 && fb != 1))
+  if ((isign != 1 && isign != -1) || (cb != 0 && cb != 1) || dim < 0 || dim > 3 || fb < 0 || fb > 2)
     return QPHIX_AMD_EINVAL;
   if (!out || !buf || !gauge || out == buf) return QPHIX_AMD_EINVAL;
   if ((op == OP_CLOVER || op == OP_TMCLOVER) && !clov) return QPHIX_AMD_EINVAL;
@@ -394,7 +418,7 @@ int pack(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int prec, int is
          void *buf, const void *in, void *stream)
 {
   if (!ctx || prec < QPHIX_AMD_F64 || prec > QPHIX_AMD_BF16) return QPHIX_AMD_EINVAL;
-  if ((isign != 1 && isign != -1) || (cb != 0 && cb != 1) || dim < 0 || dim > 3 || (fb != 0 && fb != 1))
+  if ((isign != 1 && isign != -1) || (cb != 0 && cb != 1) || dim < 0 || dim > 3 || fb < 0 || fb > 2)
     return QPHIX_AMD_EINVAL;
   if (!buf || !in || buf == in) return QPHIX_AMD_EINVAL;
   FaceSpinorGeom q = kNativeSpinor;
@@ -403,14 +427,12 @@ int pack(qphix_amd_ctx *ctx, const qphix_amd_qphix_layout *lay, int prec, int is
   if (!aligned16(buf) || !in_ok) return QPHIX_AMD_EALIGN;
   const Geom &g = ctx->g;
   const int nface = qamd::face_count(g, dim);
-  // sent backwards: the receiver's forward hop, projector (1 - isign gamma);
-  // sent forwards: its backward hop, (1 + isign gamma) (codegen/dslash.cc:451-457)
-  const int sgn = fb == 0 ? -isign : isign;
+  const long nthreads = fb == 2 ? 2L * nface : nface;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return status(with_prec(prec, [&](auto *tag) -> hipError_t {
     using S = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(face_pack_k<S>, dim3(blocks_for(nface, kFaceBlock)), dim3(kFaceBlock), 0, st,
-                       static_cast<S *>(buf), static_cast<const S *>(in), g, q, 1 - cb, dim, fb, sgn, nface);
+    hipLaunchKernelGGL(face_pack_k<S>, dim3(blocks_for(nthreads, kFaceBlock)), dim3(kFaceBlock), 0, st,
+                       static_cast<S *>(buf), static_cast<const S *>(in), g, q, 1 - cb, dim, fb, isign, nface);
     return hipGetLastError();
   }));
 }

@@ -109,8 +109,10 @@ class Geometry:
         """The traversal order the operator calls on this geometry use."""
         return int(lib().qphix_amd_traversal(self._ctx))
 
-    def new_face(self, prec, dim, device="cuda"):
-        return torch.empty(self.face_sites(dim) * 12, dtype=TORCH_DTYPE[prec], device=device)
+    def new_face(self, prec, dim, device="cuda", both=False):
+        """A face buffer of dim (both: the two faces, for fb = 2)."""
+        n = self.face_sites(dim) * 12 * (2 if both else 1)
+        return torch.empty(n, dtype=TORCH_DTYPE[prec], device=device)
 
     # ---- allocation ---------------------------------------------------
     def field_bytes(self, kind, prec, recon=18):
@@ -240,9 +242,10 @@ def _qphix_spinor_ok(geom, f):
 
 def face_pack(geom, buf, psi, isign, cb, dim, fb):
     """buf <- projected half-spinors of psi (checkerboard 1-cb) on the slice
-    x_dim = 0 (fb 0, sent backwards) or L_dim - 1 (fb 1, sent forwards); psi
-    a native spinor or a QPhiXSpinor (read in place)."""
-    if buf.dtype != TORCH_DTYPE[psi.prec] or buf.numel() < geom.face_sites(dim) * 12:
+    x_dim = 0 (fb 0, sent backwards) or L_dim - 1 (fb 1, sent forwards), or
+    both in one launch (fb 2, buf holding face 0 then face 1); psi a native
+    spinor or a QPhiXSpinor (read in place)."""
+    if buf.dtype != TORCH_DTYPE[psi.prec] or buf.numel() < geom.face_sites(dim) * 12 * (2 if fb == 2 else 1):
         raise QPhiXAMDError("face buffer of the wrong type or size")
     if isinstance(psi, QPhiXSpinor):
         _qphix_spinor_ok(geom, psi)
@@ -256,10 +259,11 @@ def face_pack(geom, buf, psi, isign, cb, dim, fb):
 
 def face_finish(geom, kind, out, buf, u, isign, cb, dim, fb, beta, clov=None, mu=0.0, muinv=0.0):
     """Complete the boundary sites of out (checkerboard cb) from a received
-    face; kind 0 Wilson (out -= r), 1 clover (out += clov r), 2 twisted mass
+    face (fb 0 or 1; fb 2: both in one launch, buf holding face 0 then face
+    1); kind 0 Wilson (out -= r), 1 clover (out += clov r), 2 twisted mass
     (out -= muinv (1 - isign i mu g5) r), 3 twisted-mass clover (out += clov r)
     with r the reconstructed beta * U h."""
-    if buf.dtype != TORCH_DTYPE[out.prec] or buf.numel() < geom.face_sites(dim) * 12:
+    if buf.dtype != TORCH_DTYPE[out.prec] or buf.numel() < geom.face_sites(dim) * 12 * (2 if fb == 2 else 1):
         raise QPhiXAMDError("face buffer of the wrong type or size")
     if u.prec != out.prec or (clov is not None and clov.prec != out.prec):
         raise QPhiXAMDError("out, u and clov must share a storage precision")

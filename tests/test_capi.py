@@ -94,7 +94,9 @@ def test_face_entry_points_validate(qamd):
         p = ctypes.c_void_p((ctypes.addressof(buf) + 15) & ~15)
         q = ctypes.c_void_p(p.value + 64)
         assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 4, 0, p, q, None) == -1   # dim
-        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 2, p, q, None) == -1   # fb
+        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 3, p, q, None) == -1   # fb (2: both faces)
+        assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, -1, p, q, None) == -1
+        assert lib.qphix_amd_face_finish(ctx, 1, 12, 1, 0, 0, 3, p, q, q, 1.0, None) == -1
         assert lib.qphix_amd_face_pack(ctx, 1, 0, 0, 0, 0, p, q, None) == -1   # isign
         assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 0, p, p, None) == -1   # buf == in
         assert lib.qphix_amd_face_pack(ctx, 1, 1, 0, 0, 0, ctypes.c_void_p(p.value + 4), q, None) == -6

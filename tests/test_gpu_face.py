@@ -302,3 +302,50 @@ def test_split_lattice_in_place_on_qphix_arrays(qamd, split, ft, qv):
             got = split_apply(qamd, lat, G, grid, kind, achi, isign, cb, prec, recon, qv=qv)
             m = cb_mask(G, cb)
             assert np.array_equal(got[m], want[m]), (split, kind, achi, isign, cb)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3], ids=["f64", "f32", "f16", "bf16"])
+def test_both_faces_in_one_launch(qamd, prec):
+    """fb = 2 packs and finishes both faces of a dimension in one launch, the
+    buffer holding face 0 then face 1: the same bits as the fb = 0 and fb = 1
+    calls, for every dimension and every finish epilogue (and, in FP32, on
+    QPhiX arrays in place)."""
+    dims = (8, 4, 6, 4)
+    print(f"case both faces prec={prec} {dims}", flush=True)
+    L = qamd.lattice
+    geom = qamd.Geometry(dims, t_boundary=-1.0)
+    geom.set_comm([1, 1, 1, 1])
+    U = L.random_gauge(dims, 0.3, 900, device=DEV)
+    clov = L.random_clover(dims, 901, device=DEV)
+    g0 = geom.pack_gauge(U, prec, 12, 0)
+    cl = geom.pack_clover(clov, prec, 0)
+    fcl = geom.pack_full_clover(L.tmclover_blocks(clov, 0.1, 1, True), prec, 0)
+    psi = geom.pack_spinor(L.random_spinor(dims, 902, device=DEV), prec, 1)
+    base = geom.pack_spinor(L.random_spinor(dims, 903, device=DEV), prec, 0)
+    variants = [(psi, base, lambda t: t)]
+    if prec == qamd.F32:
+        lay = qamd.qphix_layout(prec, 4, 4, True)
+        variants.append((qamd.QPhiXSpinor(geom.spinor_to_qphix(psi, lay), lay),
+                         qamd.QPhiXSpinor(geom.spinor_to_qphix(base, lay), lay),
+                         lambda t: qamd.QPhiXSpinor(t, lay)))
+    for src, out0, wrap in variants:
+        for dim in range(4):
+            n = geom.face_sites(dim) * 12
+            both = geom.new_face(prec, dim, both=True)
+            qamd.face_pack(geom, both, src, 1, 0, dim, 2)
+            for fb in (0, 1):
+                one = geom.new_face(prec, dim)
+                qamd.face_pack(geom, one, src, 1, 0, dim, fb)
+                assert torch.equal(one, both[fb * n:(fb + 1) * n]), (dim, fb)
+            recv = torch.randn(2 * n, device=DEV).to(both.dtype)
+            for kind, extra in ((0, {}), (1, dict(clov=cl)), (2, dict(mu=0.1, muinv=0.9)), (3, dict(clov=fcl))):
+                ref, got = wrap(out0.data.clone()), wrap(out0.data.clone())
+                if isinstance(out0, qamd.Field):
+                    ref = qamd.Field(ref, prec, out0.kind, cb=0)
+                    got = qamd.Field(got, prec, out0.kind, cb=0)
+                for fb in (0, 1):
+                    qamd.face_finish(geom, kind, ref, recv[fb * n:(fb + 1) * n], g0, 1, 0, dim, fb, 0.5, **extra)
+                qamd.face_finish(geom, kind, got, recv, g0, 1, 0, dim, 2, 0.5, **extra)
+                torch.cuda.synchronize()
+                assert torch.equal(ref.data, got.data), (dim, kind)
+                assert not torch.equal(got.data, out0.data), (dim, kind)
