@@ -422,7 +422,8 @@ int qphix_amd_ndtm_clov_dslash_achimbdpsi(qphix_amd_ctx *ctx, int prec,
    packs or finishes both faces of dim in one launch, with buf holding the
    fb 0 face and then the fb 1 face (2 * sites * 12 reals): each face is a
    small launch bound by its latency, so one launch for both halves it on
-   a host's critical path.  Operators built on a context
+   a host's critical path.  Both faces then take one beta, so a GPU at the
+   global time boundary finishes its t faces one by one.  Operators built on a context
    (qphix_amd_eo_op_create, qphix_amd_dslash_force) need comm all zero.   */
 int qphix_amd_set_comm(qphix_amd_ctx *ctx, const int comm[4]);
 long qphix_amd_face_sites(const qphix_amd_ctx *ctx, int dim);
