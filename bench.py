@@ -502,8 +502,11 @@ def shard(q, dev, n=100):
     launch each, fb = 2); the bytes of one face message.  The exchange itself needs a second GPU and
     is not timed.  critical_path_us: both kernels' interiors and finishes
     in sequence, the packs and the exchange overlapped with the interiors on
-    a second stream; efficiency: the same two kernels on the whole 32^3x64
-    lattice, divided by N, over that."""
+    a second stream; two_stream_step_us: that step measured with the packs
+    on a second stream joined before the finishes, one_stream_step_us: the
+    six launches in order on one stream (no exchange in either);
+    efficiency: the same two kernels on the whole 32^3x64 lattice, divided
+    by N, over the critical path and over the one-stream step."""
     import torch
     L = q.lattice
     prec = q.F32
@@ -569,6 +572,38 @@ def shard(q, dev, n=100):
         f_d = timed(lambda i: q.face_finish(geom, 1, f["t0"], buf, f["g0"], 1, 0, 3, 2, 1.0, clov=f["inv0"]))
         p_a = timed(lambda i: q.face_pack(geom, buf, f["t0"], 1, 1, 3, 2))
         f_a = timed(lambda i: q.face_finish(geom, 0, f["out1"], buf, f["g1"], 1, 1, 3, 2, beta))
+        # the whole step with the design's overlap, on two streams: each kernel's
+        # faces packed on a second stream while its interior runs, joined before
+        # its finishes (the second kernel's packs read t after the first's
+        # finishes); the exchange itself is absent
+        side = torch.cuda.Stream()
+
+        def overlapped(i):
+            cur = torch.cuda.current_stream()
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                q.face_pack(geom, buf, f["psi1"], 1, 0, 3, 2)
+            op.dslash(f["t0"], f["psi1"], f["g0"], f["inv0"], 1, 0)
+            cur.wait_stream(side)
+            q.face_finish(geom, 1, f["t0"], buf, f["g0"], 1, 0, 3, 2, 1.0, clov=f["inv0"])
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                q.face_pack(geom, buf2, f["t0"], 1, 1, 3, 2)
+            op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"], beta, 1, 1)
+            cur.wait_stream(side)
+            q.face_finish(geom, 0, f["out1"], buf2, f["g1"], 1, 1, 3, 2, beta)
+
+        def sequential(i):
+            q.face_pack(geom, buf, f["psi1"], 1, 0, 3, 2)
+            op.dslash(f["t0"], f["psi1"], f["g0"], f["inv0"], 1, 0)
+            q.face_finish(geom, 1, f["t0"], buf, f["g0"], 1, 0, 3, 2, 1.0, clov=f["inv0"])
+            q.face_pack(geom, buf2, f["t0"], 1, 1, 3, 2)
+            op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"], beta, 1, 1)
+            q.face_finish(geom, 0, f["out1"], buf2, f["g1"], 1, 1, 3, 2, beta)
+
+        buf2 = geom.new_face(prec, 3, both=True)
+        t_ov = timed(overlapped)
+        t_seq = timed(sequential)
         face_bytes = geom.face_sites(3) * 12 * BYTES_F32
         # each kernel's faces packed on a second stream while its interior runs, then
         # exchanged; its two finishes follow the interior on the critical path
@@ -580,10 +615,13 @@ def shard(q, dev, n=100):
             "face_finish_us_two_faces": {"clov_dslash": round(f_d, 2), "clov_dslash_achimbdpsi": round(f_a, 2)},
             "face_message_bytes": face_bytes,
             "critical_path_us": round(crit, 2),
+            "two_stream_step_us": round(t_ov, 2),
+            "one_stream_step_us": round(t_seq, 2),
             "whole_lattice_step_over_n_us": round(whole / N, 2),
             "efficiency_if_exchange_hidden": round(whole / N / crit, 3),
+            "efficiency_one_stream_measured": round(whole / N / t_seq, 3),
         }
-        del geom, f, op, buf
+        del geom, f, op, buf, buf2, side
         torch.cuda.empty_cache()
     return out
 
