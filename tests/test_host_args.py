@@ -112,3 +112,19 @@ def test_dslash_force_arguments(qamd, geom):
         autograd.dslash_force(geom, u, chi64, psi64, u, 1, 1)
     with pytest.raises(qamd.QPhiXAMDError, match="force"):
         autograd.dslash_force(geom, u[:, :2], chi64, psi64, u, 1, 0)
+
+
+def test_face_buffers_sized_for_both_faces(qamd, geom):
+    """fb = 2 (both faces in one launch) needs a buffer of two faces; a
+    one-face buffer is refused before the call (host tensors here: the size
+    check comes first)."""
+    psi = _spinor(qamd, geom, qamd.F32, cb=1)
+    one = torch.zeros(geom.face_sites(3) * 12, dtype=torch.float32)
+    with pytest.raises(qamd.QPhiXAMDError, match="size"):
+        qamd.face_pack(geom, one, psi, 1, 0, 3, 2)
+    res = _spinor(qamd, geom, qamd.F32, cb=0)
+    g = qamd.Field(torch.zeros(8), qamd.F32, qamd.FIELD_GAUGE, recon=12, cb=0)
+    with pytest.raises(qamd.QPhiXAMDError, match="size"):
+        qamd.face_finish(geom, 0, res, one, g, 1, 0, 3, 2, 1.0)
+    two = torch.zeros(2 * geom.face_sites(3) * 12, dtype=torch.float32)
+    assert two.numel() == geom.new_face(qamd.F32, 3, device="cpu", both=True).numel()
