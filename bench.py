@@ -499,12 +499,13 @@ def shard(q, dev, n=100):
     so that no host launch time sits between the small face kernels) of the
     interior clov_dslash (cb 0) and clov_dslash_achimbdpsi (cb 1), and for
     each of the two the face pack and the face finish of both t faces (one
-    launch each, fb = 2); the bytes of one face message.  The exchange itself needs a second GPU and
-    is not timed.  critical_path_us: both kernels' interiors and finishes
-    in sequence, the packs and the exchange overlapped with the interiors on
-    a second stream; two_stream_step_us: that step measured with the packs
-    on a second stream joined before the finishes, one_stream_step_us: the
-    six launches in order on one stream (no exchange in either);
+    launch each, fb = 2); the bytes of one face message.  The exchange
+    itself needs a second GPU and is not timed.  critical_path_us: both
+    kernels' interiors and finishes in sequence, the packs and the exchange
+    overlapped with the interiors; two_stream_step_us: the step measured
+    with each kernel's packs on a second stream, joined before its finishes;
+    one_stream_step_us: the six launches in order on one stream (no
+    exchange in either);
     efficiency: the same two kernels on the whole 32^3x64 lattice, divided
     by N, over the critical path and over the one-stream step."""
     import torch
@@ -562,7 +563,8 @@ def shard(q, dev, n=100):
         dims = (DIMS[0], DIMS[1], DIMS[2], DIMS[3] // N)
         geom, f = fields(dims, 1510 + N, True)
         op = q.ClovDslash(geom)
-        buf = geom.new_face(prec, 3, both=True)  # both t faces, packed and finished in one launch each
+        # both t faces, packed and finished in one launch each (one buffer per kernel)
+        buf, buf2 = geom.new_face(prec, 3, both=True), geom.new_face(prec, 3, both=True)
         t_d = timed(lambda i: op.dslash(f["t0"], f["psi1"], f["g0"], f["inv0"], 1, 0))
         t_a = timed(lambda i: op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"],
                                                       beta, 1, 1))
@@ -601,12 +603,11 @@ def shard(q, dev, n=100):
             op.dslashAChiMinusBDPsi(f["out1"], f["t0"], f["psi1"], f["g1"], f["clov1"], beta, 1, 1)
             q.face_finish(geom, 0, f["out1"], buf2, f["g1"], 1, 1, 3, 2, beta)
 
-        buf2 = geom.new_face(prec, 3, both=True)
         t_ov = timed(overlapped)
         t_seq = timed(sequential)
         face_bytes = geom.face_sites(3) * 12 * BYTES_F32
-        # each kernel's faces packed on a second stream while its interior runs, then
-        # exchanged; its two finishes follow the interior on the critical path
+        # the packs and the exchange overlapped with the interiors: each kernel's
+        # interior, then its finishes
         crit = t_d + f_d + t_a + f_a
         out["parts"][str(N)] = {
             "local_lattice": list(dims), "sites_per_cb": geom.vcb,
